@@ -1,0 +1,158 @@
+/*
+ * mega.h -- C-ABI of the MI355X explicit meshfree step (libmega_hip.so).
+ *
+ * The reference program drives one explicit step as
+ *   MAIN (predictor, MAIN.F90:348-356)
+ *   -> HANDELER(DO_INTERP)            HANDELER.F90:114-139
+ *   -> coordinate update              MAIN.F90:377-386
+ *   -> HANDELER -> [GET_BINS/SOFT_SEARCH] -> CONSTRUCT_FINT -> CONSTITUTION
+ *                                     HANDELER.F90:145-307, FINT.F90:16, CONSTITUTION.F90:18
+ *   -> EBC / corrector                MAIN.F90:432-488
+ * and passes everything by reference in column-major, 1-based layouts.  This
+ * header exposes the same stages as plain C functions over plain pointers:
+ * all arrays are in the reference layout (node vectors interleaved 3*i+k,
+ * tensors Voigt XX,YY,ZZ,YZ,XZ,XY, LOGICAL as int32 0/1), every call returns
+ * an int status (MEGA_OK or an error below), and a context owns all device
+ * memory.  A Fortran host binds these with bind(C) / iso_c_binding (see the
+ * interface block shipped beside this header).
+ *
+ * Threading: one host thread per context; all work is issued on the
+ * context's own HIP stream; functions that return host data synchronise.
+ */
+#ifndef MEGA_H
+#define MEGA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference has none: EXIT_PROGRAM / WARN / STOP) */
+enum {
+    MEGA_OK = 0,
+    MEGA_ERR_RETURN_MAP = 1,   /* J2 Newton > 20 iterations       CONSTITUTE_VONMISES.F90:115-119 */
+    MEGA_ERR_SINGULAR = 2,     /* singular matrix (warning only)  INVERSE.F90:41-45,93-95         */
+    MEGA_ERR_BINS = 3,         /* bin table inconsistent          BIN_SEARCH.F90:48-61            */
+    MEGA_ERR_NOMEM = 4,
+    MEGA_ERR_HIP = 5,
+    MEGA_ERR_ARG = 6,          /* invalid argument / unsupported option                         */
+    MEGA_ERR_MATERIAL = 7      /* material type not on this path  CONSTITUTION.F90:79-84          */
+};
+
+/* precision modes */
+enum { MEGA_FP64 = 0, MEGA_MIXED = 1 };
+
+/* What CONSTRUCT_FINT and friends read from the CONTROL module
+ * (CONTROL.F90:27-95), passed explicitly. */
+typedef struct mega_control {
+    int32_t rk_degree;       /* RK_DEGREE, must be 1                          */
+    int32_t rk_cont;         /* RK_CONT, must be 3 (cubic B-spline)           */
+    int32_t itype_int;       /* ITYPE_INT, must be 0 (direct nodal integration) */
+    int32_t ql;              /* QL switch                                      */
+    double  ql_coef;         /* QL_COEF                                        */
+    double  ql_len;          /* QL_LEN                                         */
+    int32_t shsup;           /* SHSUP: ellipsoidal (1) or box (0) support      */
+    int32_t lagrangian;      /* LLAGRANGIAN                                    */
+    int32_t finite_strain;   /* LFINITE_STRAIN                                 */
+    int32_t pdsearch;        /* bins rebuilt every pdsearch steps (PDSEARCH)   */
+    double  gravity[3];      /* IGRAVITY                                       */
+    double  dist_max[3];     /* XDIST_MAX, YDIST_MAX, ZDIST_MAX (bin size)     */
+    int32_t precision;       /* MEGA_FP64 or MEGA_MIXED                        */
+    int32_t surrogate;       /* 1: learned return map with exact fallback      */
+} mega_control;
+
+typedef struct mega_ctx mega_ctx;
+
+/* Device context for one model (replaces the SAVE'd storage HANDELER.F90:72-88
+ * and MAIN's LOCAL_* arrays, MAIN.F90:152-214).
+ *   X0[3n] initial coordinates (GCOO), win[3n] scaled windows (GWIN),
+ *   vol[n] (GVOL), mass[3n] (LOCAL_MASS), ebc[3n] fixities (LOCAL_EBC),
+ *   ebc_node[n] (GEBC_NODES), mat_type[n] (GMAT_TYPE), prop[30n] (GPROP),
+ *   vinit[3n] initial velocity (LOCAL_VEL after STATE_FEILD_INIT).
+ * Nodes with equal property rows share one material slot on the device. */
+int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const double *win,
+                const double *vol, const double *mass, const int32_t *ebc, const int32_t *ebc_node,
+                const int32_t *mat_type, const double *prop, const double *vinit, mega_ctx **out);
+int mega_destroy(mega_ctx *ctx);
+
+/* GET_BINS (optional) + SOFT_SEARCH at positions x_cur[3n] (NULL: the
+ * context's current positions).  rebuild_bins=0 keeps the previous (stale)
+ * bin assignment, as HANDELER does between PDSEARCH rebuilds.
+ * BIN_SEARCH.F90:2-64,153-328; HANDELER.F90:187-273. */
+int mega_search(mega_ctx *ctx, const double *x_cur, int32_t rebuild_bins);
+/* Export the neighbor lists in the reference's 1-based form: gn[n] (GN),
+ * gstart[n] (GSTART), gstack[cap] (GSTACK); *nnz receives the total. */
+int mega_get_neighbors(mega_ctx *ctx, int32_t *gn, int32_t *gstart, int32_t *gstack, int64_t cap, int64_t *nnz);
+
+/* RK1 for every node at its own point x_eval[3n] with nodal positions x_eval
+ * (NULL: initial coordinates), storing shapes on the device (GSTACK_SHP,
+ * GSTACK_DSHP).  RK.f90:16-453 called from FINT.F90:406-414. */
+int mega_shapes(mega_ctx *ctx, const double *x_eval);
+/* Export stored shapes: shp[nnz], dshp[3 nnz], in neighbor-list order. */
+int mega_get_shapes(mega_ctx *ctx, double *shp, double *dshp, int64_t cap);
+
+/* DO_INTERP: phys[3n] = sum_j phi_ij coef[3j+k].  HANDELER.F90:114-133. */
+int mega_interp(mega_ctx *ctx, const double *coef, double *phys);
+
+/* One CONSTRUCT_FINT call on host arrays (FINT.F90:16-1247), constitution
+ * included: inputs dinc[3n] (GDINC), dinc_tot[3n] (GDINC_TOT), x_cur[3n]
+ * (GCOO_CUURENT); in/out stress[6n], strain[6n], state[20n]; out fint[3n],
+ * fext[3n]; int_work accumulates (GINT_WORK).  linit=1 computes and stores
+ * shapes as the first call does. */
+int mega_fint(mega_ctx *ctx, int32_t linit, const double *dinc, const double *dinc_tot,
+              const double *x_cur, double *stress, double *strain, double *state,
+              double *fint, double *fext, double *int_work);
+
+/* Per-point kinematics + constitution without a context (FINT.F90:758-801):
+ * for n points, lmat[9n] (velocity-gradient increment, column-major),
+ * stress[6n], strain[6n], state[20n] in/out, one property row prop[30] of
+ * material type mat_type (1 elastic, 2 J2).  Status MEGA_ERR_RETURN_MAP if
+ * any return map failed to converge. */
+int mega_constitute(int32_t n, const double *lmat, double *stress, double *strain, double *state,
+                    const double *prop, int32_t mat_type, int32_t finite_strain, int32_t precision);
+
+/* nsteps explicit steps of size dt, device resident: predictor, DO_INTERP,
+ * coordinate update, (search + shapes when semi-Lagrangian), CONSTRUCT_FINT,
+ * EBC and corrector.  MAIN.F90:304-658. */
+int mega_step(mega_ctx *ctx, double dt, int32_t nsteps);
+
+/* fields for mega_download / mega_upload */
+enum {
+    MEGA_F_X = 0,        /* current coordinates 3n (LOCAL_COO_CURRENT)     */
+    MEGA_F_V = 1,        /* generalized velocity 3n (LOCAL_VEL)            */
+    MEGA_F_A = 2,        /* acceleration 3n (LOCAL_ACL)                    */
+    MEGA_F_D = 3,        /* displacement increment 3n (LOCAL_DSP)          */
+    MEGA_F_DTOT = 4,     /* generalized total displacement 3n              */
+    MEGA_F_UPHY = 5,     /* physical total displacement 3n                 */
+    MEGA_F_FINT = 6,     /* internal force 3n                              */
+    MEGA_F_FEXT = 7,     /* external force 3n                              */
+    MEGA_F_STRESS = 8,   /* Cauchy stress 6n                               */
+    MEGA_F_STRAIN = 9,   /* strain 6n                                      */
+    MEGA_F_EPS = 10,     /* equivalent plastic strain n (STATE(3))         */
+    MEGA_F_STRAIN_EQ = 11, /* equivalent total strain n (GSTRAIN_EQ)        */
+    MEGA_F_WAVE_VEL = 12,  /* wave speed n (GMAX_WVEL)                     */
+    MEGA_F_VPHY = 13,    /* physical velocity 3n = sum_j phi_ij v_j (current v) */
+    MEGA_F_NFIELDS = 14
+};
+int mega_download(mega_ctx *ctx, int32_t field, double *host);
+int mega_upload(mega_ctx *ctx, int32_t field, const double *host);
+
+/* energies after the last step (MAIN.F90:524-553): internal, kinetic, external */
+int mega_energies(mega_ctx *ctx, double *e_int, double *e_kin, double *e_ext);
+/* mean/max neighbor count of the current lists (k-bar used by the byte model) */
+int mega_stats(mega_ctx *ctx, int64_t *nnz, int32_t *max_neighbors, int64_t *return_map_fallbacks);
+
+/* Enable (1) or disable (0) per-kernel HIP-event timing of mega_step. */
+int mega_profile(mega_ctx *ctx, int32_t enable);
+
+/* Average device time per launch (ms) of each kernel over the last
+ * mega_step call, measured with HIP events on the context's stream.
+ * names[k] receives a static string; returns the number of kernels. */
+int mega_kernel_times(mega_ctx *ctx, const char **names, double *ms_avg, int32_t *launches, int32_t cap);
+
+const char *mega_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,272 @@
+// mega_math.hpp -- per-point math of the explicit meshfree step, as inline
+// device functions.  Each follows the reference routine cited beside it,
+// including the reference's quirks (they are its behaviour, and parity is
+// measured against it):
+//   INV3 returns adj(A)*det(A)                       INVERSE.F90:119-152
+//   DETERMINANT returns |det A|                      DETERMINANT.F90:18-35
+//   y/z kernel derivatives divide by the x window    RK.f90:191-205
+//   singular-kernel epsilon is the REAL(4) 1.0E-015  RK.f90:219
+//   J2 Newton tolerance uses the REAL(4) 1.0E-05     CONSTITUTE_VONMISES.F90:109
+// Small matrices are column-major: a[i + 3*j] = A(i+1, j+1).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define MEGA_HD __host__ __device__ __forceinline__
+
+namespace mega {
+
+// REAL(4) literals widened to double, as the reference's expressions see them
+constexpr double kSingularEps = (double)1.0e-15f;
+constexpr double kNewtonTol = (double)1.0e-05f;
+
+struct Material {       // one property row, with derived constants
+    double nu, E, rho, sig0, B, CE, damping;
+    double lam, mu, l2m;   // props 28-30 (READ.F90:496-507)
+    double bulk, shear;    // BULK_MOD, SHEAR_MOD (FINT_FUNCTIONS.F90:197-229)
+    int type;              // 1 elastic, 2 J2
+    int pad;
+};
+
+// MLS_KERNEL0 cubic B-spline, value and d/d|z| (KERNEL.F90:57-71)
+template <class T>
+MEGA_HD void cubic_kernel(T x, T &phi, T &dphi)
+{
+    if (x <= T(0.5)) {
+        phi = T(2.0 / 3.0) - T(4) * (x * x) + T(4) * (x * x * x);
+        dphi = T(-8) * x + T(12) * (x * x);
+    } else if (x <= T(1)) {
+        phi = T(4.0 / 3.0) - T(4) * x + T(4) * (x * x) - T(4.0 / 3.0) * (x * x * x);
+        dphi = T(-4) + T(8) * x - T(4) * (x * x);
+    } else {
+        phi = T(0);
+        dphi = T(0);
+    }
+}
+
+// INV3: adj(A) * det(A)  (INVERSE.F90:132-147)
+template <class T>
+MEGA_HD void inv3_ref(const T *a, T *r)
+{
+#define A_(i, j) a[((i)-1) + 3 * ((j)-1)]
+#define R_(i, j) r[((i)-1) + 3 * ((j)-1)]
+    T det = A_(1,1)*A_(2,2)*A_(3,3) - A_(1,1)*A_(2,3)*A_(3,2) - A_(1,2)*A_(2,1)*A_(3,3)
+          + A_(1,2)*A_(2,3)*A_(3,1) + A_(1,3)*A_(2,1)*A_(3,2) - A_(1,3)*A_(2,2)*A_(3,1);
+    R_(1,1) =  (A_(2,2)*A_(3,3)-A_(2,3)*A_(3,2))*det;
+    R_(2,1) = -(A_(2,1)*A_(3,3)-A_(2,3)*A_(3,1))*det;
+    R_(3,1) =  (A_(2,1)*A_(3,2)-A_(2,2)*A_(3,1))*det;
+    R_(1,2) = -(A_(1,2)*A_(3,3)-A_(1,3)*A_(3,2))*det;
+    R_(2,2) =  (A_(1,1)*A_(3,3)-A_(1,3)*A_(3,1))*det;
+    R_(3,2) = -(A_(1,1)*A_(3,2)-A_(1,2)*A_(3,1))*det;
+    R_(1,3) =  (A_(1,2)*A_(2,3)-A_(1,3)*A_(2,2))*det;
+    R_(2,3) = -(A_(1,1)*A_(2,3)-A_(1,3)*A_(2,1))*det;
+    R_(3,3) =  (A_(1,1)*A_(2,2)-A_(1,2)*A_(2,1))*det;
+#undef A_
+#undef R_
+}
+
+// DETERMINANT (DETERMINANT.F90:250-252)
+template <class T>
+MEGA_HD T det3_abs(const T *a)
+{
+#define A_(i, j) a[((i)-1) + 3 * ((j)-1)]
+    T d = A_(1,1)*(A_(2,2)*A_(3,3) - A_(3,2)*A_(2,3))
+        + A_(1,2)*(A_(3,1)*A_(2,3) - A_(2,1)*A_(3,3))
+        + A_(1,3)*(A_(2,1)*A_(3,2) - A_(3,1)*A_(2,2));
+#undef A_
+    return d < T(0) ? -d : d;
+}
+
+template <class T>
+MEGA_HD void mm3(const T *a, const T *b, T *c)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            c[i + 3 * k] = a[i] * b[3 * k] + a[i + 3] * b[1 + 3 * k] + a[i + 6] * b[2 + 3 * k];
+}
+
+// 4x4 inverse by cofactors (M44INV, INVERSE.F90:185-223) on the symmetric
+// moment matrix given by its 10 upper entries m[0..9] =
+// (00,01,02,03,11,12,13,22,23,33).  Writes the full inverse (column-major).
+MEGA_HD void m44inv_sym(const double *m, double *r)
+{
+    const double a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3], a11 = m[4], a12 = m[5], a13 = m[6],
+                 a22 = m[7], a23 = m[8], a33 = m[9];
+    // 2x2 minors of the lower two rows
+    const double s0 = a00 * a11 - a01 * a01, s1 = a00 * a12 - a02 * a01, s2 = a00 * a13 - a03 * a01;
+    const double s3 = a01 * a12 - a02 * a11, s4 = a01 * a13 - a03 * a11, s5 = a02 * a13 - a03 * a12;
+    const double c5 = a22 * a33 - a23 * a23, c4 = a12 * a33 - a13 * a23, c3 = a12 * a23 - a13 * a22;
+    const double c2 = a02 * a33 - a03 * a23, c1 = a02 * a23 - a03 * a22, c0 = a02 * a13 - a03 * a12;
+    const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    const double id = 1.0 / det;
+    r[0]  = ( a11 * c5 - a12 * c4 + a13 * c3) * id;
+    r[4]  = (-a01 * c5 + a02 * c4 - a03 * c3) * id;
+    r[8]  = ( a13 * s5 - a23 * s4 + a33 * s3) * id;
+    r[12] = (-a12 * s5 + a22 * s4 - a23 * s3) * id;
+    r[1]  = (-a01 * c5 + a12 * c2 - a13 * c1) * id;
+    r[5]  = ( a00 * c5 - a02 * c2 + a03 * c1) * id;
+    r[9]  = (-a03 * s5 + a23 * s2 - a33 * s1) * id;
+    r[13] = ( a02 * s5 - a22 * s2 + a23 * s1) * id;
+    r[2]  = ( a01 * c4 - a11 * c2 + a13 * c0) * id;
+    r[6]  = (-a00 * c4 + a01 * c2 - a03 * c0) * id;
+    r[10] = ( a03 * s4 - a13 * s2 + a33 * s0) * id;
+    r[14] = (-a02 * s4 + a12 * s2 - a23 * s0) * id;
+    r[3]  = (-a01 * c3 + a11 * c1 - a12 * c0) * id;
+    r[7]  = ( a00 * c3 - a01 * c1 + a02 * c0) * id;
+    r[11] = (-a03 * s3 + a13 * s1 - a23 * s0) * id;
+    r[15] = ( a02 * s3 - a12 * s1 + a22 * s0) * id;
+}
+
+// HUGHES_WINGET (HUGHES_WINGET.F90:16-81): A = I + L/2, G = L INV3(A),
+// W = skew G, R = I + INV3(I - W/2) W, strain = Voigt(sym G) with doubled
+// shears, d = Voigt(sym G).
+MEGA_HD void hughes_winget(const double *L, double *R, double *de, double *d)
+{
+    double A[9], Ai[9], G[9], W[9], IW[9], IWi[9], T[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) A[i + 3 * j] = (i == j ? 1.0 : 0.0) + 0.5 * L[i + 3 * j];
+    inv3_ref(A, Ai);
+    mm3(L, Ai, G);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            W[i + 3 * j] = 0.5 * (G[i + 3 * j] - G[j + 3 * i]);
+            IW[i + 3 * j] = (i == j ? 1.0 : 0.0) - 0.5 * W[i + 3 * j];
+        }
+    inv3_ref(IW, IWi);
+    mm3(IWi, W, T);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) R[e] = T[e];
+    R[0] += 1.0; R[4] += 1.0; R[8] += 1.0;
+    d[0] = G[0]; d[1] = G[4]; d[2] = G[8];
+    d[3] = 0.5 * (G[7] + G[5]);
+    d[4] = 0.5 * (G[6] + G[2]);
+    d[5] = 0.5 * (G[3] + G[1]);
+    de[0] = d[0]; de[1] = d[1]; de[2] = d[2];
+    de[3] = 2.0 * d[3]; de[4] = 2.0 * d[4]; de[5] = 2.0 * d[5];
+}
+
+// ROTATE_TENSOR: v <- R T(v) R^T (ROTATE_TENSOR.F90:21-87), Voigt XX,YY,ZZ,YZ,XZ,XY
+MEGA_HD void rotate_voigt(const double *R, double *v)
+{
+    double T[9] = {v[0], v[5], v[4], v[5], v[1], v[3], v[4], v[3], v[2]};
+    double RT[9], RTRt[9], Rt[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Rt[i + 3 * j] = R[j + 3 * i];
+    mm3(R, T, RT);
+    mm3(RT, Rt, RTRt);
+    v[0] = RTRt[0]; v[1] = RTRt[4]; v[2] = RTRt[8];
+    v[3] = RTRt[7]; v[4] = RTRt[6]; v[5] = RTRt[3];
+}
+
+// (1 + B eps)^CE with the exponents the linear law uses taken exactly
+template <class T>
+MEGA_HD T hard_pow(T base, T e)
+{
+    if (e == T(1)) return base;
+    if (e == T(0)) return T(1);
+    return pow(base, e);
+}
+
+// VON_MISES radial return with power-law hardening
+// (CONSTITUTE_VONMISES.F90:21-158).  Returns Newton iterations, or -1 where the
+// reference stops with 'PLASTICITY MATERIAL MODEL DID NOT CONVERGE'.
+template <class T>
+MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
+{
+    const T mu = T(m.mu), sig0 = T(m.sig0), B = T(m.B), CE = T(m.CE);
+    const T kk = pred[0] + pred[1] + pred[2];
+    const T p3 = kk / T(3);
+    T dev[6] = {pred[0] - p3, pred[1] - p3, pred[2] - p3, pred[3], pred[4], pred[5]};
+    const T tn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                      T(2) * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+    const T r23 = T(sqrt(2.0 / 3.0));
+    const T sig23 = r23 * sig0;   // DSQRT(2/3) sigma_0
+    T xk = sig23 * hard_pow(T(1) + B * eps, CE);
+    T yld = tn - xk;
+    int iters = 0;
+    if (yld > T(0)) {
+        T dg = T(0);
+        const T epst = eps;
+        const T tol = sig23 * T(kNewtonTol);
+        for (int it = 1; it <= 20; ++it) {
+            xk = sig23 * hard_pow(T(1) + B * eps, CE);
+            yld = tn - xk - (T(2) * mu * dg);
+            if (fabs(yld) > tol) {
+                const T xkp = B * CE * sig0 * hard_pow(T(1) + B * eps, CE - T(1));
+                const T dy = T(-2) * mu * (T(1) + xkp / T(3) / mu);
+                dg = dg - yld / dy;
+                iters = it;
+                if (it == 20) return -1;
+            } else {
+                eps = epst + r23 * dg;
+                break;
+            }
+            eps = epst + r23 * dg;
+        }
+        const T s = dg * T(2) * mu / tn;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dev[k] = dev[k] - s * dev[k];
+    }
+    stress[0] = p3 + dev[0];
+    stress[1] = p3 + dev[1];
+    stress[2] = p3 + dev[2];
+    stress[3] = dev[3];
+    stress[4] = dev[4];
+    stress[5] = dev[5];
+    return iters;
+}
+
+// elastic trial: pred = stress + C:de with props 28-30 (FINT.F90:789-794, FORM_CMAT)
+template <class T>
+MEGA_HD void elastic_trial(const T *stress, const T *de, const Material &m, T *pred)
+{
+    const T lam = T(m.lam), mu = T(m.mu), l2m = T(m.l2m);
+    pred[0] = stress[0] + (l2m * de[0] + lam * de[1] + lam * de[2]);
+    pred[1] = stress[1] + (lam * de[0] + l2m * de[1] + lam * de[2]);
+    pred[2] = stress[2] + (lam * de[0] + lam * de[1] + l2m * de[2]);
+    pred[3] = stress[3] + mu * de[3];
+    pred[4] = stress[4] + mu * de[4];
+    pred[5] = stress[5] + mu * de[5];
+}
+
+template <class T>
+MEGA_HD T tnorm(const T *v)
+{
+    return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + T(2) * (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
+}
+template <class T>
+MEGA_HD T tnorm_strain(const T *v)
+{
+    return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
+}
+
+// GMAX_WVEL from secant moduli (FINT.F90:850-870, ESTIMATE_MODULI.F90:16-86)
+MEGA_HD double wave_speed(const double *si, const double *ei, const Material &m)
+{
+    double shear = m.shear, bulk = m.bulk;
+    if (m.type > 1) {
+        const double sk = (si[0] + si[1] + si[2]) / 3.0, ek = (ei[0] + ei[1] + ei[2]) / 3.0;
+        double sd[6] = {si[0] - sk, si[1] - sk, si[2] - sk, si[3], si[4], si[5]};
+        double ss[6] = {-sk, -sk, -sk, 0, 0, 0};
+        double ed[6] = {ei[0] - ek, ei[1] - ek, ei[2] - ek, ei[3], ei[4], ei[5]};
+        double es[6] = {-ek, -ek, -ek, 0, 0, 0};
+        const double nsd = tnorm(sd), nss = tnorm(ss), ned = tnorm_strain(ed), nes = tnorm_strain(es);
+        double st = (ned > 1.0e-8 && nsd > 1.0e-8) ? nsd / ned / 2.0 : shear;
+        st = fmin(fmax(st, shear * 0.5), shear);
+        double bt = (nes > 1.0e-8 && nss > 1.0e-8) ? nss / nes / 3.0 : bulk;
+        bt = fmin(fmax(bt, bulk * 0.5), bulk);
+        shear = st;
+        bulk = bt;
+    }
+    const double pmod = bulk + 4.0 * shear / 3.0;
+    return sqrt(fmax(pmod, shear) / m.rho);
+}
+
+}  // namespace mega

@@ -1,0 +1,246 @@
+"""Python host for the MI355X explicit meshfree step: ctypes bindings of the
+C-ABI in include/mega.h (taylor_amd/libmega_hip.so).
+
+The methods follow the reference's stages and argument meanings:
+
+    Mega.search          GET_BINS + SOFT_SEARCH        BIN_SEARCH.F90:2-328
+    Mega.shapes          RK1 for all nodes             RK.f90:16-453
+    Mega.interp          HANDELER DO_INTERP            HANDELER.F90:114-133
+    Mega.construct_fint  CONSTRUCT_FINT + CONSTITUTION FINT.F90:16-1247
+    Mega.step            MAIN time loop                MAIN.F90:304-658
+    constitute           HUGHES_WINGET/ROTATE/VON_MISES per point
+                                                       FINT.F90:758-801
+
+Errors come back as MegaError carrying the library status (the reference
+stops the program instead).  The library must be built (taylor_amd/build.py);
+nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmega_hip.so")
+
+MEGA_FP64, MEGA_MIXED = 0, 1
+FIELDS = {"x": 0, "v": 1, "a": 2, "d": 3, "dtot": 4, "uphy": 5, "fint": 6, "fext": 7, "stress": 8,
+          "strain": 9, "eps": 10, "strain_eq": 11, "wave_vel": 12, "vphy": 13}
+WIDTH = {"stress": 6, "strain": 6, "eps": 1, "strain_eq": 1, "wave_vel": 1}
+
+
+class MegaError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: status {status} ({_status_string(status)})")
+
+
+class MegaControl(C.Structure):
+    _fields_ = [("rk_degree", C.c_int32), ("rk_cont", C.c_int32), ("itype_int", C.c_int32), ("ql", C.c_int32),
+                ("ql_coef", C.c_double), ("ql_len", C.c_double), ("shsup", C.c_int32), ("lagrangian", C.c_int32),
+                ("finite_strain", C.c_int32), ("pdsearch", C.c_int32), ("gravity", C.c_double * 3),
+                ("dist_max", C.c_double * 3), ("precision", C.c_int32), ("surrogate", C.c_int32)]
+
+
+_lib = None
+
+
+def load_library():
+    """Load libmega_hip.so; raise if it was not built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with python taylor_amd/build.py")
+    lib = C.CDLL(LIB_PATH)
+    vp, dp, ip, i32, i64 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int32, C.c_int64
+    sig = {
+        "mega_create": [C.POINTER(MegaControl), i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)],
+        "mega_destroy": [vp],
+        "mega_search": [vp, vp, i32],
+        "mega_get_neighbors": [vp, vp, vp, vp, i64, C.POINTER(i64)],
+        "mega_shapes": [vp, vp],
+        "mega_get_shapes": [vp, vp, vp, i64],
+        "mega_interp": [vp, vp, vp],
+        "mega_fint": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, dp],
+        "mega_constitute": [i32, vp, vp, vp, vp, vp, i32, i32, i32],
+        "mega_step": [vp, C.c_double, i32],
+        "mega_download": [vp, i32, vp],
+        "mega_upload": [vp, i32, vp],
+        "mega_energies": [vp, dp, dp, dp],
+        "mega_stats": [vp, C.POINTER(i64), ip, C.POINTER(i64)],
+        "mega_profile": [vp, i32],
+        "mega_kernel_times": [vp, C.POINTER(C.c_char_p), dp, ip, i32],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.mega_status_string.argtypes = [C.c_int]
+    lib.mega_status_string.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _status_string(s: int) -> str:
+    try:
+        return load_library().mega_status_string(s).decode()
+    except Exception:
+        return "?"
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(status: int, what: str, allow=()):
+    if status != 0 and status not in allow:
+        raise MegaError(status, what)
+    return status
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def make_control(prob, precision: int = MEGA_FP64, surrogate: int = 0) -> MegaControl:
+    c = MegaControl()
+    c.rk_degree, c.rk_cont, c.itype_int = 1, 3, 0
+    c.ql, c.ql_coef, c.ql_len = int(prob.ql), float(prob.ql_coef), float(prob.ql_len)
+    c.shsup, c.lagrangian, c.finite_strain = int(prob.shsup), int(prob.lagrangian), int(prob.finite_strain)
+    c.pdsearch = int(prob.pdsearch)
+    for k in range(3):
+        c.gravity[k] = float(prob.gravity[k])
+        c.dist_max[k] = float(prob.dist_max[k])
+    c.precision, c.surrogate = int(precision), int(surrogate)
+    return c
+
+
+class Mega:
+    """One device context holding a model (positions, windows, state, lists)."""
+
+    def __init__(self, prob, precision: int = MEGA_FP64, surrogate: int = 0):
+        lib = load_library()
+        self.lib, self.prob, self.n = lib, prob, prob.n
+        self.ctrl = make_control(prob, precision, surrogate)
+        n = prob.n
+        props = prob.props_table()[prob.mat_id]
+        mtype = prob.mat_types()[prob.mat_id]
+        keep = [_f64(prob.coo), _f64(prob.win), _f64(prob.vol), _f64(prob.mass), _i32(prob.ebc),
+                _i32(prob.ebc_node), _i32(mtype), _f64(props), _f64(prob.vinit)]
+        ctx = C.c_void_p()
+        _chk(lib.mega_create(C.byref(self.ctrl), n, *[_p(a) for a in keep], C.byref(ctx)), "mega_create")
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mega_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stages --------------------------------------------------------------
+    def search(self, x=None, rebuild_bins: bool = True):
+        xs = None if x is None else _f64(x)
+        _chk(self.lib.mega_search(self.ctx, _p(xs), int(rebuild_bins)), "mega_search")
+
+    def neighbors(self):
+        """(GN, GSTART, GSTACK) in the reference's 1-based form."""
+        nnz = C.c_int64()
+        _chk(self.lib.mega_get_neighbors(self.ctx, None, None, None, 0, C.byref(nnz)), "mega_get_neighbors")
+        gn = np.zeros(self.n, np.int32)
+        gs = np.zeros(self.n, np.int32)
+        st = np.zeros(max(nnz.value, 1), np.int32)
+        _chk(self.lib.mega_get_neighbors(self.ctx, _p(gn), _p(gs), _p(st), st.size, C.byref(nnz)), "mega_get_neighbors")
+        return gn, gs, st[:nnz.value]
+
+    def shapes(self, x_eval=None):
+        xs = None if x_eval is None else _f64(x_eval)
+        _chk(self.lib.mega_shapes(self.ctx, _p(xs)), "mega_shapes")
+
+    def get_shapes(self):
+        gn, _, _ = self.neighbors()
+        nnz = int(gn.sum())
+        shp = np.zeros(max(nnz, 1))
+        dshp = np.zeros((max(nnz, 1), 3))
+        _chk(self.lib.mega_get_shapes(self.ctx, _p(shp), _p(dshp), nnz), "mega_get_shapes")
+        return shp[:nnz], dshp[:nnz]
+
+    def interp(self, coef):
+        coef = _f64(coef, (self.n, 3))
+        out = np.zeros((self.n, 3))
+        _chk(self.lib.mega_interp(self.ctx, _p(coef), _p(out)), "mega_interp")
+        return out
+
+    def construct_fint(self, linit, dinc, dinc_tot, x_cur, stress, strain, state, int_work=0.0):
+        """One CONSTRUCT_FINT call; stress/strain/state are updated in place.
+        Returns (fint, fext, int_work, status)."""
+        for a, w in ((stress, 6), (strain, 6), (state, 20)):
+            assert a.dtype == np.float64 and a.flags.c_contiguous and a.size == w * self.n
+        fint = np.zeros((self.n, 3))
+        fext = np.zeros((self.n, 3))
+        wk = C.c_double(int_work)
+        st = self.lib.mega_fint(self.ctx, int(linit), _p(_f64(dinc)), _p(_f64(dinc_tot)),
+                                _p(None if x_cur is None else _f64(x_cur)), _p(stress), _p(strain), _p(state),
+                                _p(fint), _p(fext), C.byref(wk))
+        _chk(st, "mega_fint", allow=(1,))
+        return fint, fext, wk.value, st
+
+    def step(self, nsteps: int = 1, dt: float | None = None):
+        st = self.lib.mega_step(self.ctx, float(dt or self.prob.dt), int(nsteps))
+        _chk(st, "mega_step", allow=(1,))
+        return st
+
+    def download(self, name: str) -> np.ndarray:
+        w = WIDTH.get(name, 3)
+        out = np.zeros(self.n * w)
+        _chk(self.lib.mega_download(self.ctx, FIELDS[name], _p(out)), "mega_download")
+        return out.reshape(self.n, w) if w > 1 else out
+
+    def upload(self, name: str, arr):
+        _chk(self.lib.mega_upload(self.ctx, FIELDS[name], _p(_f64(arr))), "mega_upload")
+
+    def energies(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _chk(self.lib.mega_energies(self.ctx, C.byref(a), C.byref(b), C.byref(c)), "mega_energies")
+        return a.value, b.value, c.value
+
+    def stats(self):
+        nnz, mx, fb = C.c_int64(), C.c_int32(), C.c_int64()
+        _chk(self.lib.mega_stats(self.ctx, C.byref(nnz), C.byref(mx), C.byref(fb)), "mega_stats")
+        return nnz.value, mx.value, fb.value
+
+    def profile(self, on: bool = True):
+        _chk(self.lib.mega_profile(self.ctx, int(on)), "mega_profile")
+
+    def kernel_times(self) -> dict:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_int32 * cap)()
+        k = self.lib.mega_kernel_times(self.ctx, names, ms, cnt, cap)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+
+def constitute(lmat, stress, strain, state, props, mat_type: int = 2, finite_strain: int = 1,
+               precision: int = MEGA_FP64) -> int:
+    """Per-point HUGHES_WINGET + ROTATE_TENSOR + elastic trial + VON_MISES on the
+    GPU (FINT.F90:758-801); stress/strain/state updated in place."""
+    lib = load_library()
+    n = lmat.shape[0]
+    for a in (stress, strain, state):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    st = lib.mega_constitute(n, _p(_f64(lmat)), _p(stress), _p(strain), _p(state), _p(_f64(props)),
+                             int(mat_type), int(finite_strain), int(precision))
+    return _chk(st, "mega_constitute", allow=(1,))

@@ -180,3 +180,75 @@ class OracleRun:
             self.lib.or_model_free(C.byref(self.m))
         except Exception:
             pass
+
+
+def oracle_search(lib, prob, x=None, ijk=None):
+    """Bins from X0 geometry, cells of x (or given ijk), SOFT_SEARCH at x.
+    Returns row_ptr (int64), col (int32, 0-based), ijk."""
+    n = prob.n
+    coo = np.ascontiguousarray(prob.coo, dtype=np.float64)
+    x = coo if x is None else np.ascontiguousarray(x, dtype=np.float64)
+    win = np.ascontiguousarray(prob.win)
+    geom = np.zeros(6)
+    nb = np.zeros(3, np.int32)
+    d = prob.dist_max
+    lib.or_bin_geometry(n, coo, win, float(d[0]), float(d[1]), float(d[2]), geom, nb)
+    if ijk is None:
+        ijk = np.zeros((n, 3), np.int32)
+        lib.or_get_bins(n, x, geom, nb, ijk)
+    rp = np.zeros(n + 1, np.int64)
+    nnz = lib.or_soft_search(n, x, win, ijk, nb, int(prob.shsup), rp, None)
+    col = np.zeros(max(nnz, 1), np.int32)
+    lib.or_soft_search(n, x, win, ijk, nb, int(prob.shsup), rp, col.ctypes.data_as(C.c_void_p))
+    return rp, col[:nnz], ijk
+
+
+def oracle_shapes(lib, prob, rp, col, xs=None):
+    """RK1 at every node's own point, nodal positions xs (default X0)."""
+    n = prob.n
+    xs = np.ascontiguousarray(prob.coo if xs is None else xs, dtype=np.float64)
+    win = np.ascontiguousarray(prob.win)
+    ebc = np.ascontiguousarray(prob.ebc_node, dtype=np.int32)
+    c = control(prob)
+    nnz = int(rp[-1])
+    shp = np.zeros(nnz)
+    dshp = np.zeros((nnz, 3))
+    for i in range(n):
+        a, b = int(rp[i]), int(rp[i + 1])
+        ln = b - a
+        s1 = np.zeros(ln)
+        d1 = np.zeros(3 * ln)
+        lib.or_rk1(xs[i].copy(), n, xs, win, ebc, np.ascontiguousarray(col[a:b]), ln, C.byref(c), s1, d1)
+        shp[a:b] = s1
+        dshp[a:b] = d1.reshape(ln, 3)
+    return shp, dshp
+
+
+def oracle_fint(lib, prob, rp, col, shp, dshp, linit, dinc, dinc_tot, x_cur, stress, strain, state,
+                nthreads=1):
+    """One or_construct_fint call; shp/dshp/stress/strain/state updated in place.
+    Returns fint, fext, int_work, status."""
+    n = prob.n
+    io = OrFintIO()
+    keep = dict(win=np.ascontiguousarray(prob.win), vol=np.ascontiguousarray(prob.vol),
+                coo0=np.ascontiguousarray(prob.coo, dtype=np.float64),
+                coo_cur=np.ascontiguousarray(prob.coo if x_cur is None else x_cur, dtype=np.float64),
+                props=np.ascontiguousarray(prob.props_table()), mat_id=np.ascontiguousarray(prob.mat_id, dtype=np.int32),
+                mat_type=prob.mat_types(), ebc_node=np.ascontiguousarray(prob.ebc_node, dtype=np.int32),
+                dinc=np.ascontiguousarray(dinc, dtype=np.float64), dinc_tot=np.ascontiguousarray(dinc_tot, dtype=np.float64),
+                row_ptr=np.ascontiguousarray(rp, dtype=np.int64), col=np.ascontiguousarray(col, dtype=np.int32))
+    fint = np.zeros((n, 3))
+    fext = np.zeros((n, 3))
+    wv = np.zeros(n)
+    seq = np.zeros(n)
+    io.n = n
+    for k, a in keep.items():
+        setattr(io, k, _ptr(a))
+    for k, a in dict(shp=shp, dshp=dshp, stress=stress, strain=strain, state=state).items():
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        setattr(io, k, _ptr(a))
+    io.fint, io.fext, io.wave_vel, io.strain_eq = _ptr(fint), _ptr(fext), _ptr(wv), _ptr(seq)
+    io.bodyforce = None
+    io.int_work, io.linit, io.nthreads = 0.0, int(linit), nthreads
+    st = lib.or_construct_fint(C.byref(io), C.byref(control(prob)))
+    return fint, fext, io.int_work, st

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle on the same
+seeded inputs.  Bars: neighbor lists equal (same entries, same order);
+FP64 floating point within the stated relative tolerances."""
+import numpy as np
+import pytest
+
+from oracle_bind import OracleRun, control, oracle_fint, oracle_search, oracle_shapes
+from taylor_amd import deck
+
+pytestmark = pytest.mark.gpu
+
+H = 0.7e-3
+LEN = 6e-3
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def mega():
+    from taylor_amd import mega as m
+    m.load_library()
+    return m
+
+
+def small_problem(lagrangian=1, seed=0, v0=190.0, **kw):
+    return deck.taylor_problem(H, length=LEN, seed=seed, v0=v0, lagrangian=lagrangian, pdsearch_steps=3, **kw)
+
+
+def test_constitute_matches_oracle(mega, oracle):
+    rng = np.random.default_rng(7)
+    n = 4096
+    for mat in (deck.copper_linear(), deck.Material.j2(78.2e9, 0.3, 2700.0, 0.29e9, 125.0, 0.1)):
+        L = rng.normal(size=(n, 9)) * 2e-3
+        stress = rng.normal(size=(n, 6)) * mat.props[3] * 0.8
+        strain = rng.normal(size=(n, 6)) * 1e-3
+        state = np.zeros((n, 20))
+        state[:, 2] = np.abs(rng.normal(size=n)) * 0.05
+        s_ref, e_ref, st_ref = stress.copy(), strain.copy(), state.copy()
+        for i in range(n):
+            d = np.zeros(6)
+            oracle.or_constitute_point(np.ascontiguousarray(L[i]), s_ref[i], e_ref[i], st_ref[i], mat.props, 2, 1, d)
+        mega.constitute(L, stress, strain, state, mat.props, 2, 1)
+        assert rel_err(stress, s_ref) < 1e-12
+        assert rel_err(strain, e_ref) < 1e-12
+        assert rel_err(state[:, 2], st_ref[:, 2]) < 1e-11
+
+
+def test_search_lists_equal_reference_order(mega, oracle):
+    prob = small_problem()
+    g = mega.Mega(prob)
+    g.search()
+    gn, gs, st = g.neighbors()
+    rp, col, _ = oracle_search(oracle, prob)
+    assert np.array_equal(gn, np.diff(rp))
+    assert np.array_equal(gs, rp[:-1] + 1)
+    assert np.array_equal(st, col + 1)
+
+
+def test_shapes_match_oracle(mega, oracle):
+    for lag in (1,):
+        prob = small_problem(lagrangian=lag)
+        g = mega.Mega(prob)
+        g.search()
+        g.shapes()
+        shp, dshp = g.get_shapes()
+        rp, col, _ = oracle_search(oracle, prob)
+        s_ref, d_ref = oracle_shapes(oracle, prob, rp, col)
+        assert rel_err(shp, s_ref) < 1e-12
+        assert rel_err(dshp, d_ref) < 1e-11
+        # partition of unity holds exactly up to round-off (QL keeps sum phi = 1)
+        sums = np.add.reduceat(shp, rp[:-1].astype(np.int64))
+        assert np.abs(sums - 1).max() < 1e-12
+
+
+def test_interp_matches_oracle(mega, oracle):
+    prob = small_problem()
+    g = mega.Mega(prob)
+    g.search()
+    g.shapes()
+    rp, col, _ = oracle_search(oracle, prob)
+    shp, _ = oracle_shapes(oracle, prob, rp, col)
+    coef = np.random.default_rng(3).normal(size=(prob.n, 3))
+    out = g.interp(coef)
+    ref = np.zeros((prob.n, 3))
+    oracle.or_interp(prob.n, rp, col, shp, np.ascontiguousarray(coef), ref)
+    assert rel_err(out, ref) < 1e-13
+
+
+@pytest.mark.parametrize("lagrangian", [1, 0])
+def test_construct_fint_one_call(mega, oracle, lagrangian):
+    """Per-node internal force after one CONSTRUCT_FINT call on a plastic
+    state: normwise relative error <= 1e-10 (FP64 mode)."""
+    prob = small_problem(lagrangian=lagrangian)
+    n = prob.n
+    rng = np.random.default_rng(11)
+    g = mega.Mega(prob)
+    g.search()
+    rp, col, _ = oracle_search(oracle, prob)
+    nnz = int(rp[-1])
+    shp, dshp = np.zeros(nnz), np.zeros((nnz, 3))
+    z = np.zeros((n, 3))
+    # LINIT call stores the shapes (zero increments)
+    st0 = np.zeros((n, 6)); e0 = np.zeros((n, 6)); s0 = np.zeros((n, 20))
+    oracle_fint(oracle, prob, rp, col, shp, dshp, 1, z, z, prob.coo, st0.copy(), e0.copy(), s0.copy())
+    g.construct_fint(1, z, z, prob.coo, st0.copy(), e0.copy(), s0.copy())
+    # a deformed, yielding state
+    dtot = rng.normal(size=(n, 3)) * 2e-6
+    dinc = rng.normal(size=(n, 3)) * 4e-7 + np.array([0, 0, -1e-6])
+    stress = rng.normal(size=(n, 6)) * 3e8
+    strain = rng.normal(size=(n, 6)) * 1e-3
+    state = np.zeros((n, 20))
+    state[:, 2] = np.abs(rng.normal(size=n)) * 0.02
+    x_cur = prob.coo + dtot
+    s_r, e_r, st_r = stress.copy(), strain.copy(), state.copy()
+    f_r, fx_r, w_r, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, dinc, dtot, x_cur, s_r, e_r, st_r)
+    s_g, e_g, st_g = stress.copy(), strain.copy(), state.copy()
+    f_g, fx_g, w_g, _ = g.construct_fint(0, dinc, dtot, x_cur, s_g, e_g, st_g)
+    assert rel_err(f_g, f_r) < 1e-10
+    scale = np.abs(f_r).max()
+    assert np.abs(f_g - f_r).max() <= 1e-10 * scale
+    assert rel_err(s_g, s_r) < 1e-11
+    assert rel_err(e_g, e_r) < 1e-11
+    assert rel_err(st_g[:, 2], st_r[:, 2]) < 1e-10
+    assert abs(w_g - w_r) <= 1e-10 * abs(w_r)
+
+
+@pytest.mark.parametrize("lagrangian", [1, 0])
+def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
+    """Whole device-resident steps (predictor, interp, search when
+    semi-Lagrangian, fint, corrector) against the oracle's MAIN loop."""
+    prob = small_problem(lagrangian=lagrangian)
+    nsteps = 12
+    g = mega.Mega(prob)
+    g.step(nsteps)
+    run = OracleRun(oracle, prob).step(nsteps)
+    A = run.arr
+    assert rel_err(g.download("x"), A["x"]) < 1e-12
+    assert rel_err(g.download("uphy"), A["dphy_tot"]) < 1e-10
+    assert rel_err(g.download("v"), A["v"]) < 1e-9
+    assert rel_err(g.download("fint"), A["fint"]) < 1e-9
+    assert rel_err(g.download("stress"), A["stress"]) < 1e-9
+    eps_r = A["state"][:, 2]
+    if eps_r.max() > 0:
+        assert rel_err(g.download("eps"), eps_r) < 1e-8
+    if not lagrangian:
+        gn, gs, st = g.neighbors()
+        rp, col, _, _ = run.csr()
+        assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
