@@ -1,20 +1,40 @@
 // mega_hip.hip -- MI355X (gfx950) explicit meshfree step behind the C-ABI in
 // include/mega.h.
 //
-// Kernels (each cites the reference stage it replaces):
-//   k_predict      predictor                          MAIN.F90:348-356
-//   k_interp       DO_INTERP + coordinate update      HANDELER.F90:114-133, MAIN.F90:377-386
-//   k_bin_*        GET_BINS as a counting sort        BIN_SEARCH.F90:2-64
-//   k_search       SOFT_SEARCH (+BIN_INODE)           BIN_SEARCH.F90:69-133,153-328
-//   k_rk           RK1 shape functions/gradients      RK.f90:16-453, KERNEL.F90:3-140
-//   k_fint         CONSTRUCT_FINT + CONSTITUTION      FINT.F90:16-1247, CONSTITUTION.F90:18-89
-//   k_correct      EBC, damping, corrector, energies  MAIN.F90:432-488,524-553
+// One explicit step (MAIN.F90:304-488) runs as three kernels over HBM-resident
+// arrays, plus the search/shape kernels when the formulation needs them:
 //
-// Layout in HBM: node vectors interleaved (3*i+k), stress/strain Voigt 6 per
-// node, neighbor lists as 0-based CSR (row_ptr int32[n+1], col int32[nnz]),
-// shape values phi[nnz] and gradients dphi[3*nnz] in neighbor-list order.
-// One explicit step reads each node array once and streams the pair arrays
-// once for the gather phase and once (from cache) for the force scatter.
+//   k_gather   kGroup lanes per node stream the node's neighbor list (CSR,
+//              coalesced) and gather the neighbors' increments:
+//              F = I + sum grad phi (x) dtot, L0 = sum grad phi (x) d,
+//              u = sum phi d (DO_INTERP, fused when Lagrangian)
+//                                            FINT.F90:310-391,669-679; HANDELER.F90:114-133
+//   k_node     one lane per node: |det F|, INV3 push-forward, Hughes-Winget,
+//              rotation, elastic trial, J2 radial return, state write-back,
+//              work, and P = V J sigma F^-T
+//                                            FINT.F90:758-870; CONSTITUTION.F90:18-89
+//   k_force    kGroup lanes per node over the transposed neighbor list:
+//              f_j = sum_i P_i grad phi_ij (= B^T sigma V J scattered by the
+//              reference, FINT.F90:997-1116, gathered here without atomics,
+//              deterministic), then EBC + corrector (MAIN.F90:432-488), energies
+//              (MAIN.F90:524-553) and the next step's predictor (MAIN.F90:353-355)
+//   k_predict  predictor when it is not fused into the previous k_force
+//   k_bin_*    GET_BINS as a counting sort     BIN_SEARCH.F90:2-64
+//   k_search   SOFT_SEARCH (+BIN_INODE)        BIN_SEARCH.F90:69-133,153-328
+//   k_t*       transposed neighbor lists (built once per search)
+//   k_rk       RK1 shape functions/gradients   RK.f90:16-453, KERNEL.F90:3-140
+//
+// Precision: R = double (MEGA_FP64) or float (MEGA_MIXED) for shape data,
+// stress/strain/plastic strain, the return map and the force; positions,
+// velocities, displacements, the moment matrix and its inverse, F, F^-1 and
+// the Hughes-Winget rotation are FP64 in both modes.
+//
+// Layout in HBM: node vectors interleaved (3*i+k); stress/strain 6 per node
+// (Voigt XX,YY,ZZ,YZ,XZ,XY); neighbor lists as 0-based CSR (row_ptr int32[n+1],
+// col int32[nnz]) with phi[nnz], grad phi[3 nnz] in list order; the transposed
+// lists (trow[n+1], tsrc[nnz]) carry grad phi[3 nnz] in transposed order.
+// Pair kernels map blocks to XCDs in contiguous node ranges so that each
+// XCD's L2 holds the neighbor data its blocks gather.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -25,6 +45,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mega.h"
@@ -35,112 +56,144 @@ using mega::Material;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGroup = 16;           // lanes cooperating on one node's neighbor list
-constexpr int kNodesPerBlock = 128;  // nodes per k_fint block (LDS-staged phases)
+constexpr int kGroup = 16;      // lanes cooperating on one node's neighbor list
+constexpr int kXcd = 8;         // XCDs on MI355X
+constexpr int kPStride = 10;    // per-node P (9) + rho V J (gravity)
 
-struct Dev {
-    int n = 0;
-    int64_t nnz = 0, nnz_cap = 0;
-    // node arrays (FP64)
-    double *X0 = nullptr, *x = nullptr, *v = nullptr, *a = nullptr, *d = nullptr, *dtot = nullptr;
-    double *uphy = nullptr, *fint = nullptr, *fext = nullptr, *fext_nmo = nullptr;
-    double *stress = nullptr, *strain = nullptr, *eps = nullptr;
-    double *win = nullptr, *vol = nullptr, *mass = nullptr;
-    double *strain_eq = nullptr, *wave_vel = nullptr, *vphy = nullptr, *scratch3 = nullptr;
-    int32_t *ebc = nullptr, *ebc_node = nullptr, *mat = nullptr;
-    Material *mats = nullptr;
-    // CSR + shapes
-    int32_t *row_ptr = nullptr, *col = nullptr;
-    double *shp = nullptr, *dshp = nullptr;
-    // bins
-    int32_t *ijk = nullptr, *cell_start = nullptr, *cell_nodes = nullptr, *cell_cursor = nullptr;
-    int32_t *counts = nullptr;
-    // accumulators: [0] work, [1] ke, [2] ie (small strain), [3] ee
-    double *acc = nullptr;
-    int32_t *flags = nullptr;   // [0] return-map failures, [1] unsupported material
-    void *cub_tmp = nullptr;
-    size_t cub_tmp_bytes = 0;
-};
+// ---------------------------------------------------------------------------
+// kernel arguments
 
-struct Params {   // kernel arguments
+struct Params {
     int n;
-    const int32_t *row_ptr, *col;
-    double *shp, *dshp;
+    // lists
+    const int32_t *row_ptr, *col, *trow, *tsrc, *tpos;
+    void *shp, *dshp, *tdshp, *tshp;                 // R
+    // node arrays, FP64 masters
     const double *X0, *win, *vol, *mass;
-    double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo;
-    double *stress, *strain, *eps, *strain_eq, *wave_vel, *vphy;
+    double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
+    float *d32, *dtot32;                             // gather copies (mixed)
+    // state and scratch in R
+    void *stress, *strain, *eps, *red, *P;
     const int32_t *ebc, *ebc_node, *mat;
     const Material *mats;
-    double *acc;
-    int32_t *flags;
+    double *acc;                                     // [0] work [1] ke [2] ie [3] ee
+    int32_t *flags;                                  // [0] return-map failures [1] bad material
     int ql;
     double ql_coef, ql_len;
-    int shsup, lagrangian, finite_strain, has_gravity;
+    int shsup, finite_strain, has_gravity;
     double grav[3];
     double dt;
 };
 
-// ---------------------------------------------------------------------------
-// group reductions over kGroup lanes (xor butterfly: every lane gets the sum)
+template <class R> __device__ __forceinline__ R *as(void *p) { return static_cast<R *>(p); }
 
-template <int W>
-__device__ __forceinline__ double group_sum(double v)
+// ---------------------------------------------------------------------------
+// helpers
+
+template <int W, class T>
+__device__ __forceinline__ T group_sum(T v)
 {
 #pragma unroll
     for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
     return v;
 }
 
+// block of kBlock threads -> thread 0 gets the sum
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
-    // one value per thread -> thread 0 gets the block sum
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (l == 0) red[w] = v;
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+        for (int k = 0; k < kBlock / 64; ++k) s += red[k];
     __syncthreads();
     return s;
 }
 
+// Blocks are handed to the XCDs round-robin; this maps them so that XCD x
+// processes the contiguous range [x*per, (x+1)*per) of logical blocks.  The
+// grid is padded to a multiple of kXcd; logical blocks beyond the real count
+// exit at once.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = gridDim.x / kXcd;
+    return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
+
 // ---------------------------------------------------------------------------
-// predictor (MAIN.F90:353-355): d = dt v + dt^2/2 a; dtot += d; v += dt/2 a.
-// Also clears the force accumulators for this step's scatter.
+// predictor (MAIN.F90:353-355): d = dt v + dt^2/2 a; dtot += d; v += dt/2 a
+
+__device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
+{
+    const double dt = p.dt;
+    const double v = p.v[q], a = p.a[q];
+    const double d = dt * v + dt * dt * 0.5 * a;
+    const double t = p.dtot[q] + d;
+    p.d[q] = d;
+    p.dtot[q] = t;
+    p.v[q] = v + dt * 0.5 * a;
+    if (p.d32) { p.d32[q] = (float)d; p.dtot32[q] = (float)t; }
+}
 
 __global__ void k_predict(Params p)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) { p.acc[1] = 0.0; p.acc[2] = 0.0; }
     if (q >= 3 * (int64_t)p.n) return;
-    const double dt = p.dt;
-    const double v = p.v[q], a = p.a[q];
-    const double d = dt * v + dt * dt * 0.5 * a;
-    p.d[q] = d;
-    p.dtot[q] = p.dtot[q] + d;
-    p.v[q] = v + dt * 0.5 * a;
-    p.fint[q] = 0.0;
-    if (p.has_gravity) p.fext[q] = 0.0;
+    predict_dof(p, q);
+}
+
+__global__ void k_cast32(int64_t m, const double *a, float *b)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) b[q] = (float)a[q];
+}
+
+template <class R>
+__global__ void k_widen(int64_t m, const R *a, double *b)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) b[q] = (double)a[q];
+}
+
+template <class R>
+__global__ void k_narrow(int64_t m, const double *a, R *b)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) b[q] = (R)a[q];
+}
+
+// GSTRAIN_EQ from the stored strain (FINT.F90:831-834)
+template <class R>
+__global__ void k_strain_eq(int n, const R *strain, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double e[6];
+    for (int k = 0; k < 6; ++k) e[k] = (double)strain[6 * i + k];
+    const double q = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + 2.0 * (e[3] * e[3] + e[4] * e[4] + e[5] * e[5]);
+    out[i] = sqrt(q * 2.0 / 3.0);
 }
 
 // ---------------------------------------------------------------------------
-// DO_INTERP (HANDELER.F90:114-133): out_i = sum_j phi_ij coef_j, kGroup lanes
-// per node.  UPDATE: coef is the increment d and the result updates the
-// physical total displacement and the coordinates (MAIN.F90:377-386).
+// DO_INTERP (HANDELER.F90:114-133): out_i = sum_j phi_ij coef_j.  UPDATE: the
+// increment d is interpolated into the physical total displacement and the
+// coordinates (MAIN.F90:377-386).
 
-template <bool UPDATE>
+template <class R, bool UPDATE>
 __global__ __launch_bounds__(kBlock) void k_interp(Params p, const double *coef, double *out)
 {
     const int node = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup;
     const int lane = threadIdx.x & (kGroup - 1);
     if (node >= p.n) return;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
-    const double *src = UPDATE ? p.d : coef;   // the update always interpolates this step's increment
+    const double *src = UPDATE ? p.d : coef;
+    const R *shp = as<R>(p.shp);
     double u0 = 0, u1 = 0, u2 = 0;
     for (int q = p0 + lane; q < p1; q += kGroup) {
         const int j = p.col[q];
-        const double f = p.shp[q];
+        const double f = (double)shp[q];
         u0 += f * src[3 * j]; u1 += f * src[3 * j + 1]; u2 += f * src[3 * j + 2];
     }
     u0 = group_sum<kGroup>(u0); u1 = group_sum<kGroup>(u1); u2 = group_sum<kGroup>(u2);
@@ -159,8 +212,9 @@ __global__ __launch_bounds__(kBlock) void k_interp(Params p, const double *coef,
 
 // ---------------------------------------------------------------------------
 // RK1 at point x_i with nodal positions xs (RK.f90:16-453).  kGroup lanes per
-// node accumulate the symmetric moment matrix M and its derivatives (FP64),
-// then every lane forms M^-1 and evaluates phi, grad phi for its neighbors.
+// node accumulate the symmetric moment matrix M and its derivatives in FP64,
+// then every lane forms M^-1 and evaluates phi, grad phi for its neighbors,
+// written in list order and, through tpos, in transposed order.
 
 struct KernelEval {
     double phi, px, py, pz;
@@ -203,6 +257,7 @@ __device__ __forceinline__ KernelEval rk_window(const Params &p, const double *x
     return e;
 }
 
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
 {
     const int node = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup;
@@ -210,8 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
     if (node >= p.n) return;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
     const double xi[3] = {xs[3 * node], xs[3 * node + 1], xs[3 * node + 2]};
-    // 10 unique entries each of M, Mx, My, Mz (symmetric) + sum of phi
-    double m[41];
+    double m[41];   // 10 unique entries each of M, Mx, My, Mz + sum of phi
 #pragma unroll
     for (int k = 0; k < 41; ++k) m[k] = 0.0;
     for (int q = p0 + lane; q < p1; q += kGroup) {
@@ -225,8 +279,8 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
             for (int c = r; c < 4; ++c, ++t) {
                 const double hh = H[r] * H[c];
                 m[t] += hh * e.phi;
-                // M_alpha(r,c) = (dH_r H_c + H_r dH_c) phi + H_r H_c phi_alpha, dH = unit vector
-                m[10 + t] += (double)((r == 1) * 1) * H[c] * e.phi + (double)(c == 1) * H[r] * e.phi + hh * e.px;
+                // M_a(r,c) = (dH_r H_c + H_r dH_c) phi + H_r H_c phi_a with dH the unit vector e_a
+                m[10 + t] += (double)(r == 1) * H[c] * e.phi + (double)(c == 1) * H[r] * e.phi + hh * e.px;
                 m[20 + t] += (double)(r == 2) * H[c] * e.phi + (double)(c == 2) * H[r] * e.phi + hh * e.py;
                 m[30 + t] += (double)(r == 3) * H[c] * e.phi + (double)(c == 3) * H[r] * e.phi + hh * e.pz;
             }
@@ -234,9 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
     }
 #pragma unroll
     for (int k = 0; k < 41; ++k) m[k] = group_sum<kGroup>(m[k]);
-    // quasi-linear augmentation of M (RK.f90:251-303)
     double hs[4] = {0, 0, 0, 0};
-    if (p.ql) {
+    if (p.ql) {   // quasi-linear augmentation (RK.f90:251-303, 357-377)
         const double L = p.ql_len;
         double ms[10] = {0};
 #pragma unroll
@@ -262,9 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
     }
     double Mi[16];
     mega::m44inv_sym(m, Mi);
-    // B = e1^T M^-1; B_alpha = -(B M_alpha) M^-1
-    double B[4] = {Mi[0], Mi[4], Mi[8], Mi[12]};
-    double Bd[3][4];
+    const double B[4] = {Mi[0], Mi[4], Mi[8], Mi[12]};   // e1^T M^-1
+    double Bd[3][4];                                     // -(B M_a) M^-1
 #pragma unroll
     for (int al = 0; al < 3; ++al) {
         const double *s = m + 10 * (al + 1);
@@ -277,6 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
         for (int c = 0; c < 4; ++c)
             Bd[al][c] = -(u[0] * Mi[4 * c] + u[1] * Mi[1 + 4 * c] + u[2] * Mi[2 + 4 * c] + u[3] * Mi[3 + 4 * c]);
     }
+    R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp), *tdshp = as<R>(p.tdshp), *tshp = as<R>(p.tshp);
     for (int q = p0 + lane; q < p1; q += kGroup) {
         const int j = p.col[q];
         const KernelEval e = rk_window(p, xi, xs, j);
@@ -288,245 +341,318 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
         const double cx = Bd[0][0] * H[0] + Bd[0][1] * H[1] + Bd[0][2] * H[2] + Bd[0][3] * H[3] + B[1];
         const double cy = Bd[1][0] * H[0] + Bd[1][1] * H[1] + Bd[1][2] * H[2] + Bd[1][3] * H[3] + B[2];
         const double cz = Bd[2][0] * H[0] + Bd[2][1] * H[1] + Bd[2][2] * H[2] + Bd[2][3] * H[3] + B[3];
-        p.shp[q] = c0 * e.phi;
-        p.dshp[3 * q] = cx * e.phi + c0 * e.px;
-        p.dshp[3 * q + 1] = cy * e.phi + c0 * e.py;
-        p.dshp[3 * q + 2] = cz * e.phi + c0 * e.pz;
+        const R f = (R)(c0 * e.phi);
+        const R gx = (R)(cx * e.phi + c0 * e.px), gy = (R)(cy * e.phi + c0 * e.py), gz = (R)(cz * e.phi + c0 * e.pz);
+        shp[q] = f;
+        dshp[3 * q] = gx; dshp[3 * q + 1] = gy; dshp[3 * q + 2] = gz;
+        if (p.tpos) {
+            const int t = p.tpos[q];
+            tdshp[3 * t] = gx; tdshp[3 * t + 1] = gy; tdshp[3 * t + 2] = gz;
+            if (tshp) tshp[t] = f;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// CONSTRUCT_FINT for one block of kNodesPerBlock nodes, in three phases:
-//  A  kGroup lanes per node stream the node's neighbor list once (coalesced
-//     over the CSR) and gather d, dtot of the neighbors:
-//       F = I + sum grad phi0 (x) dtot   (FINT.F90:363-376, Lagrangian)
-//       L0 = sum grad phi0 (x) d         (FINT.F90:673-679)
-//       u  = sum phi d                   (DO_INTERP, fused when Lagrangian)
-//     the 21 sums are staged in LDS;
-//  B  one thread per node: J = |det F|, F^-1 = INV3(F) (bug-compatible),
-//     L = L0 F^-1, Hughes-Winget, rotation, elastic trial, radial return,
-//     state write-back, work, and P = V J sigma F^-T staged in LDS;
-//  C  kGroup lanes per node re-read grad phi0 and scatter f_j += P grad phi0_ij
-//     (= B^T sigma V J of FINT.F90:1007-1112) with FP64 atomics.
+// gather phase of CONSTRUCT_FINT.  FP64: red = [F-I (9) | L0 (9)] with
+// F(k,l) = sum_j g_l dtot_jk, L0(k,l) = sum_j g_l d_jk (column-major, FINT.F90:
+// 363-376, 673-679).  Mixed: the FP32 sums are taken over differences to the
+// node's own value, red = [F' (9) | L' (9) | S = sum g (3)], and k_node adds
+// S (x) own value back in FP64; this keeps the FP32 sums free of the rigid
+// translation.  UPDATE_X: DO_INTERP is fused, u = sum phi d updates the
+// physical displacement and coordinates.
 
-template <bool PUSH, bool FUSE_INTERP>
-__global__ __launch_bounds__(kBlock) void k_fint(Params p)
+template <class R, bool PUSH, bool UPDATE_X>
+__global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 {
-    __shared__ double s_sum[kNodesPerBlock][22];
-    __shared__ double s_red[kBlock / 64];
-    const int nb0 = blockIdx.x * kNodesPerBlock;
-    const int grp = threadIdx.x / kGroup, lane = threadIdx.x & (kGroup - 1);
-    constexpr int kGroups = kBlock / kGroup;
-
-    // ---- phase A
-    for (int ln = grp; ln < kNodesPerBlock; ln += kGroups) {
-        const int i = nb0 + ln;
-        double F[9], L[9], u[3];
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    if (blk == 0 && threadIdx.x == 0) { p.acc[1] = 0.0; p.acc[2] = 0.0; }
+    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int lane = threadIdx.x & (kGroup - 1);
+    if (node >= p.n) return;
+    constexpr bool MIX = std::is_same<R, float>::value;
+    const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
+    const R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp);
+    R F[9], L[9], S[3], u[3];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { F[k] = 0.0; L[k] = 0.0; }
-        u[0] = u[1] = u[2] = 0.0;
-        if (i < p.n) {
-            const int p0 = p.row_ptr[i], p1 = p.row_ptr[i + 1];
-            for (int q = p0 + lane; q < p1; q += kGroup) {
-                const int j = p.col[q];
-                const double g0 = p.dshp[3 * q], g1 = p.dshp[3 * q + 1], g2 = p.dshp[3 * q + 2];
-                const double d0 = p.d[3 * j], d1 = p.d[3 * j + 1], d2 = p.d[3 * j + 2];
-                L[0] += g0 * d0; L[1] += g0 * d1; L[2] += g0 * d2;
-                L[3] += g1 * d0; L[4] += g1 * d1; L[5] += g1 * d2;
-                L[6] += g2 * d0; L[7] += g2 * d1; L[8] += g2 * d2;
-                if (PUSH) {
-                    const double t0 = p.dtot[3 * j], t1 = p.dtot[3 * j + 1], t2 = p.dtot[3 * j + 2];
-                    F[0] += g0 * t0; F[1] += g0 * t1; F[2] += g0 * t2;
-                    F[3] += g1 * t0; F[4] += g1 * t1; F[5] += g1 * t2;
-                    F[6] += g2 * t0; F[7] += g2 * t1; F[8] += g2 * t2;
-                }
-                if (FUSE_INTERP) {
-                    const double f = p.shp[q];
-                    u[0] += f * d0; u[1] += f * d1; u[2] += f * d2;
-                }
+    for (int k = 0; k < 9; ++k) { F[k] = R(0); L[k] = R(0); }
+    S[0] = S[1] = S[2] = R(0);
+    u[0] = u[1] = u[2] = R(0);
+    R di[3] = {R(0), R(0), R(0)}, ti[3] = {R(0), R(0), R(0)};
+    if (MIX) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { di[k] = (R)p.d32[3 * node + k]; ti[k] = (R)p.dtot32[3 * node + k]; }
+    }
+    for (int q = p0 + lane; q < p1; q += kGroup) {
+        const int j = p.col[q];
+        const R g0 = dshp[3 * q], g1 = dshp[3 * q + 1], g2 = dshp[3 * q + 2];
+        R dj[3], tj[3];
+        if (MIX) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dj[k] = (R)p.d32[3 * j + k] - di[k];
+                if (PUSH) tj[k] = (R)p.dtot32[3 * j + k] - ti[k];
+            }
+            S[0] += g0; S[1] += g1; S[2] += g2;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dj[k] = (R)p.d[3 * j + k];
+                if (PUSH) tj[k] = (R)p.dtot[3 * j + k];
             }
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) L[k] = group_sum<kGroup>(L[k]);
+        for (int k = 0; k < 3; ++k) { L[k] += g0 * dj[k]; L[3 + k] += g1 * dj[k]; L[6 + k] += g2 * dj[k]; }
         if (PUSH) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] = group_sum<kGroup>(F[k]);
+            for (int k = 0; k < 3; ++k) { F[k] += g0 * tj[k]; F[3 + k] += g1 * tj[k]; F[6 + k] += g2 * tj[k]; }
         }
-        if (FUSE_INTERP) {
+        if (UPDATE_X) {
+            const R f = shp[q];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) u[k] = group_sum<kGroup>(u[k]);
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) { s_sum[ln][k] = L[k]; s_sum[ln][9 + k] = F[k]; }
-            s_sum[ln][18] = u[0]; s_sum[ln][19] = u[1]; s_sum[ln][20] = u[2];
+            for (int k = 0; k < 3; ++k) u[k] += f * dj[k];
         }
     }
-    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L[k] = group_sum<kGroup>(L[k]);
+    if (PUSH) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = group_sum<kGroup>(F[k]);
+    }
+    if (MIX) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[k] = group_sum<kGroup>(S[k]);
+    }
+    if (UPDATE_X) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) u[k] = group_sum<kGroup>(u[k]);
+    }
+    constexpr int W = MIX ? 21 : 18;
+    R *red = as<R>(p.red) + (int64_t)W * node;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { red[k] = F[k]; red[9 + k] = L[k]; }
+        if (MIX) { red[18] = S[0]; red[19] = S[1]; red[20] = S[2]; }
+    }
+    if (UPDATE_X && lane < 3) {
+        const int qd = 3 * node + lane;
+        const R uk = lane == 0 ? u[0] : (lane == 1 ? u[1] : u[2]);
+        // mixed: u = d_i + sum phi (d_j - d_i), with sum phi = 1 by RK consistency
+        const double du = MIX ? p.d[qd] + (double)uk : (double)uk;
+        const double t = p.uphy[qd] + du;
+        p.uphy[qd] = t;
+        p.x[qd] = p.X0[qd] + t;
+    }
+}
 
-    // ---- phase B
+// ---------------------------------------------------------------------------
+// per-node update of CONSTRUCT_FINT (FINT.F90:353-391, 758-870), one lane per
+// node: push-forward by INV3 (adj F * det F, the reference's own INV3), L,
+// Hughes-Winget rotation (FP64), rotated stress/strain, elastic trial and J2
+// return (FP64 or FP32), state write-back, work, wave speed, and
+// P = V J sigma F^-T for the force gather.
+
+template <class R, bool PUSH>
+__global__ __launch_bounds__(kBlock) void k_node(Params p)
+{
+    __shared__ double s_red[kBlock / 64];
+    constexpr bool MIX = std::is_same<R, float>::value;
+    constexpr int W = MIX ? 21 : 18;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double work = 0.0;
-    if (threadIdx.x < kNodesPerBlock) {
-        const int ln = threadIdx.x, i = nb0 + ln;
-        if (i < p.n) {
-            double L0[9], Fm[9];
+    if (i < p.n) {
+        const R *red = as<R>(p.red) + (int64_t)W * i;
+        double Fm[9], L0[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) { L0[k] = s_sum[ln][k]; Fm[k] = s_sum[ln][9 + k]; }
-            const Material &mt = p.mats[p.mat[i]];
-            const double vol = p.vol[i];
-            double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-            if (PUSH) {
-                // sums are stored as F_{k,l} = sum g_l t_k at index 3*l + k (column-major)
-                Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
-                det = mega::det3_abs(Fm);
-                mega::inv3_ref(Fm, IF);
-            }
-            // L(k,l) = sum_m L0(k,m) IF(m,l)
-            double Lm[9];
-            if (PUSH) mega::mm3(L0, IF, Lm);
-            else
+        for (int k = 0; k < 9; ++k) { Fm[k] = (double)red[k]; L0[k] = (double)red[9 + k]; }
+        if (MIX) {   // add back S (x) own value: F(k,l) += S_l dtot_ik, L0(k,l) += S_l d_ik
+            const double S[3] = {(double)red[18], (double)red[19], (double)red[20]};
 #pragma unroll
-                for (int k = 0; k < 9; ++k) Lm[k] = L0[k];
-            double sig[6], eps6[6], de[6], dd[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) { sig[k] = p.stress[6 * i + k]; eps6[k] = p.strain[6 * i + k]; }
-            double old_s[6], old_e[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) { old_s[k] = sig[k]; old_e[k] = eps6[k]; }
-            if (p.finite_strain) {
-                double R[9];
-                mega::hughes_winget(Lm, R, de, dd);
-                mega::rotate_voigt(R, sig);
-                mega::rotate_voigt(R, eps6);
-            } else {
-                de[0] = Lm[0]; de[1] = Lm[4]; de[2] = Lm[8];
-                de[3] = Lm[5] + Lm[7]; de[4] = Lm[2] + Lm[6]; de[5] = Lm[1] + Lm[3];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) dd[k] = 0.0;
-            }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) eps6[k] = eps6[k] + de[k];
-            double pred[6];
-            mega::elastic_trial(sig, de, mt, pred);
-            double ep = p.eps[i];
-            if (mt.type == 2) {
-                if (mega::von_mises(sig, pred, ep, mt) < 0) atomicAdd(&p.flags[0], 1);
-            } else if (mt.type == 1) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) sig[k] = pred[k];
-            } else {
-                atomicAdd(&p.flags[1], 1);
-            }
-            p.eps[i] = ep;
-            double si[6], ei[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                si[k] = sig[k] - old_s[k];
-                ei[k] = eps6[k] - old_e[k];
-                p.stress[6 * i + k] = sig[k];
-                p.strain[6 * i + k] = eps6[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                if (p.finite_strain) work += 0.5 * dd[k] * (2.0 * sig[k] - si[k]) * vol * det;
-                else work += 0.5 * ei[k] * si[k] * vol * det;
-            }
-            if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
-            if (p.strain_eq) {
-                const double qv = eps6[0] * eps6[0] + eps6[1] * eps6[1] + eps6[2] * eps6[2] +
-                                  2.0 * (eps6[3] * eps6[3] + eps6[4] * eps6[4] + eps6[5] * eps6[5]);
-                p.strain_eq[i] = sqrt(qv * 2.0 / 3.0);
-            }
-            if (FUSE_INTERP) {
+            for (int l = 0; l < 3; ++l)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const double t = p.uphy[3 * i + k] + s_sum[ln][18 + k];
-                    p.uphy[3 * i + k] = t;
-                    p.x[3 * i + k] = p.X0[3 * i + k] + t;
+                    if (PUSH) Fm[k + 3 * l] += S[l] * (double)p.dtot32[3 * i + k];
+                    L0[k + 3 * l] += S[l] * (double)p.d32[3 * i + k];
                 }
-            }
-            // P = V J sigma IF^T :  f_j = P grad phi0_j
-            const double S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
-            const double w = vol * det;
+        }
+        const Material &mt = p.mats[p.mat[i]];
+        const double vol = p.vol[i];
+        double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Lm[9];
+        if (PUSH) {
+            Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
+            det = mega::det3_abs(Fm);
+            mega::inv3_ref(Fm, IF);
+            mega::mm3(L0, IF, Lm);   // L = L0 F^-1 (FINT.F90:380-391 then 669-679)
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Lm[k] = L0[k];
+        }
+        R *stress = as<R>(p.stress) + 6 * (int64_t)i, *strain = as<R>(p.strain) + 6 * (int64_t)i;
+        double sig[6], e6[6], de[6], dd[6], old_s[6], old_e[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)stress[k]; e6[k] = old_e[k] = (double)strain[k]; }
+        if (p.finite_strain) {
+            double Rot[9];
+            mega::hughes_winget(Lm, Rot, de, dd);
+            mega::rotate_voigt(Rot, sig);
+            mega::rotate_voigt(Rot, e6);
+        } else {   // small strain (FINT.F90:772-783)
+            de[0] = Lm[0]; de[1] = Lm[4]; de[2] = Lm[8];
+            de[3] = Lm[5] + Lm[7]; de[4] = Lm[2] + Lm[6]; de[5] = Lm[1] + Lm[3];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dd[k] = 0.0;
+        }
+        R sr[6], der[6], pred[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { e6[k] = e6[k] + de[k]; sr[k] = (R)sig[k]; der[k] = (R)de[k]; }
+        mega::elastic_trial(sr, der, mt, pred);
+        R ep = as<R>(p.eps)[i];
+        if (mt.type == 2) {
+            if (mega::von_mises(sr, pred, ep, mt) < 0) atomicAdd(&p.flags[0], 1);
+        } else if (mt.type == 1) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sr[k] = pred[k];
+        } else {
+            atomicAdd(&p.flags[1], 1);
+        }
+        as<R>(p.eps)[i] = ep;
+        double si[6], ei[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            stress[k] = sr[k];
+            strain[k] = (R)e6[k];
+            sig[k] = (double)sr[k];
+            si[k] = sig[k] - old_s[k];
+            ei[k] = e6[k] - old_e[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (p.finite_strain) work += 0.5 * dd[k] * (2.0 * sig[k] - si[k]) * vol * det;
+            else work += 0.5 * ei[k] * si[k] * vol * det;
+        }
+        if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
+        // P = V J sigma IF^T  (f_j = P grad phi_ij)
+        const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
+        const double w = vol * det;
+        R *P = as<R>(p.P) + kPStride * (int64_t)i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    s_sum[ln][r + 3 * c] =
-                        w * (S[r] * IF[c] + S[r + 3] * IF[c + 3] + S[r + 6] * IF[c + 6]);
-            if (p.has_gravity) s_sum[ln][9] = mt.rho * w;
+                P[r + 3 * c] = (R)(w * (Sg[r] * IF[c] + Sg[r + 3] * IF[c + 3] + Sg[r + 6] * IF[c + 6]));
+        P[9] = (R)(mt.rho * w);
+    }
+    const double ws = block_sum(work, s_red);
+    if (threadIdx.x == 0 && ws != 0.0) atomicAdd(&p.acc[0], ws);
+}
+
+// ---------------------------------------------------------------------------
+// force gather over the transposed lists, then (CORRECT) EBC, damping and the
+// corrector, energies, and (PREDICT) the next predictor, per degree of freedom.
+
+template <class R, bool CORRECT, bool PREDICT>
+__global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
+{
+    __shared__ double s_red[kBlock / 64];
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int lane = threadIdx.x & (kGroup - 1);
+    const bool live = node < p.n;
+    R f0 = R(0), f1 = R(0), f2 = R(0), fe = R(0);
+    if (live) {
+        const int t0 = p.trow[node], t1 = p.trow[node + 1];
+        const R *tg = as<R>(p.tdshp), *Pb = as<R>(p.P), *ts = as<R>(p.tshp);
+        for (int t = t0 + lane; t < t1; t += kGroup) {
+            const int i = p.tsrc[t];
+            const R g0 = tg[3 * t], g1 = tg[3 * t + 1], g2 = tg[3 * t + 2];
+            const R *P = Pb + kPStride * (int64_t)i;
+            f0 += P[0] * g0 + P[3] * g1 + P[6] * g2;
+            f1 += P[1] * g0 + P[4] * g1 + P[7] * g2;
+            f2 += P[2] * g0 + P[5] * g1 + P[8] * g2;
+            if (p.has_gravity) fe += ts[t] * P[9];
         }
     }
-    const double wsum = block_sum(work, s_red);
-    if (threadIdx.x == 0 && wsum != 0.0) atomicAdd(&p.acc[0], wsum);
-    // block_sum synchronised the block: phase B results are visible
-
-    // ---- phase C
-    for (int ln = grp; ln < kNodesPerBlock; ln += kGroups) {
-        const int i = nb0 + ln;
-        if (i >= p.n) break;
-        double P[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) P[k] = s_sum[ln][k];
-        const double rw = p.has_gravity ? s_sum[ln][9] : 0.0;
-        const int p0 = p.row_ptr[i], p1 = p.row_ptr[i + 1];
-        for (int q = p0 + lane; q < p1; q += kGroup) {
-            const int j = p.col[q];
-            const double g0 = p.dshp[3 * q], g1 = p.dshp[3 * q + 1], g2 = p.dshp[3 * q + 2];
-            atomicAdd(&p.fint[3 * j], P[0] * g0 + P[3] * g1 + P[6] * g2);
-            atomicAdd(&p.fint[3 * j + 1], P[1] * g0 + P[4] * g1 + P[7] * g2);
-            atomicAdd(&p.fint[3 * j + 2], P[2] * g0 + P[5] * g1 + P[8] * g2);
-            if (p.has_gravity) {
-                const double f = p.shp[q] * rw;
-                atomicAdd(&p.fext[3 * j], f * p.grav[0]);
-                atomicAdd(&p.fext[3 * j + 1], f * p.grav[1]);
-                atomicAdd(&p.fext[3 * j + 2], f * p.grav[2]);
+    f0 = group_sum<kGroup>(f0); f1 = group_sum<kGroup>(f1); f2 = group_sum<kGroup>(f2);
+    if (p.has_gravity) fe = group_sum<kGroup>(fe);
+    double ke = 0.0, ie = 0.0, ee = 0.0;
+    if (live && lane < 3) {
+        const int64_t q = 3 * (int64_t)node + lane;
+        const double f = (double)(lane == 0 ? f0 : (lane == 1 ? f1 : f2));
+        p.fint[q] = f;
+        double fx = p.has_gravity ? (double)fe * p.grav[lane] : 0.0;
+        if (!CORRECT) {
+            p.fext[q] = fx;
+        } else {
+            const double fx_old = p.has_gravity ? p.fext_nmo[q] : 0.0;
+            double v = p.v[q];
+            if (p.ebc[q] == 1) {
+                v = 0.0;
+                p.a[q] = 0.0;
+                fx = -f;
+            } else {
+                const double mass = p.mass[q];
+                const double c = p.mats[p.mat[node]].damping * mass;
+                const double acc = (fx - f - c * v) / (mass + 0.5 * p.dt * c);
+                p.a[q] = acc;
+                v = v + 0.5 * p.dt * acc;
             }
+            p.v[q] = v;
+            p.fext[q] = fx;
+            if (p.has_gravity) p.fext_nmo[q] = fx;
+            ke = 0.5 * p.mass[q] * v * v;
+            if (!p.finite_strain) ie = 0.5 * f * p.dtot[q];
+            ee = 0.5 * (fx + fx_old) * p.d[q];
+            if (PREDICT) predict_dof(p, q);
+        }
+    }
+    if (CORRECT) {
+        const double sk = block_sum(ke, s_red);
+        const double si = block_sum(ie, s_red);
+        const double se = block_sum(ee, s_red);
+        if (threadIdx.x == 0) {
+            if (sk != 0.0) atomicAdd(&p.acc[1], sk);
+            if (si != 0.0) atomicAdd(&p.acc[2], si);
+            if (se != 0.0) atomicAdd(&p.acc[3], se);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// EBC, mass-proportional damping and corrector (MAIN.F90:432-488), energies
-// (MAIN.F90:524-553).  One thread per degree of freedom.
+// transposed lists: trow[j] .. trow[j+1] hold the pairs (i -> j) in increasing
+// source i; tpos maps a pair's list position to its transposed position.
 
-__global__ __launch_bounds__(kBlock) void k_correct(Params p)
+__global__ void k_tcount(int64_t nnz, const int32_t *col, int32_t *cnt)
 {
-    __shared__ double red[kBlock / 64];
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double ke = 0.0, ie = 0.0, ee = 0.0;
-    if (q < 3 * (int64_t)p.n) {
-        const int i = (int)(q / 3);
-        const double f = p.fint[q];
-        const double fx_old = p.has_gravity ? p.fext_nmo[q] : 0.0;
-        double fx = p.has_gravity ? p.fext[q] : 0.0;
-        double v = p.v[q];
-        if (p.ebc[q] == 1) {
-            v = 0.0;
-            p.a[q] = 0.0;
-            fx = -f;
-        } else {
-            const double mass = p.mass[q];
-            const double c = p.mats[p.mat[i]].damping * mass;
-            const double acc = (fx - f - c * v) / (mass + 0.5 * p.dt * c);
-            p.a[q] = acc;
-            v = v + 0.5 * p.dt * acc;
-        }
-        p.v[q] = v;
-        p.fext[q] = fx;
-        if (p.has_gravity) p.fext_nmo[q] = fx;
-        ke = 0.5 * p.mass[q] * v * v;
-        if (!p.finite_strain) ie = 0.5 * f * p.dtot[q];
-        ee = 0.5 * (fx + fx_old) * p.d[q];
+    if (q < nnz) atomicAdd(&cnt[col[q]], 1);
+}
+
+__global__ void k_tfill(int n, const int32_t *row_ptr, const int32_t *col, int32_t *cursor, int32_t *tsrc, int32_t *tq)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+        const int t = atomicAdd(&cursor[col[q]], 1);
+        tsrc[t] = i;
+        tq[t] = q;
     }
-    const double sk = block_sum(ke, red);
-    const double si = block_sum(ie, red);
-    const double se = block_sum(ee, red);
-    if (threadIdx.x == 0) {
-        if (sk != 0.0) atomicAdd(&p.acc[1], sk);
-        if (si != 0.0) atomicAdd(&p.acc[2], si);
-        if (se != 0.0) atomicAdd(&p.acc[3], se);
+}
+
+__global__ void k_tsort(int n, const int32_t *trow, int32_t *tsrc, int32_t *tq, int32_t *tpos)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int s = trow[j], e = trow[j + 1];
+    for (int a = s + 1; a < e; ++a) {
+        const int vs = tsrc[a], vq = tq[a];
+        int b = a - 1;
+        while (b >= s && tsrc[b] > vs) { tsrc[b + 1] = tsrc[b]; tq[b + 1] = tq[b]; --b; }
+        tsrc[b + 1] = vs;
+        tq[b + 1] = vq;
     }
+    for (int a = s; a < e; ++a) tpos[tq[a]] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -581,7 +707,8 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, int32_t *ce
 // the reference's order (ix slowest, then iy, iz; BIN_SEARCH.F90:105-127), each
 // cell's nodes in node order; 64 candidates are tested at a time and the kept
 // ones are compacted with a ballot, so the list comes out in the reference's
-// order.  Criterion: x_i inside node j's support, |x_j - x_i|_k / a_jk < 1.
+// order.  Criterion: x_i inside node j's support, |x_j - x_i|_k / a_jk < 1
+// (box) or the ellipsoidal norm < 1 (BIN_SEARCH.F90:255-291).
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_search(int n, const double *x, const double *win, const int32_t *ijk,
                                                    BinGeom g, const int32_t *cell_start, const int32_t *cell_nodes,
@@ -604,8 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int n, const double *x, const
         }
         s_beg[wv][lane] = b;
     }
-    // exclusive prefix of the 27 counts across the wave
-    int incl = cnt;
+    int incl = cnt;   // inclusive prefix of the 27 counts across the wave
 #pragma unroll
     for (int o = 1; o < 32; o <<= 1) {
         const int t = __shfl_up(incl, o, 64);
@@ -645,8 +771,9 @@ __global__ __launch_bounds__(kBlock) void k_search(int n, const double *x, const
 }
 
 // ---------------------------------------------------------------------------
-// per-point constitution (context-free entry point)
+// per-point constitution (context-free entry point), FP64 or FP32 return map
 
+template <class R>
 __global__ void k_constitute(int n, const double *lmat, double *stress, double *strain, double *state,
                              Material mt, int finite, int32_t *fail)
 {
@@ -656,39 +783,26 @@ __global__ void k_constitute(int n, const double *lmat, double *stress, double *
     for (int k = 0; k < 9; ++k) L[k] = lmat[9 * i + k];
     for (int k = 0; k < 6; ++k) { sig[k] = stress[6 * i + k]; e6[k] = strain[6 * i + k]; }
     if (finite) {
-        double R[9];
-        mega::hughes_winget(L, R, de, dd);
-        mega::rotate_voigt(R, sig);
-        mega::rotate_voigt(R, e6);
+        double Rt[9];
+        mega::hughes_winget(L, Rt, de, dd);
+        mega::rotate_voigt(Rt, sig);
+        mega::rotate_voigt(Rt, e6);
     } else {
         de[0] = L[0]; de[1] = L[4]; de[2] = L[8];
         de[3] = L[5] + L[7]; de[4] = L[2] + L[6]; de[5] = L[1] + L[3];
     }
-    for (int k = 0; k < 6; ++k) e6[k] = e6[k] + de[k];
-    double pred[6];
-    mega::elastic_trial(sig, de, mt, pred);
-    double ep = state[20 * i + 2];
+    R sr[6], der[6], pred[6];
+    for (int k = 0; k < 6; ++k) { e6[k] = e6[k] + de[k]; sr[k] = (R)sig[k]; der[k] = (R)de[k]; }
+    mega::elastic_trial(sr, der, mt, pred);
+    R ep = (R)state[20 * i + 2];
     if (mt.type == 2) {
-        if (mega::von_mises(sig, pred, ep, mt) < 0) atomicAdd(fail, 1);
+        if (mega::von_mises(sr, pred, ep, mt) < 0) atomicAdd(fail, 1);
     } else {
-        for (int k = 0; k < 6; ++k) sig[k] = pred[k];
+        for (int k = 0; k < 6; ++k) sr[k] = pred[k];
     }
-    state[20 * i + 2] = ep;
-    for (int k = 0; k < 6; ++k) { stress[6 * i + k] = sig[k]; strain[6 * i + k] = e6[k]; }
+    state[20 * i + 2] = (double)ep;
+    for (int k = 0; k < 6; ++k) { stress[6 * i + k] = (double)sr[k]; strain[6 * i + k] = e6[k]; }
 }
-
-// ---------------------------------------------------------------------------
-// host side
-
-#define HIPCHK(expr)                                                      \
-    do {                                                                  \
-        hipError_t e_ = (expr);                                           \
-        if (e_ != hipSuccess) {                                           \
-            std::fprintf(stderr, "mega: %s failed: %s (%s:%d)\n", #expr, \
-                         hipGetErrorString(e_), __FILE__, __LINE__);      \
-            return MEGA_ERR_HIP;                                          \
-        }                                                                 \
-    } while (0)
 
 Material make_material(const double *prop, int type)
 {
@@ -704,19 +818,44 @@ Material make_material(const double *prop, int type)
 
 }  // namespace
 
+// ===========================================================================
+// context
+
 struct mega_ctx {
     mega_control c{};
     int n = 0;
-    Dev dv;
+    bool mixed = false;
+    size_t rsz = 8;   // bytes of R
+    // FP64 node arrays
+    double *X0 = nullptr, *x = nullptr, *v = nullptr, *a = nullptr, *d = nullptr, *dtot = nullptr, *uphy = nullptr;
+    double *fint = nullptr, *fext = nullptr, *fext_nmo = nullptr, *win = nullptr, *vol = nullptr, *mass = nullptr;
+    double *wave_vel = nullptr, *scratch = nullptr;   // scratch: 6n doubles
+    float *d32 = nullptr, *dtot32 = nullptr;
+    int32_t *ebc = nullptr, *ebc_node = nullptr, *mat = nullptr;
+    Material *mats = nullptr;
+    // R arrays
+    void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr;
+    void *shp = nullptr, *dshp = nullptr, *tdshp = nullptr, *tshp = nullptr;
+    // lists
+    int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tq = nullptr, *tpos = nullptr;
+    int32_t *counts = nullptr;
+    int64_t nnz = 0, nnz_cap = 0;
+    // bins
     BinGeom geom{};
     int64_t ncell = 0;
+    int32_t *ijk = nullptr, *cell_start = nullptr, *cell_nodes = nullptr, *cell_cursor = nullptr;
+    // misc
+    double *acc = nullptr;
+    int32_t *flags = nullptr;
+    void *cub_tmp = nullptr;
+    size_t cub_tmp_bytes = 0;
     hipStream_t stream = nullptr;
     int64_t step = 0;
     int search_count = 0;
-    int has_shapes = 0;
+    int has_lists = 0, has_shapes = 0, predicted = 0;
     double dt = 0.0;
     int has_gravity = 0;
-    std::vector<Material> mats;
+    std::vector<Material> host_mats;
     // profiling
     int profile = 0;
     struct Timer { const char *name; hipEvent_t a, b; };
@@ -726,30 +865,39 @@ struct mega_ctx {
 
 namespace {
 
+#define HIPCHK(expr)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (expr);                                           \
+        if (e_ != hipSuccess) {                                           \
+            std::fprintf(stderr, "mega: %s failed: %s (%s:%d)\n", #expr, \
+                         hipGetErrorString(e_), __FILE__, __LINE__);      \
+            return MEGA_ERR_HIP;                                          \
+        }                                                                 \
+    } while (0)
+
 template <class T>
-int dalloc(T **p, size_t count)
+int dalloc(T **p, size_t count, size_t elem = sizeof(T))
 {
     if (count == 0) count = 1;
-    if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess) return MEGA_ERR_NOMEM;
+    if (hipMalloc((void **)p, elem * count) != hipSuccess) { *p = nullptr; return MEGA_ERR_NOMEM; }
     return MEGA_OK;
 }
 
 Params params(mega_ctx *c)
 {
-    Dev &d = c->dv;
     Params p{};
     p.n = c->n;
-    p.row_ptr = d.row_ptr; p.col = d.col; p.shp = d.shp; p.dshp = d.dshp;
-    p.X0 = d.X0; p.win = d.win; p.vol = d.vol; p.mass = d.mass;
-    p.x = d.x; p.v = d.v; p.a = d.a; p.d = d.d; p.dtot = d.dtot; p.uphy = d.uphy;
-    p.fint = d.fint; p.fext = d.fext; p.fext_nmo = d.fext_nmo;
-    p.stress = d.stress; p.strain = d.strain; p.eps = d.eps;
-    p.strain_eq = d.strain_eq; p.wave_vel = d.wave_vel; p.vphy = d.vphy;
-    p.ebc = d.ebc; p.ebc_node = d.ebc_node; p.mat = d.mat; p.mats = d.mats;
-    p.acc = d.acc; p.flags = d.flags;
+    p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
+    p.shp = c->shp; p.dshp = c->dshp; p.tdshp = c->tdshp; p.tshp = c->tshp;
+    p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
+    p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
+    p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
+    p.d32 = c->d32; p.dtot32 = c->dtot32;
+    p.stress = c->stress; p.strain = c->strain; p.eps = c->eps; p.red = c->red; p.P = c->P;
+    p.ebc = c->ebc; p.ebc_node = c->ebc_node; p.mat = c->mat; p.mats = c->mats;
+    p.acc = c->acc; p.flags = c->flags;
     p.ql = c->c.ql; p.ql_coef = c->c.ql_coef; p.ql_len = c->c.ql_len;
-    p.shsup = c->c.shsup; p.lagrangian = c->c.lagrangian; p.finite_strain = c->c.finite_strain;
-    p.has_gravity = c->has_gravity;
+    p.shsup = c->c.shsup; p.finite_strain = c->c.finite_strain; p.has_gravity = c->has_gravity;
     for (int k = 0; k < 3; ++k) p.grav[k] = c->c.gravity[k];
     p.dt = c->dt;
     return p;
@@ -764,32 +912,52 @@ struct Scope {
         if (!c->profile) return;
         if (c->timer_used == c->timers.size()) {
             mega_ctx::Timer nt{name, nullptr, nullptr};
-            hipEventCreate(&nt.a);
-            hipEventCreate(&nt.b);
+            if (hipEventCreate(&nt.a) != hipSuccess || hipEventCreate(&nt.b) != hipSuccess) return;
             c->timers.push_back(nt);
         }
         t = &c->timers[c->timer_used++];
         t->name = name;
-        hipEventRecord(t->a, c->stream);
+        (void)hipEventRecord(t->a, c->stream);
     }
     ~Scope()
     {
-        if (t) hipEventRecord(t->b, c->stream);
+        if (t) (void)hipEventRecord(t->b, c->stream);
     }
 };
 
 unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
 
+// grid for the XCD-mapped pair kernels (kBlock/kGroup nodes per block)
+unsigned xcd_grid(int n, int *nblocks)
+{
+    *nblocks = (n + kBlock / kGroup - 1) / (kBlock / kGroup);
+    const int per = (*nblocks + kXcd - 1) / kXcd;
+    return (unsigned)(per * kXcd);
+}
+
+void free_pairs(mega_ctx *c)
+{
+    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tq, c->tpos};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c->col = c->tsrc = c->tq = c->tpos = nullptr;
+    c->shp = c->dshp = c->tdshp = c->tshp = nullptr;
+    c->nnz_cap = 0;
+}
+
 int ensure_pairs(mega_ctx *c, int64_t nnz)
 {
-    Dev &d = c->dv;
     if (nnz > INT32_MAX) return MEGA_ERR_ARG;
-    if (nnz <= d.nnz_cap) return MEGA_OK;
-    hipFree(d.col); hipFree(d.shp); hipFree(d.dshp);
-    d.col = nullptr; d.shp = nullptr; d.dshp = nullptr;
+    if (nnz <= c->nnz_cap) return MEGA_OK;
+    free_pairs(c);
     const int64_t cap = nnz + nnz / 8 + 1024;
-    if (dalloc(&d.col, cap) || dalloc(&d.shp, cap) || dalloc(&d.dshp, 3 * cap)) return MEGA_ERR_NOMEM;
-    d.nnz_cap = cap;
+    const size_t r = c->rsz;
+    int e = 0;
+    e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tq, cap); e |= dalloc(&c->tpos, cap);
+    e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r); e |= dalloc(&c->tdshp, 3 * cap, r);
+    if (c->has_gravity) e |= dalloc(&c->tshp, cap, r);
+    if (e) { free_pairs(c); return MEGA_ERR_NOMEM; }
+    c->nnz_cap = cap;
     c->has_shapes = 0;
     return MEGA_OK;
 }
@@ -797,57 +965,75 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
 int cub_scan(mega_ctx *c, const int32_t *in, int32_t *out, int64_t count)
 {
     size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, c->stream);
-    if (bytes > c->dv.cub_tmp_bytes) {
-        hipFree(c->dv.cub_tmp);
-        if (hipMalloc(&c->dv.cub_tmp, bytes) != hipSuccess) return MEGA_ERR_NOMEM;
-        c->dv.cub_tmp_bytes = bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, c->stream));
+    if (bytes > c->cub_tmp_bytes) {
+        (void)hipFree(c->cub_tmp);
+        c->cub_tmp = nullptr;
+        if (hipMalloc(&c->cub_tmp, bytes) != hipSuccess) return MEGA_ERR_NOMEM;
+        c->cub_tmp_bytes = bytes;
     }
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->dv.cub_tmp, bytes, in, out, (int)count, c->stream));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, in, out, (int)count, c->stream));
     return MEGA_OK;
 }
 
 int do_bins(mega_ctx *c, const double *x)
 {
-    Dev &d = c->dv;
     const int n = c->n;
     Scope s(c, "k_bin");
-    HIPCHK(hipMemsetAsync(d.cell_cursor, 0, sizeof(int32_t) * (c->ncell + 1), c->stream));
-    k_bin_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, x, c->geom, d.ijk, d.cell_cursor);
-    int r = cub_scan(c, d.cell_cursor, d.cell_start, c->ncell + 1);
+    HIPCHK(hipMemsetAsync(c->cell_cursor, 0, sizeof(int32_t) * (c->ncell + 1), c->stream));
+    k_bin_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, x, c->geom, c->ijk, c->cell_cursor);
+    int r = cub_scan(c, c->cell_cursor, c->cell_start, c->ncell + 1);
     if (r) return r;
-    HIPCHK(hipMemcpyAsync(d.cell_cursor, d.cell_start, sizeof(int32_t) * c->ncell, hipMemcpyDeviceToDevice, c->stream));
-    k_bin_fill<<<grid_for(n), kBlock, 0, c->stream>>>(n, d.ijk, c->geom, d.cell_cursor, d.cell_nodes);
-    k_bin_sort<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, d.cell_start, d.cell_nodes);
+    HIPCHK(hipMemcpyAsync(c->cell_cursor, c->cell_start, sizeof(int32_t) * c->ncell, hipMemcpyDeviceToDevice, c->stream));
+    k_bin_fill<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->ijk, c->geom, c->cell_cursor, c->cell_nodes);
+    k_bin_sort<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->cell_nodes);
+    HIPCHK(hipGetLastError());
+    return MEGA_OK;
+}
+
+int do_transpose(mega_ctx *c)
+{
+    const int n = c->n;
+    Scope s(c, "k_transpose");
+    HIPCHK(hipMemsetAsync(c->counts, 0, sizeof(int32_t) * (n + 1), c->stream));
+    k_tcount<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->col, c->counts);
+    int r = cub_scan(c, c->counts, c->trow, n + 1);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(c->counts, c->trow, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+    k_tfill<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->row_ptr, c->col, c->counts, c->tsrc, c->tq);
+    k_tsort<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->trow, c->tsrc, c->tq, c->tpos);
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }
 
 int do_search(mega_ctx *c, const double *x)
 {
-    Dev &d = c->dv;
     const int n = c->n;
     {
         Scope s(c, "k_search_count");
         k_search<false><<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(
-            n, x, d.win, d.ijk, c->geom, d.cell_start, d.cell_nodes, c->c.shsup, d.counts, nullptr, nullptr);
+            n, x, c->win, c->ijk, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts, nullptr, nullptr);
     }
-    HIPCHK(hipMemsetAsync(d.counts + n, 0, sizeof(int32_t), c->stream));
-    int r = cub_scan(c, d.counts, d.row_ptr, n + 1);
+    HIPCHK(hipMemsetAsync(c->counts + n, 0, sizeof(int32_t), c->stream));
+    int r = cub_scan(c, c->counts, c->row_ptr, n + 1);
     if (r) return r;
     int32_t nnz32 = 0;
-    HIPCHK(hipMemcpyAsync(&nnz32, d.row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&nnz32, c->row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (nnz32 < 0) return MEGA_ERR_ARG;
     r = ensure_pairs(c, nnz32);
     if (r) return r;
-    d.nnz = nnz32;
+    c->nnz = nnz32;
     {
         Scope s(c, "k_search_fill");
         k_search<true><<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(
-            n, x, d.win, d.ijk, c->geom, d.cell_start, d.cell_nodes, c->c.shsup, nullptr, d.row_ptr, d.col);
+            n, x, c->win, c->ijk, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr, c->row_ptr, c->col);
     }
     HIPCHK(hipGetLastError());
+    r = do_transpose(c);
+    if (r) return r;
+    c->has_lists = 1;
+    c->has_shapes = 0;
     return MEGA_OK;
 }
 
@@ -867,23 +1053,67 @@ int search_step(mega_ctx *c, const double *x, int first)
 
 int do_shapes(mega_ctx *c, const double *xs)
 {
+    if (!c->has_lists) return MEGA_ERR_ARG;
     Params p = params(c);
     Scope s(c, "k_rk");
-    k_rk<<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, xs);
+    if (c->mixed) k_rk<float><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, xs);
+    else k_rk<double><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, xs);
     HIPCHK(hipGetLastError());
     c->has_shapes = 1;
     return MEGA_OK;
 }
 
-int do_fint(mega_ctx *c, int push, int fuse_interp)
+template <class R>
+int fint_kernels(mega_ctx *c, bool push, bool update_x, int force_mode)
 {
     Params p = params(c);
-    const unsigned grid = (unsigned)((c->n + kNodesPerBlock - 1) / kNodesPerBlock);
-    Scope s(c, "k_fint");
-    if (push && fuse_interp) k_fint<true, true><<<grid, kBlock, 0, c->stream>>>(p);
-    else if (push) k_fint<true, false><<<grid, kBlock, 0, c->stream>>>(p);
-    else if (fuse_interp) k_fint<false, true><<<grid, kBlock, 0, c->stream>>>(p);
-    else k_fint<false, false><<<grid, kBlock, 0, c->stream>>>(p);
+    int nb = 0;
+    const unsigned g = xcd_grid(c->n, &nb);
+    {
+        Scope s(c, "k_gather");
+        if (push && update_x) k_gather<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else if (push) k_gather<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else if (update_x) k_gather<R, false, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else k_gather<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+    }
+    {
+        Scope s(c, "k_node");
+        if (push) k_node<R, true><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+        else k_node<R, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+    }
+    if (force_mode >= 0) {
+        Scope s(c, "k_force");
+        if (force_mode == 0) k_force<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else if (force_mode == 1) k_force<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else k_force<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+    }
+    HIPCHK(hipGetLastError());
+    return MEGA_OK;
+}
+
+// force_mode: -1 no force, 0 force only, 1 force + corrector, 2 + next predictor
+int do_fint(mega_ctx *c, bool push, bool update_x, int force_mode)
+{
+    if (!c->has_shapes) return MEGA_ERR_ARG;
+    return c->mixed ? fint_kernels<float>(c, push, update_x, force_mode)
+                    : fint_kernels<double>(c, push, update_x, force_mode);
+}
+
+int do_predict(mega_ctx *c)
+{
+    Params p = params(c);
+    Scope s(c, "k_predict");
+    k_predict<<<grid_for(3 * (int64_t)c->n), kBlock, 0, c->stream>>>(p);
+    HIPCHK(hipGetLastError());
+    return MEGA_OK;
+}
+
+int sync_d32(mega_ctx *c)
+{
+    if (!c->mixed) return MEGA_OK;
+    const int64_t m = 3 * (int64_t)c->n;
+    k_cast32<<<grid_for(m), kBlock, 0, c->stream>>>(m, c->d, c->d32);
+    k_cast32<<<grid_for(m), kBlock, 0, c->stream>>>(m, c->dtot, c->dtot32);
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }
@@ -891,34 +1121,34 @@ int do_fint(mega_ctx *c, int push, int fuse_interp)
 int check_flags(mega_ctx *c)
 {
     int32_t f[2];
-    HIPCHK(hipMemcpyAsync(f, c->dv.flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (f[1]) return MEGA_ERR_MATERIAL;
     if (f[0]) return MEGA_ERR_RETURN_MAP;
     return MEGA_OK;
 }
 
-double *field_ptr(mega_ctx *c, int f, int *width)
+// copy an R-typed state array (m values) to/from FP64 host memory
+int state_out(mega_ctx *c, const void *src, int64_t m, double *host)
 {
-    Dev &d = c->dv;
-    *width = 3;
-    switch (f) {
-    case MEGA_F_X: return d.x;
-    case MEGA_F_V: return d.v;
-    case MEGA_F_A: return d.a;
-    case MEGA_F_D: return d.d;
-    case MEGA_F_DTOT: return d.dtot;
-    case MEGA_F_UPHY: return d.uphy;
-    case MEGA_F_FINT: return d.fint;
-    case MEGA_F_FEXT: return d.fext;
-    case MEGA_F_STRESS: *width = 6; return d.stress;
-    case MEGA_F_STRAIN: *width = 6; return d.strain;
-    case MEGA_F_EPS: *width = 1; return d.eps;
-    case MEGA_F_STRAIN_EQ: *width = 1; return d.strain_eq;
-    case MEGA_F_WAVE_VEL: *width = 1; return d.wave_vel;
-    case MEGA_F_VPHY: return d.vphy;
-    default: return nullptr;
+    if (!c->mixed) {
+        HIPCHK(hipMemcpyAsync(host, src, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)src, c->scratch);
+        HIPCHK(hipMemcpyAsync(host, c->scratch, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
     }
+    return MEGA_OK;
+}
+
+int state_in(mega_ctx *c, void *dst, int64_t m, const double *host)
+{
+    if (!c->mixed) {
+        HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->scratch, host, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+        k_narrow<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, c->scratch, (float *)dst);
+    }
+    return MEGA_OK;
 }
 
 }  // namespace
@@ -950,12 +1180,15 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (!ctrl || !out || n <= 0 || !X0 || !win || !vol || !mass || !ebc || !ebc_node || !mat_type || !prop)
         return MEGA_ERR_ARG;
     if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3 || ctrl->itype_int != 0) return MEGA_ERR_ARG;
-    if (ctrl->precision != MEGA_FP64) return MEGA_ERR_ARG;
+    if (ctrl->precision != MEGA_FP64 && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
+    if (ctrl->surrogate) return MEGA_ERR_ARG;
     *out = nullptr;
     mega_ctx *c = new mega_ctx();
     c->c = *ctrl;
     if (c->c.pdsearch < 1) c->c.pdsearch = 1;
     c->n = n;
+    c->mixed = ctrl->precision == MEGA_MIXED;
+    c->rsz = c->mixed ? 4 : 8;
     c->has_gravity = (ctrl->gravity[0] != 0.0 || ctrl->gravity[1] != 0.0 || ctrl->gravity[2] != 0.0);
     // materials: nodes with identical property rows share a slot
     std::vector<int32_t> mid(n);
@@ -965,9 +1198,9 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         row.push_back((double)mat_type[i]);
         auto it = seen.find(row);
         if (it == seen.end()) {
-            const int k = (int)c->mats.size();
+            const int k = (int)c->host_mats.size();
             seen.emplace(row, k);
-            c->mats.push_back(make_material(prop + 30 * (int64_t)i, mat_type[i]));
+            c->host_mats.push_back(make_material(prop + 30 * (int64_t)i, mat_type[i]));
             mid[i] = k;
         } else {
             mid[i] = it->second;
@@ -992,43 +1225,42 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         c->geom.bin[k] = bin[k];
     }
     c->ncell = (int64_t)c->geom.nb[0] * c->geom.nb[1] * c->geom.nb[2];
-    if (c->ncell <= 0 || c->ncell > INT32_MAX) { delete c; return MEGA_ERR_ARG; }
+    if (c->ncell <= 0 || c->ncell >= INT32_MAX) { delete c; return MEGA_ERR_ARG; }
 
-    Dev &d = c->dv;
-    d.n = n;
-    const size_t n3 = 3 * (size_t)n, n6 = 6 * (size_t)n;
-    int r = 0;
-    r |= dalloc(&d.X0, n3); r |= dalloc(&d.x, n3); r |= dalloc(&d.v, n3); r |= dalloc(&d.a, n3);
-    r |= dalloc(&d.d, n3); r |= dalloc(&d.dtot, n3); r |= dalloc(&d.uphy, n3); r |= dalloc(&d.fint, n3);
-    r |= dalloc(&d.fext, n3); r |= dalloc(&d.vphy, n3); r |= dalloc(&d.scratch3, n3);
-    if (c->has_gravity) r |= dalloc(&d.fext_nmo, n3);
-    r |= dalloc(&d.stress, n6); r |= dalloc(&d.strain, n6); r |= dalloc(&d.eps, n);
-    r |= dalloc(&d.win, n3); r |= dalloc(&d.vol, n); r |= dalloc(&d.mass, n3);
-    r |= dalloc(&d.strain_eq, n); r |= dalloc(&d.wave_vel, n);
-    r |= dalloc(&d.ebc, n3); r |= dalloc(&d.ebc_node, n); r |= dalloc(&d.mat, n);
-    r |= dalloc(&d.mats, c->mats.size());
-    r |= dalloc(&d.row_ptr, n + 1); r |= dalloc(&d.counts, n + 1);
-    r |= dalloc(&d.ijk, n3); r |= dalloc(&d.cell_start, c->ncell + 1); r |= dalloc(&d.cell_cursor, c->ncell + 1);
-    r |= dalloc(&d.cell_nodes, n);
-    r |= dalloc(&d.acc, 8); r |= dalloc(&d.flags, 4);
-    if (r) { mega_destroy(c); return MEGA_ERR_NOMEM; }
+    const size_t n3 = 3 * (size_t)n, n6 = 6 * (size_t)n, r = c->rsz;
+    int e = 0;
+    e |= dalloc(&c->X0, n3); e |= dalloc(&c->x, n3); e |= dalloc(&c->v, n3); e |= dalloc(&c->a, n3);
+    e |= dalloc(&c->d, n3); e |= dalloc(&c->dtot, n3); e |= dalloc(&c->uphy, n3); e |= dalloc(&c->fint, n3);
+    e |= dalloc(&c->fext, n3); e |= dalloc(&c->scratch, n6);
+    if (c->has_gravity) e |= dalloc(&c->fext_nmo, n3);
+    if (c->mixed) { e |= dalloc(&c->d32, n3); e |= dalloc(&c->dtot32, n3); }
+    else e |= dalloc(&c->wave_vel, n);
+    e |= dalloc(&c->win, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
+    e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
+    e |= dalloc(&c->mats, c->host_mats.size());
+    e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
+    e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, kPStride * (size_t)n, r);
+    e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
+    e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
+    e |= dalloc(&c->cell_nodes, n);
+    e |= dalloc(&c->acc, 8); e |= dalloc(&c->flags, 4);
+    if (e) { mega_destroy(c); return MEGA_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { mega_destroy(c); return MEGA_ERR_HIP; }
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    bool ok = up(d.X0, X0, n3 * 8) && up(d.x, X0, n3 * 8) && up(d.win, win, n3 * 8) && up(d.vol, vol, (size_t)n * 8) &&
-              up(d.mass, mass, n3 * 8) && up(d.ebc, ebc, n3 * 4) && up(d.ebc_node, ebc_node, (size_t)n * 4) &&
-              up(d.mat, mid.data(), (size_t)n * 4) && up(d.mats, c->mats.data(), c->mats.size() * sizeof(Material));
-    if (vinit) ok = ok && up(d.v, vinit, n3 * 8);
-    else ok = ok && hipMemset(d.v, 0, n3 * 8) == hipSuccess;
-    ok = ok && hipMemset(d.a, 0, n3 * 8) == hipSuccess && hipMemset(d.d, 0, n3 * 8) == hipSuccess &&
-         hipMemset(d.dtot, 0, n3 * 8) == hipSuccess && hipMemset(d.uphy, 0, n3 * 8) == hipSuccess &&
-         hipMemset(d.fint, 0, n3 * 8) == hipSuccess && hipMemset(d.fext, 0, n3 * 8) == hipSuccess &&
-         hipMemset(d.vphy, 0, n3 * 8) == hipSuccess && hipMemset(d.stress, 0, n6 * 8) == hipSuccess &&
-         hipMemset(d.strain, 0, n6 * 8) == hipSuccess && hipMemset(d.eps, 0, (size_t)n * 8) == hipSuccess &&
-         hipMemset(d.strain_eq, 0, (size_t)n * 8) == hipSuccess && hipMemset(d.wave_vel, 0, (size_t)n * 8) == hipSuccess &&
-         hipMemset(d.acc, 0, 64) == hipSuccess && hipMemset(d.flags, 0, 16) == hipSuccess;
-    if (ok && d.fext_nmo) ok = hipMemset(d.fext_nmo, 0, n3 * 8) == hipSuccess;
+    auto zero = [&](void *dst, size_t bytes) { return hipMemset(dst, 0, bytes) == hipSuccess; };
+    bool ok = up(c->X0, X0, n3 * 8) && up(c->x, X0, n3 * 8) && up(c->win, win, n3 * 8) &&
+              up(c->vol, vol, (size_t)n * 8) && up(c->mass, mass, n3 * 8) && up(c->ebc, ebc, n3 * 4) &&
+              up(c->ebc_node, ebc_node, (size_t)n * 4) && up(c->mat, mid.data(), (size_t)n * 4) &&
+              up(c->mats, c->host_mats.data(), c->host_mats.size() * sizeof(Material));
+    ok = ok && (vinit ? up(c->v, vinit, n3 * 8) : zero(c->v, n3 * 8));
+    ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
+         zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
+         zero(c->eps, n * r) && zero(c->acc, 64) && zero(c->flags, 16);
+    if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
+    if (ok && c->mixed) ok = zero(c->d32, n3 * 4) && zero(c->dtot32, n3 * 4);
+    if (ok && c->wave_vel) ok = zero(c->wave_vel, (size_t)n * 8);
     if (!ok) { mega_destroy(c); return MEGA_ERR_HIP; }
     *out = c;
     return MEGA_OK;
@@ -1037,16 +1269,16 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
 int mega_destroy(mega_ctx *c)
 {
     if (!c) return MEGA_OK;
-    Dev &d = c->dv;
-    void *ptrs[] = {d.X0, d.x, d.v, d.a, d.d, d.dtot, d.uphy, d.fint, d.fext, d.fext_nmo, d.stress, d.strain,
-                    d.eps, d.win, d.vol, d.mass, d.strain_eq, d.wave_vel, d.vphy, d.scratch3, d.ebc, d.ebc_node,
-                    d.mat, d.mats, d.row_ptr, d.col, d.shp, d.dshp, d.ijk, d.cell_start, d.cell_nodes,
-                    d.cell_cursor, d.counts, d.acc, d.flags, d.cub_tmp};
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_pairs(c);
+    void *ptrs[] = {c->X0, c->x, c->v, c->a, c->d, c->dtot, c->uphy, c->fint, c->fext, c->fext_nmo, c->win,
+                    c->vol, c->mass, c->wave_vel, c->scratch, c->d32, c->dtot32, c->ebc, c->ebc_node, c->mat,
+                    c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
+                    c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->acc, c->flags, c->cub_tmp};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    for (auto &t : c->timers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MEGA_OK;
 }
@@ -1054,10 +1286,10 @@ int mega_destroy(mega_ctx *c)
 int mega_search(mega_ctx *c, const double *x_cur, int32_t rebuild_bins)
 {
     if (!c) return MEGA_ERR_ARG;
-    const double *xs = c->dv.x;
+    const double *xs = c->x;
     if (x_cur) {
-        HIPCHK(hipMemcpyAsync(c->dv.scratch3, x_cur, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
-        xs = c->dv.scratch3;
+        HIPCHK(hipMemcpyAsync(c->scratch, x_cur, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
+        xs = c->scratch;
     }
     if (rebuild_bins || c->search_count == 0) {
         int r = do_bins(c, xs);
@@ -1072,10 +1304,10 @@ int mega_search(mega_ctx *c, const double *x_cur, int32_t rebuild_bins)
 
 int mega_get_neighbors(mega_ctx *c, int32_t *gn, int32_t *gstart, int32_t *gstack, int64_t cap, int64_t *nnz)
 {
-    if (!c) return MEGA_ERR_ARG;
+    if (!c || !c->has_lists) return MEGA_ERR_ARG;
     const int n = c->n;
     std::vector<int32_t> rp(n + 1);
-    HIPCHK(hipMemcpyAsync(rp.data(), c->dv.row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(rp.data(), c->row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (nnz) *nnz = rp[n];
     if (gn)
@@ -1084,7 +1316,7 @@ int mega_get_neighbors(mega_ctx *c, int32_t *gn, int32_t *gstart, int32_t *gstac
         for (int i = 0; i < n; ++i) gstart[i] = rp[i] + 1;
     if (gstack) {
         if (cap < rp[n]) return MEGA_ERR_ARG;
-        HIPCHK(hipMemcpyAsync(gstack, c->dv.col, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(gstack, c->col, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int64_t q = 0; q < rp[n]; ++q) gstack[q] += 1;
     }
@@ -1093,11 +1325,11 @@ int mega_get_neighbors(mega_ctx *c, int32_t *gn, int32_t *gstart, int32_t *gstac
 
 int mega_shapes(mega_ctx *c, const double *x_eval)
 {
-    if (!c || c->dv.nnz_cap == 0) return MEGA_ERR_ARG;
-    const double *xs = c->dv.X0;
+    if (!c || !c->has_lists) return MEGA_ERR_ARG;
+    const double *xs = c->X0;
     if (x_eval) {
-        HIPCHK(hipMemcpyAsync(c->dv.scratch3, x_eval, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
-        xs = c->dv.scratch3;
+        HIPCHK(hipMemcpyAsync(c->scratch, x_eval, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
+        xs = c->scratch;
     }
     int r = do_shapes(c, xs);
     if (r) return r;
@@ -1107,75 +1339,90 @@ int mega_shapes(mega_ctx *c, const double *x_eval)
 
 int mega_get_shapes(mega_ctx *c, double *shp, double *dshp, int64_t cap)
 {
-    if (!c || cap < c->dv.nnz) return MEGA_ERR_ARG;
-    if (shp) HIPCHK(hipMemcpyAsync(shp, c->dv.shp, sizeof(double) * c->dv.nnz, hipMemcpyDeviceToHost, c->stream));
-    if (dshp) HIPCHK(hipMemcpyAsync(dshp, c->dv.dshp, sizeof(double) * 3 * c->dv.nnz, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return MEGA_OK;
+    if (!c || !c->has_shapes || cap < c->nnz) return MEGA_ERR_ARG;
+    const int64_t m = c->nnz;
+    double *buf = nullptr;
+    if (dalloc(&buf, 3 * (size_t)m)) return MEGA_ERR_NOMEM;
+    int st = MEGA_OK;
+    if (shp) {
+        if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)c->shp, buf);
+        else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)c->shp, buf);
+        if (hipMemcpyAsync(shp, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            st = MEGA_ERR_HIP;
+    }
+    if (dshp && !st) {
+        if (c->mixed) k_widen<float><<<grid_for(3 * m), kBlock, 0, c->stream>>>(3 * m, (const float *)c->dshp, buf);
+        else k_widen<double><<<grid_for(3 * m), kBlock, 0, c->stream>>>(3 * m, (const double *)c->dshp, buf);
+        if (hipMemcpyAsync(dshp, buf, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            st = MEGA_ERR_HIP;
+    }
+    (void)hipFree(buf);
+    return st;
 }
 
 int mega_interp(mega_ctx *c, const double *coef, double *phys)
 {
     if (!c || !coef || !phys || !c->has_shapes) return MEGA_ERR_ARG;
-    Dev &d = c->dv;
     const size_t b = sizeof(double) * 3 * c->n;
     double *out = nullptr;
     if (dalloc(&out, 3 * (size_t)c->n)) return MEGA_ERR_NOMEM;
     int st = MEGA_OK;
-    if (hipMemcpyAsync(d.scratch3, coef, b, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = MEGA_ERR_HIP;
+    if (hipMemcpyAsync(c->scratch, coef, b, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = MEGA_ERR_HIP;
     if (!st) {
         Params p = params(c);
-        k_interp<false><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, d.scratch3, out);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(phys, out, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        if (c->mixed) k_interp<float, false><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->scratch, out);
+        else k_interp<double, false><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->scratch, out);
+        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(phys, out, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             st = MEGA_ERR_HIP;
     }
-    hipFree(out);
+    (void)hipFree(out);
     return st;
 }
 
 int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc_tot, const double *x_cur,
               double *stress, double *strain, double *state, double *fint, double *fext, double *int_work)
 {
-    if (!c || !dinc || !dinc_tot || !stress || !strain || !state || !fint || !fext || c->dv.nnz_cap == 0)
+    if (!c || !dinc || !dinc_tot || !stress || !strain || !state || !fint || !fext || !c->has_lists)
         return MEGA_ERR_ARG;
-    Dev &d = c->dv;
     const int n = c->n;
-    const size_t b3 = sizeof(double) * 3 * n, b6 = sizeof(double) * 6 * n;
+    const size_t b3 = sizeof(double) * 3 * n;
     std::vector<double> eps(n);
     for (int i = 0; i < n; ++i) eps[i] = state[20 * (int64_t)i + 2];
-    HIPCHK(hipMemcpyAsync(d.d, dinc, b3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d.dtot, dinc_tot, b3, hipMemcpyHostToDevice, c->stream));
-    if (x_cur) HIPCHK(hipMemcpyAsync(d.x, x_cur, b3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d.stress, stress, b6, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d.strain, strain, b6, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d.eps, eps.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(d.fint, 0, b3, c->stream));
-    HIPCHK(hipMemsetAsync(d.fext, 0, b3, c->stream));
-    HIPCHK(hipMemsetAsync(d.acc, 0, sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(d.flags, 0, 16, c->stream));
-    int r = 0;
+    HIPCHK(hipMemcpyAsync(c->d, dinc, b3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dtot, dinc_tot, b3, hipMemcpyHostToDevice, c->stream));
+    if (x_cur) HIPCHK(hipMemcpyAsync(c->x, x_cur, b3, hipMemcpyHostToDevice, c->stream));
+    int r = sync_d32(c);
+    if (!r) r = state_in(c, c->stress, 6 * (int64_t)n, stress);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_in(c, c->strain, 6 * (int64_t)n, strain);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_in(c, c->eps, n, eps.data());
+    if (r) return r;
+    HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
     const bool lag = c->c.lagrangian;
-    if (linit || !lag) {
-        r = do_shapes(c, lag ? d.X0 : d.x);
-        if (r) return r;
-    } else if (!c->has_shapes) {
-        return MEGA_ERR_ARG;
-    }
-    r = do_fint(c, lag && c->c.finite_strain, 0);
+    if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x);
+    else if (!c->has_shapes) r = MEGA_ERR_ARG;
+    if (!r) r = do_fint(c, lag && c->c.finite_strain, false, 0);
     if (r) return r;
     const int st = check_flags(c);
     double w = 0.0;
-    HIPCHK(hipMemcpyAsync(&w, d.acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(stress, d.stress, b6, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(strain, d.strain, b6, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(eps.data(), d.eps, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(fint, d.fint, b3, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(fext, d.fext, b3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&w, c->acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    r = state_out(c, c->stress, 6 * (int64_t)n, stress);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_out(c, c->strain, 6 * (int64_t)n, strain);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_out(c, c->eps, n, eps.data());
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(fint, c->fint, b3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fext, c->fext, b3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < n; ++i) state[20 * (int64_t)i + 2] = eps[i];
     if (int_work) *int_work += w;
+    c->predicted = 0;
     return st;
 }
 
@@ -1184,13 +1431,13 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
 {
     if (n <= 0 || !lmat || !stress || !strain || !state || !prop) return MEGA_ERR_ARG;
     if (mat_type != 1 && mat_type != 2) return MEGA_ERR_MATERIAL;
-    if (precision != MEGA_FP64) return MEGA_ERR_ARG;
+    if (precision != MEGA_FP64 && precision != MEGA_MIXED) return MEGA_ERR_ARG;
     const Material mt = make_material(prop, mat_type);
     double *dl = nullptr, *ds = nullptr, *de = nullptr, *dst = nullptr;
     int32_t *fail = nullptr;
     const size_t nn = (size_t)n;
     if (dalloc(&dl, 9 * nn) || dalloc(&ds, 6 * nn) || dalloc(&de, 6 * nn) || dalloc(&dst, 20 * nn) || dalloc(&fail, 1)) {
-        hipFree(dl); hipFree(ds); hipFree(de); hipFree(dst); hipFree(fail);
+        (void)hipFree(dl); (void)hipFree(ds); (void)hipFree(de); (void)hipFree(dst); (void)hipFree(fail);
         return MEGA_ERR_NOMEM;
     }
     int st = MEGA_OK;
@@ -1201,7 +1448,8 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
         hipMemcpy(dst, state, 160 * nn, hipMemcpyHostToDevice) != hipSuccess || hipMemset(fail, 0, 4) != hipSuccess) {
         st = MEGA_ERR_HIP;
     } else {
-        k_constitute<<<grid_for(n), kBlock>>>(n, dl, ds, de, dst, mt, finite_strain, fail);
+        if (precision == MEGA_MIXED) k_constitute<float><<<grid_for(n), kBlock>>>(n, dl, ds, de, dst, mt, finite_strain, fail);
+        else k_constitute<double><<<grid_for(n), kBlock>>>(n, dl, ds, de, dst, mt, finite_strain, fail);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(stress, ds, 48 * nn, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(strain, de, 48 * nn, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -1211,7 +1459,7 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
         else if (f)
             st = MEGA_ERR_RETURN_MAP;
     }
-    hipFree(dl); hipFree(ds); hipFree(de); hipFree(dst); hipFree(fail);
+    (void)hipFree(dl); (void)hipFree(ds); (void)hipFree(de); (void)hipFree(dst); (void)hipFree(fail);
     return st;
 }
 
@@ -1220,60 +1468,94 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
     if (!c || nsteps < 0 || !(dt > 0.0)) return MEGA_ERR_ARG;
     c->dt = dt;
     if (c->profile) c->timer_used = 0;
-    Dev &d = c->dv;
     const int n = c->n;
     const bool lag = c->c.lagrangian;
     const bool push = lag && c->c.finite_strain;
-    HIPCHK(hipMemsetAsync(d.flags, 0, 16, c->stream));
+    HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         const bool first = (c->step == 0);
+        const bool last = (s == nsteps - 1);
         int r = 0;
         if (first) {
             // LINIT call of HANDELER with zero increments (MAIN.F90:320-334)
-            HIPCHK(hipMemsetAsync(d.d, 0, sizeof(double) * 3 * n, c->stream));
-            r = search_step(c, d.x, 1);
-            if (!r) r = do_shapes(c, lag ? d.X0 : d.x);
-            if (!r) r = do_fint(c, push, 0);
+            HIPCHK(hipMemsetAsync(c->d, 0, sizeof(double) * 3 * n, c->stream));
+            r = sync_d32(c);
+            if (!r) r = search_step(c, c->x, 1);
+            if (!r) r = do_shapes(c, lag ? c->X0 : c->x);
+            if (!r) r = do_fint(c, push, false, -1);
             if (r) return r;
         }
-        Params p = params(c);
-        {
-            Scope sc(c, "k_predict");
-            k_predict<<<grid_for(3 * (int64_t)n), kBlock, 0, c->stream>>>(p);
-        }
+        // the predictor is fused into the previous k_force within one call
+        if (!c->predicted) r = do_predict(c);
+        if (r) return r;
         if (lag) {
-            r = do_fint(c, push, 1);   // DO_INTERP fused into the gather phase
+            r = do_fint(c, push, true, last ? 1 : 2);   // DO_INTERP fused into the gather
         } else {
+            Params p = params(c);
             {
                 Scope sc(c, "k_interp");
-                k_interp<true><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, d.d, nullptr);
+                if (c->mixed) k_interp<float, true><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, nullptr, nullptr);
+                else k_interp<double, true><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, nullptr, nullptr);
             }
-            r = search_step(c, d.x, first);
-            if (!r) r = do_shapes(c, d.x);
-            if (!r) r = do_fint(c, false, 0);
+            r = search_step(c, c->x, first);
+            if (!r) r = do_shapes(c, c->x);
+            if (!r) r = do_fint(c, false, false, last ? 1 : 2);
         }
         if (r) return r;
-        {
-            Scope sc(c, "k_correct");
-            k_correct<<<grid_for(3 * (int64_t)n), kBlock, 0, c->stream>>>(p);
-        }
+        c->predicted = last ? 0 : 1;
         HIPCHK(hipGetLastError());
         c->step++;
     }
     return check_flags(c);
 }
 
+static double *field_ptr(mega_ctx *c, int f, int *width)
+{
+    *width = 3;
+    switch (f) {
+    case MEGA_F_X: return c->x;
+    case MEGA_F_V: return c->v;
+    case MEGA_F_A: return c->a;
+    case MEGA_F_D: return c->d;
+    case MEGA_F_DTOT: return c->dtot;
+    case MEGA_F_UPHY: return c->uphy;
+    case MEGA_F_FINT: return c->fint;
+    case MEGA_F_FEXT: return c->fext;
+    case MEGA_F_WAVE_VEL: *width = 1; return c->wave_vel;
+    default: return nullptr;
+    }
+}
+
 int mega_download(mega_ctx *c, int32_t field, double *host)
 {
     if (!c || !host) return MEGA_ERR_ARG;
-    int w = 0;
-    double *src = field_ptr(c, field, &w);
-    if (!src) return MEGA_ERR_ARG;
-    if (field == MEGA_F_VPHY && c->has_shapes) {   // sum_j phi_ij v_j of the current velocity
+    const int n = c->n;
+    int r = MEGA_OK;
+    switch (field) {
+    case MEGA_F_STRESS: r = state_out(c, c->stress, 6 * (int64_t)n, host); break;
+    case MEGA_F_STRAIN: r = state_out(c, c->strain, 6 * (int64_t)n, host); break;
+    case MEGA_F_EPS: r = state_out(c, c->eps, n, host); break;
+    case MEGA_F_STRAIN_EQ:
+        if (c->mixed) k_strain_eq<float><<<grid_for(n), kBlock, 0, c->stream>>>(n, (const float *)c->strain, c->scratch);
+        else k_strain_eq<double><<<grid_for(n), kBlock, 0, c->stream>>>(n, (const double *)c->strain, c->scratch);
+        HIPCHK(hipMemcpyAsync(host, c->scratch, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        break;
+    case MEGA_F_VPHY: {   // sum_j phi_ij v_j of the current velocity
+        if (!c->has_shapes) return MEGA_ERR_ARG;
         Params p = params(c);
-        k_interp<false><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->dv.v, c->dv.vphy);
+        if (c->mixed) k_interp<float, false><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, c->v, c->scratch);
+        else k_interp<double, false><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, c->v, c->scratch);
+        HIPCHK(hipMemcpyAsync(host, c->scratch, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        break;
     }
-    HIPCHK(hipMemcpyAsync(host, src, sizeof(double) * w * c->n, hipMemcpyDeviceToHost, c->stream));
+    default: {
+        int w = 0;
+        double *src = field_ptr(c, field, &w);
+        if (!src) return MEGA_ERR_ARG;
+        HIPCHK(hipMemcpyAsync(host, src, sizeof(double) * w * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    }
+    if (r) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     return MEGA_OK;
 }
@@ -1281,10 +1563,21 @@ int mega_download(mega_ctx *c, int32_t field, double *host)
 int mega_upload(mega_ctx *c, int32_t field, const double *host)
 {
     if (!c || !host) return MEGA_ERR_ARG;
-    int w = 0;
-    double *dst = field_ptr(c, field, &w);
-    if (!dst) return MEGA_ERR_ARG;
-    HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * w * c->n, hipMemcpyHostToDevice, c->stream));
+    const int n = c->n;
+    int r = MEGA_OK;
+    switch (field) {
+    case MEGA_F_STRESS: r = state_in(c, c->stress, 6 * (int64_t)n, host); break;
+    case MEGA_F_STRAIN: r = state_in(c, c->strain, 6 * (int64_t)n, host); break;
+    case MEGA_F_EPS: r = state_in(c, c->eps, n, host); break;
+    default: {
+        int w = 0;
+        double *dst = field_ptr(c, field, &w);
+        if (!dst || field == MEGA_F_WAVE_VEL) return MEGA_ERR_ARG;
+        HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * w * n, hipMemcpyHostToDevice, c->stream));
+        if (field == MEGA_F_D || field == MEGA_F_DTOT) r = sync_d32(c);
+    }
+    }
+    if (r) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     return MEGA_OK;
 }
@@ -1293,7 +1586,7 @@ int mega_energies(mega_ctx *c, double *e_int, double *e_kin, double *e_ext)
 {
     if (!c) return MEGA_ERR_ARG;
     double acc[4];
-    HIPCHK(hipMemcpyAsync(acc, c->dv.acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(acc, c->acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (e_int) *e_int = c->c.finite_strain ? acc[0] : acc[2];
     if (e_kin) *e_kin = acc[1];
@@ -1303,10 +1596,10 @@ int mega_energies(mega_ctx *c, double *e_int, double *e_kin, double *e_ext)
 
 int mega_stats(mega_ctx *c, int64_t *nnz, int32_t *max_neighbors, int64_t *fallbacks)
 {
-    if (!c) return MEGA_ERR_ARG;
+    if (!c || !c->has_lists) return MEGA_ERR_ARG;
     const int n = c->n;
     std::vector<int32_t> rp(n + 1);
-    HIPCHK(hipMemcpyAsync(rp.data(), c->dv.row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(rp.data(), c->row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     int32_t mx = 0;
     for (int i = 0; i < n; ++i) mx = std::max(mx, rp[i + 1] - rp[i]);

@@ -149,3 +149,54 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
         gn, gs, st = g.neighbors()
         rp, col, _, _ = run.csr()
         assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
+
+
+def test_construct_fint_mixed_precision(mega, oracle):
+    """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
+    rotation) against the FP64 oracle on the same inputs: force within 1e-4
+    normwise, stress within 1e-5."""
+    prob = small_problem(lagrangian=1)
+    n = prob.n
+    rng = np.random.default_rng(5)
+    g = mega.Mega(prob, precision=mega.MEGA_MIXED)
+    g.search()
+    rp, col, _ = oracle_search(oracle, prob)
+    nnz = int(rp[-1])
+    shp, dshp = np.zeros(nnz), np.zeros((nnz, 3))
+    z = np.zeros((n, 3))
+    st0, e0, s0 = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 20))
+    oracle_fint(oracle, prob, rp, col, shp, dshp, 1, z, z, prob.coo, st0.copy(), e0.copy(), s0.copy())
+    g.construct_fint(1, z, z, prob.coo, st0.copy(), e0.copy(), s0.copy())
+    dtot = rng.normal(size=(n, 3)) * 2e-6 + np.array([0, 0, -3e-5])
+    dinc = rng.normal(size=(n, 3)) * 4e-7 + np.array([0, 0, -1e-6])
+    stress = rng.normal(size=(n, 6)) * 3e8
+    strain = rng.normal(size=(n, 6)) * 1e-3
+    state = np.zeros((n, 20))
+    state[:, 2] = np.abs(rng.normal(size=n)) * 0.02
+    s_r, e_r, st_r = stress.copy(), strain.copy(), state.copy()
+    f_r, _, _, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, dinc, dtot, prob.coo + dtot, s_r, e_r, st_r)
+    s_g, e_g, st_g = stress.copy(), strain.copy(), state.copy()
+    f_g, _, _, _ = g.construct_fint(0, dinc, dtot, prob.coo + dtot, s_g, e_g, st_g)
+    assert rel_err(f_g, f_r) < 1e-4
+    assert rel_err(s_g, s_r) < 1e-5
+    assert rel_err(st_g[:, 2], st_r[:, 2]) < 1e-4
+
+
+@pytest.mark.parametrize("lagrangian", [1, 0])
+def test_explicit_steps_mixed_precision(mega, oracle, lagrangian):
+    """Mixed mode over 12 steps against the FP64 oracle.  FP32 force and
+    stress noise (~1e-6 relative per step) enters the velocities and, through
+    them, the positions; bounds: positions 1e-7, displacement 1e-4, velocity
+    and stress 1e-3 (relative to each field's max)."""
+    prob = small_problem(lagrangian=lagrangian)
+    nsteps = 12
+    g = mega.Mega(prob, precision=mega.MEGA_MIXED)
+    g.step(nsteps)
+    run = OracleRun(oracle, prob).step(nsteps)
+    A = run.arr
+    errs = {k: rel_err(g.download(k), A[r]) for k, r in (("x", "x"), ("uphy", "dphy_tot"), ("v", "v"), ("stress", "stress"))}
+    print("mixed step errors", lagrangian, errs)
+    assert errs["x"] < 1e-7
+    assert errs["uphy"] < 1e-4
+    assert errs["v"] < 1e-3
+    assert errs["stress"] < 1e-3
