@@ -1,0 +1,210 @@
+"""Write the golden fixtures under tests/golden/ from the reference program.
+
+Runs only where the reference was compiled (oracle/_ref, built from the
+reference's own sources by oracle/Makefile): calls the reference subroutines
+through the bind(C) harness and runs the reference program MEGA on synthetic
+decks written by taylor_amd.deck, then stores inputs and outputs as small
+compressed .npz files.  The fixtures are data only.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from oracle_bind import REF_MEGA, load_ref, ref_set_control  # noqa: E402
+from taylor_amd import deck  # noqa: E402
+from vtk_read import read_unformatted_vtk  # noqa: E402
+
+H, LEN = 0.7e-3, 6e-3
+
+
+def small(lagrangian=1, **kw):
+    return deck.taylor_problem(H, length=LEN, seed=0, lagrangian=lagrangian, pdsearch_steps=3, **kw)
+
+
+def dense_vectors(ref, rng):
+    out = {}
+    n = 400
+    xs = rng.uniform(0, 1.2, n)
+    k = np.zeros((n, 2))
+    for i, x in enumerate(xs):
+        a, b = C.c_double(), C.c_double()
+        ref.ref_mls_kernel0(float(x), 1.0, 3, C.byref(a), C.byref(b))
+        k[i] = a.value, b.value
+    out["kernel_x"], out["kernel_out"] = xs, k
+    A4 = rng.normal(size=(n, 16)) + 4 * np.eye(4).ravel()
+    out["m44_in"] = A4
+    out["m44_out"] = np.array([_call(ref.ref_m44inv, a, 16) for a in A4])
+    A3 = rng.normal(size=(n, 9))
+    out["inv3_in"] = A3
+    out["inv3_out"] = np.array([_call(ref.ref_inv3, a, 9) for a in A3])
+    out["inverse_out"] = np.array([_call(lambda a, b: ref.ref_inverse(a, 3, b), a, 9) for a in A3])
+    det = []
+    for a in A3:
+        d = C.c_double()
+        ref.ref_determinant(np.ascontiguousarray(a), C.byref(d))
+        det.append(d.value)
+    out["det_out"] = np.array(det)
+    L = rng.normal(size=(n, 9)) * 0.02
+    out["hw_in"] = L
+    hw = []
+    for l in L:
+        R, S, D = np.zeros(9), np.zeros(6), np.zeros(6)
+        ref.ref_hughes_winget(np.ascontiguousarray(l), R, S, D)
+        hw.append(np.concatenate([R, S, D]))
+    out["hw_out"] = np.array(hw)
+    V = rng.normal(size=(n, 6))
+    rot = []
+    for r, v in zip(out["hw_out"][:, :9], V):
+        w = v.copy()
+        ref.ref_rotate_tensor(np.ascontiguousarray(r), w)
+        rot.append(w)
+    out["rot_in"], out["rot_out"] = V, np.array(rot)
+    # J2 return maps for the linear law (config materials) and the shipped power law
+    for tag, mat in (("lin", deck.copper_linear()), ("pow", deck.Material.j2(78.2e9, 0.3, 2700.0, 0.29e9, 125.0, 0.1))):
+        pred = rng.normal(size=(n, 6)) * mat.props[3] * rng.uniform(0.1, 3.0, size=(n, 1))
+        eps0 = np.abs(rng.normal(size=n)) * 0.1
+        res = []
+        for p, e in zip(pred, eps0):
+            s = np.zeros(6)
+            st = np.zeros(20)
+            st[2] = e
+            ref.ref_von_mises(s, np.zeros(6), np.ascontiguousarray(p), st, mat.props)
+            res.append(np.concatenate([s, [st[2]]]))
+        out[f"vm_{tag}_props"], out[f"vm_{tag}_pred"], out[f"vm_{tag}_eps0"] = mat.props, pred, eps0
+        out[f"vm_{tag}_out"] = np.array(res)
+    return out
+
+
+def _call(f, a, m):
+    o = np.zeros(m)
+    f(np.ascontiguousarray(a), o)
+    return o
+
+
+def geometry_vectors(ref):
+    mesh = deck.taylor_bar_mesh(H, length=LEN, seed=0)
+    n, ne = mesh.coo.shape[0], mesh.elcon.shape[0]
+    ad, vol, sm, dm = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 6)), np.zeros(3)
+    ref.ref_preprocess(n, ne, np.ascontiguousarray(mesh.elcon), np.ascontiguousarray(mesh.coo), ad, vol, sm, dm)
+    return {"coo": mesh.coo, "elcon": mesh.elcon, "adial": ad, "vol": vol, "sm_len": sm, "dist_max": dm}
+
+
+def search_rk_fint_vectors(ref, rng):
+    out = {}
+    prob = small()
+    ref_set_control(ref, prob)
+    n = prob.n
+    coo, win = np.ascontiguousarray(prob.coo), np.ascontiguousarray(prob.win)
+    d = prob.dist_max
+    geo, nb = np.zeros(6), np.zeros(3, np.int32)
+    ref.ref_bin_geometry(n, coo, win, float(d[0]), float(d[1]), float(d[2]), geo, nb)
+    ijk = np.zeros((n, 3), np.int32)
+    ref.ref_get_bins(n, coo, ijk)
+    sm = np.zeros((n, 6))
+
+    def soft(x):
+        dim = n * 200
+        gn, gs, st = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(dim, np.int32)
+        mx, nz = C.c_int(), C.c_int()
+        ref.ref_soft_search(n, np.ascontiguousarray(x), win, sm, float(d[0]), float(d[1]), float(d[2]), gn, gs, dim, st,
+                            C.byref(mx), C.byref(nz))
+        return gn, gs, st[:nz.value]
+
+    gn, gs, st = soft(coo)
+    out.update(bin_geom=geo, bin_nb=nb, bin_ijk=ijk, gn=gn, gstart=gs, gstack=st)
+    # stale bins: nodes moved, bins kept (HANDELER between PDSEARCH rebuilds)
+    xm = coo + rng.normal(size=coo.shape) * 0.3 * H
+    gn2, gs2, st2 = soft(xm)
+    out.update(x_moved=xm, gn_stale=gn2, gstart_stale=gs2, gstack_stale=st2)
+    # RK1 at every node (QL on, singular kernel at the wall)
+    ebc = np.ascontiguousarray(prob.ebc_node, dtype=np.int32)
+    shp, dshp = [], []
+    for i in range(n):
+        a, b = gs[i] - 1, gs[i] - 1 + gn[i]
+        ls = np.ascontiguousarray(st[a:b])
+        s1, d1 = np.zeros(gn[i]), np.zeros(3 * gn[i])
+        ref.ref_rk1(coo[i].copy(), n, coo, win, ls, int(gn[i]), ebc, int(ebc[i]), s1, d1)
+        shp.append(s1)
+        dshp.append(d1.reshape(-1, 3))
+    shp, dshp = np.concatenate(shp), np.concatenate(dshp)
+    out.update(rk_shp=shp, rk_dshp=dshp)
+    # one CONSTRUCT_FINT call per formulation on a yielding random state
+    nnz = int(gn.sum())
+    for lag in (1, 0):
+        p2 = small(lagrangian=lag)
+        ref_set_control(ref, p2)
+        dtot = rng.normal(size=(n, 3)) * 2e-6
+        dinc = rng.normal(size=(n, 3)) * 4e-7 + np.array([0, 0, -1e-6])
+        stress = rng.normal(size=(n, 6)) * 3e8
+        strain = rng.normal(size=(n, 6)) * 1e-3
+        state = np.zeros((n, 20))
+        state[:, 2] = np.abs(rng.normal(size=n)) * 0.02
+        x_cur = coo + dtot
+        s, e, t = stress.copy(), strain.copy(), state.copy()
+        f, fx, wv, seq = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+        w = C.c_double(0)
+        props = np.ascontiguousarray(p2.props_table()[p2.mat_id])
+        ref.ref_construct_fint(n, win, np.ascontiguousarray(p2.vol), coo, np.ascontiguousarray(x_cur), props, t, s, e,
+                               np.ascontiguousarray(dinc), np.ascontiguousarray(dtot), np.full(n, 2, np.int32), ebc,
+                               gn, gs, nnz, st, shp.copy(), np.ascontiguousarray(dshp).copy(), int(gn.max()), 0,
+                               f, fx, wv, np.zeros((n, 3)), np.ones(n, np.int32), seq, C.byref(w), p2.dt)
+        out.update({f"fint{lag}_dtot": dtot, f"fint{lag}_dinc": dinc, f"fint{lag}_stress0": stress,
+                    f"fint{lag}_strain0": strain, f"fint{lag}_eps0": state[:, 2].copy(), f"fint{lag}_f": f,
+                    f"fint{lag}_stress": s, f"fint{lag}_strain": e, f"fint{lag}_eps": t[:, 2].copy(),
+                    f"fint{lag}_work": np.array([w.value]), f"fint{lag}_wave": wv, f"fint{lag}_strain_eq": seq})
+    return out
+
+
+def mega_run(prob, mesh, nsteps, frames, workdir):
+    """Run the reference program on the deck; return the chosen frames."""
+    kfile = os.path.join(workdir, "bar.k")
+    deck.write_k(kfile, mesh)
+    prob.time_end = prob.dt * (nsteps - 0.5)
+    deck.write_control(os.path.join(workdir, "Control.dat"), kfile, prob, ["BAR", "WALL"], time_output=-1.0,
+                       out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps")
+    with open(os.path.join(workdir, "mega.out"), "w") as fo:
+        subprocess.run([REF_MEGA, "-j", "1"], cwd=workdir, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
+                       check=True, timeout=3600)
+    out = {}
+    for k in frames:
+        fr = read_unformatted_vtk(os.path.join(workdir, "MEGAOUT_%04d.vtk" % k))
+        out[f"step{k}_x"] = fr["points"]
+        out[f"step{k}_u"] = np.stack([fr["displacement_x"], fr["displacement_y"], fr["displacement_z"]], 1)
+        out[f"step{k}_vphy"] = np.stack([fr["velocity_x"], fr["velocity_y"], fr["velocity_z"]], 1)
+        out[f"step{k}_fint"] = np.stack([fr["fint_x"], fr["fint_y"], fr["fint_z"]], 1)
+        out[f"step{k}_stress"] = np.stack([fr[f"stress_{c}"] for c in ("xx", "yy", "zz", "yz", "xz", "xy")], 1)
+        out[f"step{k}_eps"] = fr["eps"]
+    return out
+
+
+def main():
+    ref = load_ref()
+    rng = np.random.default_rng(20260101)
+    np.savez_compressed(os.path.join(HERE, "dense.npz"), **dense_vectors(ref, rng))
+    np.savez_compressed(os.path.join(HERE, "geometry.npz"), **geometry_vectors(ref))
+    np.savez_compressed(os.path.join(HERE, "search_rk_fint.npz"), **search_rk_fint_vectors(ref, rng))
+    mesh = deck.taylor_bar_mesh(H, length=LEN, seed=0)
+    for lag, name in ((1, "run_lagrangian.npz"), (0, "run_semilagrangian.npz")):
+        with tempfile.TemporaryDirectory() as wd:
+            prob = small(lagrangian=lag)
+            np.savez_compressed(os.path.join(HERE, name), **mega_run(prob, mesh, 40, (1, 10, 40), wd))
+    for f in sorted(glob.glob(os.path.join(HERE, "*.npz"))):
+        print(os.path.basename(f), os.path.getsize(f))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+"""The oracle against the golden fixtures written from the reference program
+(tests/golden/make_golden.py).  With one thread the oracle keeps the
+reference's operation order, so the bar is bit-exact equality."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle_bind import OracleRun, control, oracle_fint, oracle_search, oracle_shapes
+from taylor_amd import deck
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+H, LEN = 0.7e-3, 6e-3
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def small(lagrangian=1):
+    return deck.taylor_problem(H, length=LEN, seed=0, lagrangian=lagrangian, pdsearch_steps=3)
+
+
+def test_dense_routines_bit_exact(oracle):
+    g = load("dense.npz")
+    for x, (phi, dphi) in zip(g["kernel_x"], g["kernel_out"]):
+        a, b = C.c_double(), C.c_double()
+        oracle.or_kernel(float(x), C.byref(a), C.byref(b))
+        assert a.value == phi and b.value == dphi
+    for a, r44, a3, r3, rinv, det in zip(g["m44_in"], g["m44_out"], g["inv3_in"], g["inv3_out"], g["inverse_out"], g["det_out"]):
+        o = np.zeros(16)
+        oracle.or_m44inv(np.ascontiguousarray(a), o)
+        assert np.array_equal(o, r44)
+        o = np.zeros(9)
+        oracle.or_inv3(np.ascontiguousarray(a3), o)
+        assert np.array_equal(o, r3)
+        o = np.zeros(9)
+        oracle.or_inverse(np.ascontiguousarray(a3), 3, o)
+        assert np.array_equal(o, rinv)
+        assert oracle.or_determinant(np.ascontiguousarray(a3)) == det
+    for l, out in zip(g["hw_in"], g["hw_out"]):
+        R, S, D = np.zeros(9), np.zeros(6), np.zeros(6)
+        oracle.or_hughes_winget(np.ascontiguousarray(l), R, S, D)
+        assert np.array_equal(np.concatenate([R, S, D]), out)
+    for r, v, out in zip(g["hw_out"][:, :9], g["rot_in"], g["rot_out"]):
+        w = v.copy()
+        oracle.or_rotate_tensor(np.ascontiguousarray(r), w)
+        assert np.array_equal(w, out)
+
+
+@pytest.mark.parametrize("law", ["lin", "pow"])
+def test_return_map_bit_exact(oracle, law):
+    g = load("dense.npz")
+    props = g[f"vm_{law}_props"]
+    for p, e0, out in zip(g[f"vm_{law}_pred"], g[f"vm_{law}_eps0"], g[f"vm_{law}_out"]):
+        s, st = np.zeros(6), np.zeros(20)
+        st[2] = e0
+        assert oracle.or_von_mises(s, np.ascontiguousarray(p), st, props) >= 0
+        assert np.array_equal(s, out[:6]) and st[2] == out[6]
+
+
+def test_geometry_preprocessing_bit_exact():
+    g = load("geometry.npz")
+    mesh = deck.Mesh(coo=g["coo"], elcon=g["elcon"], sets={})
+    geo = deck.preprocess(mesh)
+    assert np.array_equal(geo.win0, g["adial"])
+    assert np.array_equal(geo.vol, g["vol"])
+    assert np.array_equal(geo.sm_len, g["sm_len"])
+    assert np.array_equal(geo.dist_max, g["dist_max"])
+    # the generator itself is deterministic: same seed, same deck
+    assert np.array_equal(deck.taylor_bar_mesh(H, length=LEN, seed=0).coo, g["coo"])
+
+
+def test_bins_and_search_bit_exact(oracle):
+    g = load("search_rk_fint.npz")
+    prob = small()
+    rp, col, ijk = oracle_search(oracle, prob)
+    assert np.array_equal(ijk, g["bin_ijk"])
+    assert np.array_equal(np.diff(rp), g["gn"])
+    assert np.array_equal(rp[:-1] + 1, g["gstart"])
+    assert np.array_equal(col + 1, g["gstack"])      # same entries in the same order
+    # stale bins: positions moved, bin assignment kept
+    rp2, col2, _ = oracle_search(oracle, prob, x=g["x_moved"], ijk=g["bin_ijk"])
+    assert np.array_equal(np.diff(rp2), g["gn_stale"])
+    assert np.array_equal(col2 + 1, g["gstack_stale"])
+
+
+def test_rk_shapes_bit_exact(oracle):
+    g = load("search_rk_fint.npz")
+    prob = small()
+    rp, col, _ = oracle_search(oracle, prob)
+    shp, dshp = oracle_shapes(oracle, prob, rp, col)
+    assert np.array_equal(shp, g["rk_shp"])
+    assert np.array_equal(dshp, g["rk_dshp"])
+
+
+@pytest.mark.parametrize("lag", [1, 0])
+def test_construct_fint_bit_exact(oracle, lag):
+    g = load("search_rk_fint.npz")
+    prob = small(lagrangian=lag)
+    n = prob.n
+    rp, col, _ = oracle_search(oracle, prob)
+    shp, dshp = g["rk_shp"].copy(), g["rk_dshp"].copy()
+    s, e = g[f"fint{lag}_stress0"].copy(), g[f"fint{lag}_strain0"].copy()
+    st = np.zeros((n, 20))
+    st[:, 2] = g[f"fint{lag}_eps0"]
+    dtot = g[f"fint{lag}_dtot"]
+    f, fx, w, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, g[f"fint{lag}_dinc"], dtot, prob.coo + dtot, s, e, st)
+    assert np.array_equal(f, g[f"fint{lag}_f"])
+    assert np.array_equal(s, g[f"fint{lag}_stress"])
+    assert np.array_equal(e, g[f"fint{lag}_strain"])
+    assert np.array_equal(st[:, 2], g[f"fint{lag}_eps"])
+    assert w == g[f"fint{lag}_work"][0]
+
+
+@pytest.mark.parametrize("lag,name", [(1, "run_lagrangian.npz"), (0, "run_semilagrangian.npz")])
+def test_whole_run_bit_exact(oracle, lag, name):
+    """40 explicit steps of the reference program MEGA (frames 1, 10, 40)."""
+    g = load(name)
+    run = OracleRun(oracle, small(lagrangian=lag))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        assert np.array_equal(A["x"], g[f"step{k}_x"])
+        assert np.array_equal(A["dphy_tot"], g[f"step{k}_u"])
+        assert np.array_equal(A["vphy"], g[f"step{k}_vphy"])
+        assert np.array_equal(A["fint"], g[f"step{k}_fint"])
+        assert np.array_equal(A["stress"], g[f"step{k}_stress"])
+        assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])

@@ -1,0 +1,218 @@
+"""Known-answer tests for the oracle from what the mathematics fixes: closed
+forms, invariants and brute force on small inputs.  These pin the oracle
+independently of the reference run (a dropped term, a wrong sign, index or
+transposed operand fails one of them)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_bind import OracleRun, control, oracle_fint, oracle_search, oracle_shapes
+from taylor_amd import deck
+
+
+def kern(oracle, x):
+    a, b = C.c_double(), C.c_double()
+    oracle.or_kernel(float(x), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def test_cubic_bspline_closed_form(oracle):
+    assert kern(oracle, 0.0) == (2.0 / 3.0, 0.0)
+    assert abs(kern(oracle, 1.0)[0]) < 1e-15 and kern(oracle, 1.5) == (0.0, 0.0)   # 4/3-4+4-4/3 rounds to -2^-52
+    # C2 at the knot z = 1/2: values, slopes and curvatures agree from both sides
+    e = 1e-7
+    lo, hi = kern(oracle, 0.5 - e), kern(oracle, 0.5 + e)
+    assert abs(lo[0] - hi[0]) < 1e-6 and abs(lo[1] - hi[1]) < 1e-6
+    # derivative is d/dz of the value
+    for z in np.linspace(0.01, 0.99, 37):
+        fd = (kern(oracle, z + 1e-6)[0] - kern(oracle, z - 1e-6)[0]) / 2e-6
+        assert abs(fd - kern(oracle, z)[1]) < 1e-6
+    # integral over [-1, 1] is 1/2 (the cubic B-spline of support 2 scaled by 1/2)
+    z = np.linspace(0, 1, 20001)
+    w = np.array([kern(oracle, t)[0] for t in z])
+    assert abs(2 * np.trapezoid(w, z) - 0.5) < 1e-8
+
+
+def test_inv3_is_adjugate_times_det(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        A = rng.normal(size=(3, 3))
+        out = np.zeros(9)
+        oracle.or_inv3(np.ascontiguousarray(A.ravel(order="F")), out)
+        expect = np.linalg.det(A) ** 2 * np.linalg.inv(A)   # adj(A) det(A)
+        assert np.allclose(out.reshape(3, 3, order="F"), expect, rtol=1e-12, atol=1e-12)
+        assert abs(oracle.or_determinant(np.ascontiguousarray(A.ravel(order="F"))) - abs(np.linalg.det(A))) < 1e-12
+
+
+def test_m44inv_and_inverse_are_inverses(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        A = rng.normal(size=(4, 4)) + 4 * np.eye(4)
+        out = np.zeros(16)
+        oracle.or_m44inv(np.ascontiguousarray(A.ravel(order="F")), out)
+        assert np.allclose(out.reshape(4, 4, order="F") @ A, np.eye(4), atol=1e-12)
+        B = rng.normal(size=(3, 3)) + 3 * np.eye(3)
+        P = np.eye(3)[rng.permutation(3)]          # pivoting exercised
+        B = P @ B
+        out = np.zeros(9)
+        assert oracle.or_inverse(np.ascontiguousarray(B.ravel(order="F")), 3, out) == 1
+        assert np.allclose(out.reshape(3, 3, order="F") @ B, np.eye(3), atol=1e-12)
+    out = np.zeros(9)
+    assert oracle.or_inverse(np.zeros(9), 3, out) == 0   # singular -> the reference's warning path
+
+
+def hw(oracle, L):
+    R, S, D = np.zeros(9), np.zeros(6), np.zeros(6)
+    oracle.or_hughes_winget(np.ascontiguousarray(L.ravel(order="F")), R, S, D)
+    return R.reshape(3, 3, order="F"), S, D
+
+
+def test_hughes_winget_closed_form(oracle):
+    R, S, D = hw(oracle, np.zeros((3, 3)))
+    assert np.array_equal(R, np.eye(3)) and not S.any() and not D.any()
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        L = rng.normal(size=(3, 3)) * 1e-3
+        A = np.eye(3) + L / 2
+        G = L @ (np.linalg.det(A) ** 2 * np.linalg.inv(A))          # bug-compatible INV3
+        W = (G - G.T) / 2
+        IW = np.eye(3) - W / 2
+        Rx = np.eye(3) + (np.linalg.det(IW) ** 2 * np.linalg.inv(IW)) @ W
+        E = (G + G.T) / 2
+        R, S, D = hw(oracle, L)
+        assert np.allclose(R, Rx, atol=1e-15)
+        assert np.allclose(D, [E[0, 0], E[1, 1], E[2, 2], E[1, 2], E[0, 2], E[0, 1]], atol=1e-16)
+        assert np.allclose(S, [E[0, 0], E[1, 1], E[2, 2], 2 * E[1, 2], 2 * E[0, 2], 2 * E[0, 1]], atol=1e-16)
+        # a pure spin W gives strain and a departure of R from orthogonality
+        # of second order only (sym of W (I + W/2)^-1 is O(|W|^2))
+        Wm = L - L.T
+        R, S, _ = hw(oracle, Wm)
+        w2 = np.abs(Wm).max() ** 2
+        assert np.abs(S).max() < 10 * w2 and np.abs(R.T @ R - np.eye(3)).max() < 10 * w2
+
+
+def test_rotate_tensor_matches_matrix_product(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(100):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        v = rng.normal(size=6)
+        T = np.array([[v[0], v[5], v[4]], [v[5], v[1], v[3]], [v[4], v[3], v[2]]])
+        w = v.copy()
+        oracle.or_rotate_tensor(np.ascontiguousarray(q.ravel(order="F")), w)
+        Tr = q @ T @ q.T
+        assert np.allclose(w, [Tr[0, 0], Tr[1, 1], Tr[2, 2], Tr[1, 2], Tr[0, 2], Tr[0, 1]], atol=1e-13)
+        assert abs(w[0] + w[1] + w[2] - np.trace(T)) < 1e-13
+
+
+def test_radial_return_linear_hardening_closed_form(oracle):
+    """CE = 1: one Newton step is exact; the returned deviator lies on the
+    hardened yield surface and the plastic strain is eps + sqrt(2/3) dgamma
+    with dgamma = f / (2 mu + 2/3 B sigma_0) (Simo & Hughes radial return)."""
+    mat = deck.copper_linear()
+    p = mat.props
+    mu, sig0, B = p[28], p[3], p[4]
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        pred = rng.normal(size=6) * sig0 * 1.5
+        e0 = abs(rng.normal()) * 0.05
+        dev = pred.copy()
+        dev[:3] -= pred[:3].sum() / 3
+        tn = np.sqrt((dev[:3] ** 2).sum() + 2 * (dev[3:] ** 2).sum())
+        f = tn - np.sqrt(2 / 3) * sig0 * (1 + B * e0)
+        s, st = np.zeros(6), np.zeros(20)
+        st[2] = e0
+        it = oracle.or_von_mises(s, pred, st, p)
+        if f <= 0:
+            assert it == 0 and np.allclose(s, pred, rtol=1e-14, atol=1e-6) and st[2] == e0
+            continue
+        dg = f / (2 * mu + 2.0 / 3.0 * B * sig0)
+        assert it == 1
+        assert abs(st[2] - (e0 + np.sqrt(2 / 3) * dg)) < 1e-12 * max(1, st[2])
+        sd = s.copy()
+        sd[:3] -= s[:3].sum() / 3
+        ns = np.sqrt((sd[:3] ** 2).sum() + 2 * (sd[3:] ** 2).sum())
+        assert abs(ns - np.sqrt(2 / 3) * sig0 * (1 + B * st[2])) < 1e-9 * sig0
+        assert abs(s[:3].sum() - pred[:3].sum()) < 1e-6       # pressure unchanged
+
+
+def test_search_matches_brute_force(oracle):
+    """SOFT_SEARCH keeps j iff x_i lies strictly inside j's box support;
+    brute force over all pairs with the same criterion."""
+    prob = deck.taylor_problem(0.8e-3, length=5e-3, seed=9)
+    rp, col, _ = oracle_search(oracle, prob)
+    x, w = prob.coo, prob.win
+    inside = np.all(np.abs(x[None, :, :] - x[:, None, :]) / w[None, :, :] < 1.0, axis=2)   # [i, j]
+    for i in range(prob.n):
+        assert set(col[rp[i]:rp[i + 1]].tolist()) == set(np.nonzero(inside[i])[0].tolist())
+    prob.shsup = 1
+    rp, col, _ = oracle_search(oracle, prob)
+    inside = np.sqrt(((np.abs(x[None, :, :] - x[:, None, :]) / w[None, :, :]) ** 2).sum(2)) < 1.0
+    for i in range(prob.n):
+        assert set(col[rp[i]:rp[i + 1]].tolist()) == set(np.nonzero(inside[i])[0].tolist())
+
+
+def test_rk_consistency_without_ql(oracle):
+    """Linear reproducing conditions (TEST.F90:1-300): sum phi = 1,
+    sum phi x = x, sum grad phi = 0, sum grad phi (x) x = I, exact with QL off
+    and no singular kernel."""
+    prob = deck.taylor_problem(0.8e-3, length=5e-3, seed=9, ql=0)
+    prob.ebc_node[:] = 0
+    rp, col, _ = oracle_search(oracle, prob)
+    shp, dshp = oracle_shapes(oracle, prob, rp, col)
+    x = prob.coo
+    for i in range(prob.n):
+        a, b = rp[i], rp[i + 1]
+        xj = x[col[a:b]]
+        assert abs(shp[a:b].sum() - 1) < 1e-12
+        assert np.abs(shp[a:b] @ xj - x[i]).max() < 1e-15
+        assert np.abs(dshp[a:b].sum(0)).max() < 1e-8
+        assert np.abs(dshp[a:b].T @ xj - np.eye(3)).max() < 1e-11
+
+
+def test_rk_partition_of_unity_with_ql_and_singular_kernel(oracle):
+    prob = deck.taylor_problem(0.8e-3, length=5e-3, seed=9)
+    rp, col, _ = oracle_search(oracle, prob)
+    shp, _ = oracle_shapes(oracle, prob, rp, col)
+    sums = np.add.reduceat(shp, rp[:-1])
+    assert np.abs(sums - 1).max() < 1e-10
+
+
+def test_force_balance_without_ql(oracle):
+    """With sum grad phi = 0 the assembled internal forces sum to zero
+    (momentum conservation of the nodal integration)."""
+    prob = deck.taylor_problem(0.8e-3, length=5e-3, seed=9, ql=0)
+    prob.ebc_node[:] = 0
+    n = prob.n
+    rp, col, _ = oracle_search(oracle, prob)
+    nnz = int(rp[-1])
+    shp, dshp = np.zeros(nnz), np.zeros((nnz, 3))
+    z = np.zeros((n, 3))
+    oracle_fint(oracle, prob, rp, col, shp, dshp, 1, z, z, prob.coo, np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 20)))
+    rng = np.random.default_rng(6)
+    stress = rng.normal(size=(n, 6)) * 1e8
+    f, _, _, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, rng.normal(size=(n, 3)) * 1e-7, z, prob.coo,
+                             stress, np.zeros((n, 6)), np.zeros((n, 20)))
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max()
+
+
+def test_free_flight_is_rigid(oracle):
+    """An unconstrained, stress-free bar in uniform motion (QL off, so
+    sum grad phi = 0): the strain increments vanish to round-off, the velocity
+    is kept and every node advances by v dt per step."""
+    mesh = deck.taylor_bar_mesh(0.8e-3, length=5e-3, seed=9)
+    prob = deck.assemble(mesh, {}, {"BAR": (0.0, 0.0, -10.0)}, deck.copper_linear(), norm_win=1.84,
+                         dt=1e-8, time_end=1.0, ql=0)
+    run = OracleRun(oracle, prob).step(5)
+    A = run.arr
+    assert np.abs(A["v"][:, 2] + 10.0).max() < 1e-9
+    assert np.abs(A["dtot"][:, 2] + 10.0 * 5e-8).max() < 1e-18
+    assert np.abs(A["x"] - prob.coo - np.array([0, 0, -5e-7])).max() < 1e-15
+    assert np.abs(A["stress"]).max() < 1e-3          # Pa, against a 400 MPa yield stress
+
+
+def test_fixed_dofs_stay_at_rest(oracle):
+    prob = deck.taylor_problem(0.8e-3, length=5e-3, seed=9, v0=100.0)
+    run = OracleRun(oracle, prob).step(5)
+    fixed = prob.ebc == 1
+    assert np.all(run.arr["v"][fixed] == 0.0) and np.all(run.arr["dtot"][fixed] == 0.0)
