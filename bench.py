@@ -1,0 +1,266 @@
+"""Benchmark of the explicit meshfree step on one MI355X.
+
+Workload: the Taylor bar of 25.4 mm x 3.8 mm radius on a cubic lattice of
+spacing 0.088 mm (jitter 5 % of the spacing, seed 2; ~1.75M nodes, ~58
+neighbors per node), copper J2 with linear hardening, 190 m/s onto a fixed
+plane, dt = 0.3 h / c_p, Lagrangian finite strain with direct nodal
+integration.  Synthetic data; no model weights are involved.
+
+A "step" is one explicit step of the whole model (predictor, interpolation,
+internal force with the constitution, corrector).  The timed region is K
+steps after W warm-up steps, with all inputs resident in HBM; it is run in
+FP64 (plain) and in mixed precision (FP32 shape data, stress, return map and
+force; FP64 positions, F, F^-1 and rotation).  value = mixed-precision
+node-steps/s; the FP64 rate and the ratio are reported beside it.
+
+Unless --no-validate, both precisions are also run to 80 us simulated time
+and the final bar length (max z above the wall plane) and mushroom radius
+(max radius over the impact-face nodes) of the mixed run are compared with
+the FP64 run (gate: length within 0.5 %).
+
+roofline: the dominant kernel's algorithmic bytes per launch (compulsory
+HBM traffic: every pair and node array it must touch read or written once,
+neighbor re-reads excluded) over its mean launch time from HIP events on the
+context's stream; peak 8 TB/s.  traffic: HBM bytes per launch measured with
+rocprofv3 counters, read from profiles/ when present.
+
+cpu_baseline: the reference program (oracle/_ref/MEGA, OpenMP) on the host
+cores, on a bounded sample of the same workload (same spacing and material,
+a 2.54 mm slice of the bar, ~181k nodes): node-steps/s from the difference of
+a 12-step and a 2-step run (setup cancels).  Falls back to the oracle
+restatement when the reference build is absent.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-validate] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from taylor_amd import deck  # noqa: E402
+from taylor_amd import mega  # noqa: E402
+
+H = 0.088e-3
+SEED = 2
+V0 = 190.0
+T_END = 80e-6
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
+    """Algorithmic (compulsory) HBM bytes of one launch of the step kernels.
+    r: bytes of the shape/stress type; node masters are FP64."""
+    r = 4 if mixed else 8
+    if name == "k_gather":
+        # pairs: col + phi + grad phi; nodes: own d (FP64) read, uphy r/w, X0 read,
+        # x write, gather sums write (18 or 21 values), plus dtot own for F
+        return nnz * (4 + r + 3 * r) + n * (24 + 48 + 24 + 24 + (21 if mixed else 18) * r + (12 if mixed else 0))
+    if name == "k_node":
+        # sums read, stress/strain r/w, eps r/w, vol, mat id, P write, (d, dtot own in mixed)
+        return n * ((21 if mixed else 18) * r + 4 * 6 * r + 2 * r + 8 + 4 + 10 * r + (24 if mixed else 8))
+    if name == "k_force":
+        # pairs: source + grad phi (transposed); nodes: P read once, fint write, mass, ebc,
+        # mat id, v r/w, a write, d write, dtot r/w (+FP32 copies of d, dtot in mixed)
+        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 + (24 if mixed else 0))
+    if name == "k_predict":
+        return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
+    return 0.0
+
+
+def node_step_bytes(n: int, nnz: int, mixed: bool) -> float:
+    """Algorithmic bytes per node-step of the whole step (fused-pass model: one
+    pass over the pairs, index + phi + grad phi, and each node array once)."""
+    kbar = nnz / n
+    return kbar * (20 if mixed else 36) + (368 if mixed else 468)
+
+
+def read_traffic(kernel: str, mixed: bool):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter
+    summary (profiles/pmc_<kernel>_taylor1p75M_<mode>.txt), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_taylor1p75M_{'mixed' if mixed else 'fp64'}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_dispatch"):
+            return float(line.split()[1])
+    return None
+
+
+def timed_run(prob, precision: int, steps: int, warmup: int):
+    g = mega.Mega(prob, precision=precision)
+    g.step(max(warmup, 1))               # includes the first step (search, shapes)
+    g.profile(True)
+    t0 = time.perf_counter()
+    g.step(steps)                        # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    kt = g.kernel_times()
+    nnz, mx, _ = g.stats()
+    g.close()
+    return (t1 - t0), kt, nnz, mx
+
+
+def full_run(prob, precision: int):
+    g = mega.Mega(prob, precision=precision)
+    nsteps = int(np.floor(T_END / prob.dt)) + 1      # MAIN stops once TIME > TIME_END
+    done = 0
+    while done < nsteps:
+        k = min(2000, nsteps - done)
+        g.step(k)
+        done += k
+        log(f"  precision {precision}: {done}/{nsteps} steps")
+    x = g.download("x")
+    eps = g.download("eps")
+    g.close()
+    wall = prob.sets["WALL"] - 1
+    length = float(x[:, 2].max())                   # wall plane z = 0
+    radius = float(np.sqrt(x[wall, 0] ** 2 + x[wall, 1] ** 2).max())
+    return length, radius, eps, nsteps
+
+
+def cpu_baseline_reference(sample_len: float, cores: int):
+    from oracle_bind import REF_MEGA  # noqa: E402
+    if not os.path.exists(REF_MEGA):
+        return None
+    mesh = deck.taylor_bar_mesh(H, length=sample_len, seed=SEED)
+    prob = deck.taylor_problem(H, v0=V0, seed=SEED, length=sample_len)
+    times = {}
+    with tempfile.TemporaryDirectory() as wd:
+        kfile = os.path.join(wd, "bar.k")
+        deck.write_k(kfile, mesh)
+        for nsteps in (2, 12):
+            prob.time_end = prob.dt * (nsteps - 0.5)
+            deck.write_control(os.path.join(wd, "Control.dat"), kfile, prob, ["BAR", "WALL"],
+                               time_output=1.0, out_vars="node_id")
+            t0 = time.perf_counter()
+            with open(os.path.join(wd, "mega.out"), "w") as fo:
+                subprocess.run([REF_MEGA, "-j", str(cores)], cwd=wd, stdin=subprocess.DEVNULL, stdout=fo,
+                               stderr=fo, check=True, timeout=600)
+            times[nsteps] = time.perf_counter() - t0
+    rate = prob.n * 10 / (times[12] - times[2])
+    return {"value": rate, "unit": "node-steps/s", "cores": cores, "kind": "reference",
+            "sample": f"reference program MEGA -j {cores}, Taylor bar slice 2.54 mm at h=0.088 mm "
+                      f"({prob.n} nodes), FP64 Lagrangian; 12-step minus 2-step wall time "
+                      f"({times[12]:.1f} s - {times[2]:.1f} s)"}
+
+
+def cpu_baseline_port(sample_len: float, cores: int):
+    from oracle_bind import OracleRun, load_oracle  # noqa: E402
+    prob = deck.taylor_problem(H, v0=V0, seed=SEED, length=sample_len)
+    run = OracleRun(load_oracle(), prob, nthreads=cores)
+    run.step(1)
+    t0 = time.perf_counter()
+    run.step(10)
+    dt = time.perf_counter() - t0
+    return {"value": prob.n * 10 / dt, "unit": "node-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle restatement, {cores} threads, Taylor bar slice 2.54 mm at h=0.088 mm "
+                      f"({prob.n} nodes), FP64 Lagrangian, 10 steps after the first"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+    t = time.time()
+    prob = deck.taylor_problem(H, v0=V0, seed=SEED)
+    n = prob.n
+    log(f"workload: {n} nodes, dt {prob.dt:.4e} s, deck {time.time() - t:.1f} s")
+
+    res = {}
+    for name, prec in (("fp64", mega.MEGA_FP64), ("mixed", mega.MEGA_MIXED)):
+        wall, kt, nnz, mx = timed_run(prob, prec, args.steps, args.warmup)
+        res[name] = dict(wall=wall, kt=kt, nnz=nnz, max_neighbors=mx, rate=n * args.steps / wall)
+        log(f"{name}: {1e3 * wall / args.steps:.3f} ms/step, {res[name]['rate']:.4e} node-steps/s, "
+            + ", ".join(f"{k} {v[0]:.3f} ms" for k, v in kt.items()))
+
+    mixed = res["mixed"]
+    nnz = mixed["nnz"]
+    # dominant kernel of the reported (mixed) step
+    dom = max(mixed["kt"].items(), key=lambda kv: kv[1][0] * kv[1][1])
+    kname, (kms, _) = dom
+    kb = kernel_bytes(kname, n, nnz, True)
+    achieved = kb / (kms * 1e-3)
+    traffic = read_traffic(kname, True)
+
+    validation = None
+    if not args.no_validate:
+        l64, r64, e64, ns = full_run(prob, mega.MEGA_FP64)
+        l32, r32, e32, _ = full_run(prob, mega.MEGA_MIXED)
+        vol = prob.vol
+        validation = {
+            "t_end_s": T_END, "steps": ns,
+            "length_fp64_m": l64, "length_mixed_m": l32, "length_rel_err": abs(l32 - l64) / l64,
+            "radius_fp64_m": r64, "radius_mixed_m": r32, "radius_rel_err": abs(r32 - r64) / r64,
+            "eqps_l2_rel_err": float(np.sqrt((vol * (e32 - e64) ** 2).sum() / (vol * e64 ** 2).sum())),
+            "length_gate_0p5pct": bool(abs(l32 - l64) / l64 <= 0.005),
+        }
+        log("validation:", validation)
+
+    cpu = None
+    if not args.no_cpu:
+        cores = max(1, min(16, os.cpu_count() or 1))
+        try:
+            cpu = cpu_baseline_reference(2.54e-3, cores)
+        except Exception as e:   # the reference program is only a reported baseline
+            log("reference CPU baseline failed:", e)
+        if cpu is None:
+            cpu = cpu_baseline_port(2.54e-3, cores)
+        log("cpu_baseline:", cpu)
+
+    out = {
+        "metric": "node-steps/sec, surrogate+FP32 vs plain FP64, final length err <=0.5%",
+        "value": mixed["rate"],
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "Taylor bar 25.4 mm x r 3.8 mm, h 0.088 mm, jitter 5% seed 2, copper J2 linear "
+                               "hardening, 190 m/s, Lagrangian DNI, mixed precision (surrogate off)",
+                   "nodes": n, "pairs": nnz, "mean_neighbors": nnz / n, "max_neighbors": mixed["max_neighbors"],
+                   "dt_s": prob.dt, "surrogate": False},
+        "fp64_value": res["fp64"]["rate"],
+        "fp64_ms_per_step": 1e3 * res["fp64"]["wall"] / args.steps,
+        "mixed_over_fp64": mixed["rate"] / res["fp64"]["rate"],
+        "step_roofline_frac": mixed["rate"] * node_step_bytes(n, nnz, True) / PEAK_HBM,
+        "step_roofline_frac_fp64": res["fp64"]["rate"] * node_step_bytes(n, nnz, False) / PEAK_HBM,
+        "kernel_ms": {k: v[0] for k, v in mixed["kt"].items()},
+        "kernel_ms_fp64": {k: v[0] for k, v in res["fp64"]["kt"].items()},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
+                     "algorithmic_bytes": kb},
+        "validation": validation,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -58,25 +58,27 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kGroup = 16;      // lanes cooperating on one node's neighbor list
 constexpr int kXcd = 8;         // XCDs on MI355X
-constexpr int kPStride = 10;    // per-node P (9) + rho V J (gravity)
 
 // ---------------------------------------------------------------------------
 // kernel arguments
 
 struct Params {
     int n;
+    int64_t cap;                                     // component stride of the pair arrays
     // lists
     const int32_t *row_ptr, *col, *trow, *tsrc, *tpos;
-    void *shp, *dshp, *tdshp, *tshp;                 // R
+    void *shp, *dshp, *tdshp, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap
     // node arrays, FP64 masters
     const double *X0, *win, *vol, *mass;
     double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
-    float *d32, *dtot32;                             // gather copies (mixed)
+    double *Fst;                                     // deformation gradient per node (9, column-major)
+    float4 *d4, *t4;                                 // FP32 gather copies of d and dtot (mixed)
     // state and scratch in R
     void *stress, *strain, *eps, *red, *P;
     const int32_t *ebc, *ebc_node, *mat;
     const Material *mats;
-    double *acc;                                     // [0] work [1] ke [2] ie [3] ee
+    double *wpart;                                   // per-block cumulative work (k_node)
+    double *epart;                                   // per-block [ke, ie, ee cumulative] (k_force)
     int32_t *flags;                                  // [0] return-map failures [1] bad material
     int ql;
     double ql_coef, ql_len;
@@ -86,6 +88,10 @@ struct Params {
 };
 
 template <class R> __device__ __forceinline__ R *as(void *p) { return static_cast<R *>(p); }
+template <class R> constexpr int pstride() { return std::is_same<R, float>::value ? 12 : 10; }
+
+// deformation-gradient handling of the gather/node pair of kernels
+enum FMode { F_NONE = 0, F_DIRECT = 1, F_INCR = 2 };
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -134,7 +140,11 @@ __device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
     p.d[q] = d;
     p.dtot[q] = t;
     p.v[q] = v + dt * 0.5 * a;
-    if (p.d32) { p.d32[q] = (float)d; p.dtot32[q] = (float)t; }
+    if (p.d4) {
+        const int64_t i = q / 3, k = q - 3 * i;
+        reinterpret_cast<float *>(p.d4)[4 * i + k] = (float)d;
+        reinterpret_cast<float *>(p.t4)[4 * i + k] = (float)t;
+    }
 }
 
 __global__ void k_predict(Params p)
@@ -144,10 +154,10 @@ __global__ void k_predict(Params p)
     predict_dof(p, q);
 }
 
-__global__ void k_cast32(int64_t m, const double *a, float *b)
+__global__ void k_cast4(int n, const double *a, float4 *b)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < m) b[q] = (float)a[q];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = make_float4((float)a[3 * i], (float)a[3 * i + 1], (float)a[3 * i + 2], 0.f);
 }
 
 template <class R>
@@ -344,79 +354,99 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
         const R f = (R)(c0 * e.phi);
         const R gx = (R)(cx * e.phi + c0 * e.px), gy = (R)(cy * e.phi + c0 * e.py), gz = (R)(cz * e.phi + c0 * e.pz);
         shp[q] = f;
-        dshp[3 * q] = gx; dshp[3 * q + 1] = gy; dshp[3 * q + 2] = gz;
+        dshp[q] = gx; dshp[p.cap + q] = gy; dshp[2 * p.cap + q] = gz;
         if (p.tpos) {
             const int t = p.tpos[q];
-            tdshp[3 * t] = gx; tdshp[3 * t + 1] = gy; tdshp[3 * t + 2] = gz;
+            tdshp[t] = gx; tdshp[p.cap + t] = gy; tdshp[2 * p.cap + t] = gz;
             if (tshp) tshp[t] = f;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// gather phase of CONSTRUCT_FINT.  FP64: red = [F-I (9) | L0 (9)] with
-// F(k,l) = sum_j g_l dtot_jk, L0(k,l) = sum_j g_l d_jk (column-major, FINT.F90:
-// 363-376, 673-679).  Mixed: the FP32 sums are taken over differences to the
-// node's own value, red = [F' (9) | L' (9) | S = sum g (3)], and k_node adds
-// S (x) own value back in FP64; this keeps the FP32 sums free of the rigid
-// translation.  UPDATE_X: DO_INTERP is fused, u = sum phi d updates the
-// physical displacement and coordinates.
+// gather phase of CONSTRUCT_FINT, kGroup lanes per node.  Sums per node:
+//   L0(k,l) = sum_j g_l d_jk                        (FINT.F90:673-679)
+//   F(k,l)  = sum_j g_l dtot_jk  (F_DIRECT only)    (FINT.F90:363-376)
+// With F_INCR the deformation gradient is carried per node instead,
+// F_n = F_{n-1} + L0_n (dtot_n = dtot_{n-1} + d_n with the same fixed
+// gradients), so dtot is not gathered.  Mixed: the FP32 sums are taken over
+// differences to the node's own value plus S = sum g, and k_node adds
+// S (x) own value back; this keeps the rigid translation out of the FP32 sums.
+// red per node: FP64 [L0 | F], mixed [L0 | S | F].  UPDATE_X: DO_INTERP is
+// fused, u = sum phi d updates the physical displacement and coordinates.
+// Each lane issues the loads of up to U list entries before using any of them.
 
-template <class R, bool PUSH, bool UPDATE_X>
+template <class R, int FM, bool UPDATE_X>
 __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 {
+    constexpr bool MIX = std::is_same<R, float>::value;
+    constexpr bool DIR = FM == F_DIRECT;
+    constexpr int U = MIX ? 6 : 4;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    if (blk == 0 && threadIdx.x == 0) { p.acc[1] = 0.0; p.acc[2] = 0.0; }
     const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
     const int lane = threadIdx.x & (kGroup - 1);
     if (node >= p.n) return;
-    constexpr bool MIX = std::is_same<R, float>::value;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
-    const R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp);
-    R F[9], L[9], S[3], u[3];
+    const R *__restrict__ shp = as<R>(p.shp);
+    const R *__restrict__ gx = as<R>(p.dshp);
+    const R *__restrict__ gy = gx + p.cap;
+    const R *__restrict__ gz = gy + p.cap;
+    R L[9], F[9], S[3], u[3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) { F[k] = R(0); L[k] = R(0); }
+    for (int k = 0; k < 9; ++k) { L[k] = R(0); F[k] = R(0); }
     S[0] = S[1] = S[2] = R(0);
     u[0] = u[1] = u[2] = R(0);
-    R di[3] = {R(0), R(0), R(0)}, ti[3] = {R(0), R(0), R(0)};
+    float4 di = make_float4(0.f, 0.f, 0.f, 0.f), ti = di;
     if (MIX) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { di[k] = (R)p.d32[3 * node + k]; ti[k] = (R)p.dtot32[3 * node + k]; }
+        di = p.d4[node];
+        if (DIR) ti = p.t4[node];
     }
-    for (int q = p0 + lane; q < p1; q += kGroup) {
-        const int j = p.col[q];
-        const R g0 = dshp[3 * q], g1 = dshp[3 * q + 1], g2 = dshp[3 * q + 2];
-        R dj[3], tj[3];
-        if (MIX) {
+    for (int base = p0 + lane; base < p1; base += kGroup * U) {
+        int j[U];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                dj[k] = (R)p.d32[3 * j + k] - di[k];
-                if (PUSH) tj[k] = (R)p.dtot32[3 * j + k] - ti[k];
-            }
-            S[0] += g0; S[1] += g1; S[2] += g2;
-        } else {
+        for (int e = 0; e < U; ++e) {
+            const int q = base + kGroup * e;
+            j[e] = q < p1 ? p.col[q] : node;
+        }
+        R g0[U], g1[U], g2[U], f[U], dj[U][3], tj[U][3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                dj[k] = (R)p.d[3 * j + k];
-                if (PUSH) tj[k] = (R)p.dtot[3 * j + k];
+        for (int e = 0; e < U; ++e) {
+            const int q = base + kGroup * e;
+            const bool ok = q < p1;
+            g0[e] = ok ? gx[q] : R(0);
+            g1[e] = ok ? gy[q] : R(0);
+            g2[e] = ok ? gz[q] : R(0);
+            f[e] = (UPDATE_X && ok) ? shp[q] : R(0);
+            if (MIX) {
+                const float4 dv = p.d4[j[e]];
+                dj[e][0] = dv.x - di.x; dj[e][1] = dv.y - di.y; dj[e][2] = dv.z - di.z;
+                if (DIR) {
+                    const float4 tv = p.t4[j[e]];
+                    tj[e][0] = tv.x - ti.x; tj[e][1] = tv.y - ti.y; tj[e][2] = tv.z - ti.z;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    dj[e][k] = (R)p.d[3 * j[e] + k];
+                    if (DIR) tj[e][k] = (R)p.dtot[3 * j[e] + k];
+                }
             }
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { L[k] += g0 * dj[k]; L[3 + k] += g1 * dj[k]; L[6 + k] += g2 * dj[k]; }
-        if (PUSH) {
+        for (int e = 0; e < U; ++e) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) { F[k] += g0 * tj[k]; F[3 + k] += g1 * tj[k]; F[6 + k] += g2 * tj[k]; }
-        }
-        if (UPDATE_X) {
-            const R f = shp[q];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) u[k] += f * dj[k];
+            for (int k = 0; k < 3; ++k) {
+                L[k] += g0[e] * dj[e][k]; L[3 + k] += g1[e] * dj[e][k]; L[6 + k] += g2[e] * dj[e][k];
+                if (DIR) { F[k] += g0[e] * tj[e][k]; F[3 + k] += g1[e] * tj[e][k]; F[6 + k] += g2[e] * tj[e][k]; }
+                if (UPDATE_X) u[k] += f[e] * dj[e][k];
+            }
+            if (MIX) { S[0] += g0[e]; S[1] += g1[e]; S[2] += g2[e]; }
         }
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) L[k] = group_sum<kGroup>(L[k]);
-    if (PUSH) {
+    if (DIR) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) F[k] = group_sum<kGroup>(F[k]);
     }
@@ -429,11 +459,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
         for (int k = 0; k < 3; ++k) u[k] = group_sum<kGroup>(u[k]);
     }
     constexpr int W = MIX ? 21 : 18;
+    constexpr int FO = MIX ? 12 : 9;
     R *red = as<R>(p.red) + (int64_t)W * node;
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { red[k] = F[k]; red[9 + k] = L[k]; }
-        if (MIX) { red[18] = S[0]; red[19] = S[1]; red[20] = S[2]; }
+        for (int k = 0; k < 9; ++k) red[k] = L[k];
+        if (MIX) { red[9] = S[0]; red[10] = S[1]; red[11] = S[2]; }
+        if (DIR) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) red[FO + k] = F[k];
+        }
     }
     if (UPDATE_X && lane < 3) {
         const int qd = 3 * node + lane;
@@ -448,39 +483,58 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 
 // ---------------------------------------------------------------------------
 // per-node update of CONSTRUCT_FINT (FINT.F90:353-391, 758-870), one lane per
-// node: push-forward by INV3 (adj F * det F, the reference's own INV3), L,
-// Hughes-Winget rotation (FP64), rotated stress/strain, elastic trial and J2
-// return (FP64 or FP32), state write-back, work, wave speed, and
-// P = V J sigma F^-T for the force gather.
+// node: F and J = |det F|, push-forward by INV3 (adj F * det F, the
+// reference's own INV3), L, Hughes-Winget rotation (FP64), rotated
+// stress/strain, elastic trial and J2 return (FP64 or FP32), state
+// write-back, work, wave speed, and P = V J sigma F^-T for the force gather.
 
-template <class R, bool PUSH>
+template <class R, int FM>
 __global__ __launch_bounds__(kBlock) void k_node(Params p)
 {
     __shared__ double s_red[kBlock / 64];
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr int W = MIX ? 21 : 18;
+    constexpr int FO = MIX ? 12 : 9;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double work = 0.0;
     if (i < p.n) {
         const R *red = as<R>(p.red) + (int64_t)W * i;
-        double Fm[9], L0[9];
+        double L0[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { Fm[k] = (double)red[k]; L0[k] = (double)red[9 + k]; }
-        if (MIX) {   // add back S (x) own value: F(k,l) += S_l dtot_ik, L0(k,l) += S_l d_ik
-            const double S[3] = {(double)red[18], (double)red[19], (double)red[20]};
+        for (int k = 0; k < 9; ++k) L0[k] = (double)red[k];
+        double S[3] = {0, 0, 0}, dI[3] = {0, 0, 0}, tI[3] = {0, 0, 0};
+        if (MIX) {   // add back S (x) own value: L0(k,l) += S_l d_ik
+            const float4 dv = p.d4[i];
+            S[0] = (double)red[9]; S[1] = (double)red[10]; S[2] = (double)red[11];
+            dI[0] = dv.x; dI[1] = dv.y; dI[2] = dv.z;
 #pragma unroll
             for (int l = 0; l < 3; ++l)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    if (PUSH) Fm[k + 3 * l] += S[l] * (double)p.dtot32[3 * i + k];
-                    L0[k + 3 * l] += S[l] * (double)p.d32[3 * i + k];
-                }
+                for (int k = 0; k < 3; ++k) L0[k + 3 * l] += S[l] * dI[k];
         }
         const Material &mt = p.mats[p.mat[i]];
         const double vol = p.vol[i];
         double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Lm[9];
-        if (PUSH) {
-            Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
+        if (FM != F_NONE) {
+            double Fm[9];
+            if (FM == F_DIRECT) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) Fm[k] = (double)red[FO + k];
+                if (MIX) {
+                    const float4 tv = p.t4[i];
+                    tI[0] = tv.x; tI[1] = tv.y; tI[2] = tv.z;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) Fm[k + 3 * l] += S[l] * tI[k];
+                }
+                Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) Fm[k] = p.Fst[9 * (int64_t)i + k] + L0[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) p.Fst[9 * (int64_t)i + k] = Fm[k];
             det = mega::det3_abs(Fm);
             mega::inv3_ref(Fm, IF);
             mega::mm3(L0, IF, Lm);   // L = L0 F^-1 (FINT.F90:380-391 then 669-679)
@@ -535,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_node(Params p)
         // P = V J sigma IF^T  (f_j = P grad phi_ij)
         const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
         const double w = vol * det;
-        R *P = as<R>(p.P) + kPStride * (int64_t)i;
+        R *P = as<R>(p.P) + pstride<R>() * (int64_t)i;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -544,17 +598,34 @@ __global__ __launch_bounds__(kBlock) void k_node(Params p)
         P[9] = (R)(mt.rho * w);
     }
     const double ws = block_sum(work, s_red);
-    if (threadIdx.x == 0 && ws != 0.0) atomicAdd(&p.acc[0], ws);
+    if (threadIdx.x == 0) p.wpart[blockIdx.x] += ws;
 }
 
 // ---------------------------------------------------------------------------
 // force gather over the transposed lists, then (CORRECT) EBC, damping and the
 // corrector, energies, and (PREDICT) the next predictor, per degree of freedom.
 
+template <class R>
+__device__ __forceinline__ void load_P(const R *Pb, int i, R *P)
+{
+    if (std::is_same<R, float>::value) {
+        const float4 *q = reinterpret_cast<const float4 *>(Pb + 12 * (int64_t)i);
+        const float4 a = q[0], b = q[1], c = q[2];
+        P[0] = a.x; P[1] = a.y; P[2] = a.z; P[3] = a.w; P[4] = b.x; P[5] = b.y; P[6] = b.z; P[7] = b.w;
+        P[8] = c.x; P[9] = c.y;
+    } else {
+        const double2 *q = reinterpret_cast<const double2 *>(Pb + 10 * (int64_t)i);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { const double2 v = q[k]; P[2 * k] = (R)v.x; P[2 * k + 1] = (R)v.y; }
+    }
+}
+
 template <class R, bool CORRECT, bool PREDICT>
 __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
 {
     __shared__ double s_red[kBlock / 64];
+    constexpr bool MIX = std::is_same<R, float>::value;
+    constexpr int U = MIX ? 6 : 4;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
     const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
@@ -563,15 +634,36 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
     R f0 = R(0), f1 = R(0), f2 = R(0), fe = R(0);
     if (live) {
         const int t0 = p.trow[node], t1 = p.trow[node + 1];
-        const R *tg = as<R>(p.tdshp), *Pb = as<R>(p.P), *ts = as<R>(p.tshp);
-        for (int t = t0 + lane; t < t1; t += kGroup) {
-            const int i = p.tsrc[t];
-            const R g0 = tg[3 * t], g1 = tg[3 * t + 1], g2 = tg[3 * t + 2];
-            const R *P = Pb + kPStride * (int64_t)i;
-            f0 += P[0] * g0 + P[3] * g1 + P[6] * g2;
-            f1 += P[1] * g0 + P[4] * g1 + P[7] * g2;
-            f2 += P[2] * g0 + P[5] * g1 + P[8] * g2;
-            if (p.has_gravity) fe += ts[t] * P[9];
+        const R *__restrict__ tx = as<R>(p.tdshp);
+        const R *__restrict__ ty = tx + p.cap;
+        const R *__restrict__ tz = ty + p.cap;
+        const R *__restrict__ Pb = as<R>(p.P);
+        const R *__restrict__ ts = as<R>(p.tshp);
+        for (int base = t0 + lane; base < t1; base += kGroup * U) {
+            int src[U];
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                const int t = base + kGroup * e;
+                src[e] = t < t1 ? p.tsrc[t] : node;
+            }
+            R g0[U], g1[U], g2[U], P[U][10], sh[U];
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                const int t = base + kGroup * e;
+                const bool ok = t < t1;
+                g0[e] = ok ? tx[t] : R(0);
+                g1[e] = ok ? ty[t] : R(0);
+                g2[e] = ok ? tz[t] : R(0);
+                sh[e] = (ok && p.has_gravity) ? ts[t] : R(0);
+                load_P<R>(Pb, src[e], P[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                f0 += P[e][0] * g0[e] + P[e][3] * g1[e] + P[e][6] * g2[e];
+                f1 += P[e][1] * g0[e] + P[e][4] * g1[e] + P[e][7] * g2[e];
+                f2 += P[e][2] * g0[e] + P[e][5] * g1[e] + P[e][8] * g2[e];
+                fe += sh[e] * P[e][9];
+            }
         }
     }
     f0 = group_sum<kGroup>(f0); f1 = group_sum<kGroup>(f1); f2 = group_sum<kGroup>(f2);
@@ -607,14 +699,14 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
             if (PREDICT) predict_dof(p, q);
         }
     }
-    if (CORRECT) {
+    if (CORRECT) {   // per-block partials: no same-address atomics, deterministic
         const double sk = block_sum(ke, s_red);
         const double si = block_sum(ie, s_red);
         const double se = block_sum(ee, s_red);
         if (threadIdx.x == 0) {
-            if (sk != 0.0) atomicAdd(&p.acc[1], sk);
-            if (si != 0.0) atomicAdd(&p.acc[2], si);
-            if (se != 0.0) atomicAdd(&p.acc[3], se);
+            p.epart[3 * blk] = sk;
+            p.epart[3 * blk + 1] = si;
+            p.epart[3 * blk + 2] += se;
         }
     }
 }
@@ -830,7 +922,9 @@ struct mega_ctx {
     double *X0 = nullptr, *x = nullptr, *v = nullptr, *a = nullptr, *d = nullptr, *dtot = nullptr, *uphy = nullptr;
     double *fint = nullptr, *fext = nullptr, *fext_nmo = nullptr, *win = nullptr, *vol = nullptr, *mass = nullptr;
     double *wave_vel = nullptr, *scratch = nullptr;   // scratch: 6n doubles
-    float *d32 = nullptr, *dtot32 = nullptr;
+    float4 *d4 = nullptr, *t4 = nullptr;             // FP32 gather copies of d, dtot (mixed)
+    double *Fst = nullptr;                           // deformation gradient per node
+    int F_valid = 0;                                 // Fst consistent with dtot
     int32_t *ebc = nullptr, *ebc_node = nullptr, *mat = nullptr;
     Material *mats = nullptr;
     // R arrays
@@ -845,7 +939,8 @@ struct mega_ctx {
     int64_t ncell = 0;
     int32_t *ijk = nullptr, *cell_start = nullptr, *cell_nodes = nullptr, *cell_cursor = nullptr;
     // misc
-    double *acc = nullptr;
+    double *wpart = nullptr, *epart = nullptr;      // per-block work / energy partial sums
+    int nb_node = 0, nb_force = 0;
     int32_t *flags = nullptr;
     void *cub_tmp = nullptr;
     size_t cub_tmp_bytes = 0;
@@ -887,15 +982,16 @@ Params params(mega_ctx *c)
 {
     Params p{};
     p.n = c->n;
+    p.cap = c->nnz_cap;
     p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
     p.shp = c->shp; p.dshp = c->dshp; p.tdshp = c->tdshp; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
     p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
-    p.d32 = c->d32; p.dtot32 = c->dtot32;
+    p.d4 = c->d4; p.t4 = c->t4; p.Fst = c->Fst;
     p.stress = c->stress; p.strain = c->strain; p.eps = c->eps; p.red = c->red; p.P = c->P;
     p.ebc = c->ebc; p.ebc_node = c->ebc_node; p.mat = c->mat; p.mats = c->mats;
-    p.acc = c->acc; p.flags = c->flags;
+    p.wpart = c->wpart; p.epart = c->epart; p.flags = c->flags;
     p.ql = c->c.ql; p.ql_coef = c->c.ql_coef; p.ql_len = c->c.ql_len;
     p.shsup = c->c.shsup; p.finite_strain = c->c.finite_strain; p.has_gravity = c->has_gravity;
     for (int k = 0; k < 3; ++k) p.grav[k] = c->c.gravity[k];
@@ -1063,23 +1159,20 @@ int do_shapes(mega_ctx *c, const double *xs)
     return MEGA_OK;
 }
 
-template <class R>
-int fint_kernels(mega_ctx *c, bool push, bool update_x, int force_mode)
+template <class R, int FM>
+int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
 {
     Params p = params(c);
     int nb = 0;
     const unsigned g = xcd_grid(c->n, &nb);
     {
         Scope s(c, "k_gather");
-        if (push && update_x) k_gather<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else if (push) k_gather<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else if (update_x) k_gather<R, false, true><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else k_gather<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        if (update_x) k_gather<R, FM, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else k_gather<R, FM, false><<<g, kBlock, 0, c->stream>>>(p, nb);
     }
     {
         Scope s(c, "k_node");
-        if (push) k_node<R, true><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
-        else k_node<R, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+        k_node<R, FM><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
     }
     if (force_mode >= 0) {
         Scope s(c, "k_force");
@@ -1088,15 +1181,40 @@ int fint_kernels(mega_ctx *c, bool push, bool update_x, int force_mode)
         else k_force<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
     }
     HIPCHK(hipGetLastError());
+    if (FM == F_DIRECT) c->F_valid = 1;
     return MEGA_OK;
 }
 
-// force_mode: -1 no force, 0 force only, 1 force + corrector, 2 + next predictor
-int do_fint(mega_ctx *c, bool push, bool update_x, int force_mode)
+template <class R>
+int fint_kernels(mega_ctx *c, int fm, bool update_x, int force_mode)
+{
+    if (fm == F_DIRECT) return fint_kernels_fm<R, F_DIRECT>(c, update_x, force_mode);
+    if (fm == F_INCR) return fint_kernels_fm<R, F_INCR>(c, update_x, force_mode);
+    return fint_kernels_fm<R, F_NONE>(c, update_x, force_mode);
+}
+
+// fm: F_NONE (semi-Lagrangian / small strain), F_DIRECT (F from dtot), F_INCR
+// (F carried per node); force_mode: -1 no force, 0 force only, 1 force +
+// corrector, 2 + next predictor
+int do_fint(mega_ctx *c, int fm, bool update_x, int force_mode)
 {
     if (!c->has_shapes) return MEGA_ERR_ARG;
-    return c->mixed ? fint_kernels<float>(c, push, update_x, force_mode)
-                    : fint_kernels<double>(c, push, update_x, force_mode);
+    if (fm == F_INCR && !c->F_valid) fm = F_DIRECT;
+    return c->mixed ? fint_kernels<float>(c, fm, update_x, force_mode)
+                    : fint_kernels<double>(c, fm, update_x, force_mode);
+}
+
+double host_sum(mega_ctx *c, const double *dev, int64_t count, int64_t stride, int64_t offset, int *err)
+{
+    std::vector<double> h((size_t)count * stride);
+    if (hipMemcpyAsync(h.data(), dev, sizeof(double) * count * stride, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        *err = MEGA_ERR_HIP;
+        return 0.0;
+    }
+    double s = 0.0;
+    for (int64_t b = 0; b < count; ++b) s += h[(size_t)(b * stride + offset)];
+    return s;
 }
 
 int do_predict(mega_ctx *c)
@@ -1108,12 +1226,11 @@ int do_predict(mega_ctx *c)
     return MEGA_OK;
 }
 
-int sync_d32(mega_ctx *c)
+int sync_d4(mega_ctx *c)
 {
     if (!c->mixed) return MEGA_OK;
-    const int64_t m = 3 * (int64_t)c->n;
-    k_cast32<<<grid_for(m), kBlock, 0, c->stream>>>(m, c->d, c->d32);
-    k_cast32<<<grid_for(m), kBlock, 0, c->stream>>>(m, c->dtot, c->dtot32);
+    k_cast4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->d, c->d4);
+    k_cast4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->dtot, c->t4);
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }
@@ -1233,17 +1350,21 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->d, n3); e |= dalloc(&c->dtot, n3); e |= dalloc(&c->uphy, n3); e |= dalloc(&c->fint, n3);
     e |= dalloc(&c->fext, n3); e |= dalloc(&c->scratch, n6);
     if (c->has_gravity) e |= dalloc(&c->fext_nmo, n3);
-    if (c->mixed) { e |= dalloc(&c->d32, n3); e |= dalloc(&c->dtot32, n3); }
+    if (c->mixed) { e |= dalloc(&c->d4, n); e |= dalloc(&c->t4, n); }
     else e |= dalloc(&c->wave_vel, n);
+    e |= dalloc(&c->Fst, 9 * (size_t)n);
     e |= dalloc(&c->win, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
     e |= dalloc(&c->mats, c->host_mats.size());
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
-    e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, kPStride * (size_t)n, r);
+    e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, (c->mixed ? 12 : 10) * (size_t)n, r);
     e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
     e |= dalloc(&c->cell_nodes, n);
-    e |= dalloc(&c->acc, 8); e |= dalloc(&c->flags, 4);
+    c->nb_node = (int)grid_for(n);
+    xcd_grid(n, &c->nb_force);
+    e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
+    e |= dalloc(&c->flags, 4);
     if (e) { mega_destroy(c); return MEGA_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { mega_destroy(c); return MEGA_ERR_HIP; }
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -1257,9 +1378,10 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     ok = ok && (vinit ? up(c->v, vinit, n3 * 8) : zero(c->v, n3 * 8));
     ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
-         zero(c->eps, n * r) && zero(c->acc, 64) && zero(c->flags, 16);
+         zero(c->eps, n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
+         zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
-    if (ok && c->mixed) ok = zero(c->d32, n3 * 4) && zero(c->dtot32, n3 * 4);
+    if (ok && c->mixed) ok = zero(c->d4, (size_t)n * 16) && zero(c->t4, (size_t)n * 16);
     if (ok && c->wave_vel) ok = zero(c->wave_vel, (size_t)n * 8);
     if (!ok) { mega_destroy(c); return MEGA_ERR_HIP; }
     *out = c;
@@ -1272,9 +1394,10 @@ int mega_destroy(mega_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_pairs(c);
     void *ptrs[] = {c->X0, c->x, c->v, c->a, c->d, c->dtot, c->uphy, c->fint, c->fext, c->fext_nmo, c->win,
-                    c->vol, c->mass, c->wave_vel, c->scratch, c->d32, c->dtot32, c->ebc, c->ebc_node, c->mat,
+                    c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->Fst, c->ebc, c->ebc_node, c->mat,
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
-                    c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->acc, c->flags, c->cub_tmp};
+                    c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
+                    c->cub_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -1343,20 +1466,26 @@ int mega_get_shapes(mega_ctx *c, double *shp, double *dshp, int64_t cap)
     const int64_t m = c->nnz;
     double *buf = nullptr;
     if (dalloc(&buf, 3 * (size_t)m)) return MEGA_ERR_NOMEM;
-    int st = MEGA_OK;
-    if (shp) {
-        if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)c->shp, buf);
-        else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)c->shp, buf);
-        if (hipMemcpyAsync(shp, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            st = MEGA_ERR_HIP;
+    // shp, then the x, y, z gradient blocks (stride nnz_cap on the device)
+    for (int part = 0; part < 4; ++part) {
+        const size_t off = part == 0 ? 0 : (size_t)(part - 1) * c->nnz_cap;
+        const void *src = part == 0 ? c->shp : (const char *)c->dshp + off * c->rsz;
+        double *dst = buf + (part == 0 ? 0 : (part - 1) * m);
+        if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)src, dst);
+        else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)src, dst);
+        if (part == 0) {
+            if (shp) HIPCHK(hipMemcpyAsync(shp, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
     }
-    if (dshp && !st) {
-        if (c->mixed) k_widen<float><<<grid_for(3 * m), kBlock, 0, c->stream>>>(3 * m, (const float *)c->dshp, buf);
-        else k_widen<double><<<grid_for(3 * m), kBlock, 0, c->stream>>>(3 * m, (const double *)c->dshp, buf);
-        if (hipMemcpyAsync(dshp, buf, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+    int st = MEGA_OK;
+    if (dshp) {
+        std::vector<double> h(3 * (size_t)m);
+        if (hipMemcpyAsync(h.data(), buf, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             st = MEGA_ERR_HIP;
+        for (int64_t q = 0; q < m && !st; ++q)
+            for (int k = 0; k < 3; ++k) dshp[3 * q + k] = h[(size_t)(k * m + q)];
     }
     (void)hipFree(buf);
     return st;
@@ -1394,23 +1523,25 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     HIPCHK(hipMemcpyAsync(c->d, dinc, b3, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->dtot, dinc_tot, b3, hipMemcpyHostToDevice, c->stream));
     if (x_cur) HIPCHK(hipMemcpyAsync(c->x, x_cur, b3, hipMemcpyHostToDevice, c->stream));
-    int r = sync_d32(c);
+    int r = sync_d4(c);
     if (!r) r = state_in(c, c->stress, 6 * (int64_t)n, stress);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_in(c, c->strain, 6 * (int64_t)n, strain);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_in(c, c->eps, n, eps.data());
     if (r) return r;
-    HIPCHK(hipMemsetAsync(c->acc, 0, sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
+    int err = 0;
+    const double w0 = host_sum(c, c->wpart, c->nb_node, 1, 0, &err);
+    if (err) return err;
     const bool lag = c->c.lagrangian;
     if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x);
     else if (!c->has_shapes) r = MEGA_ERR_ARG;
-    if (!r) r = do_fint(c, lag && c->c.finite_strain, false, 0);
+    if (!r) r = do_fint(c, (lag && c->c.finite_strain) ? F_DIRECT : F_NONE, false, 0);
     if (r) return r;
     const int st = check_flags(c);
-    double w = 0.0;
-    HIPCHK(hipMemcpyAsync(&w, c->acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const double w = host_sum(c, c->wpart, c->nb_node, 1, 0, &err) - w0;
+    if (err) return err;
     r = state_out(c, c->stress, 6 * (int64_t)n, stress);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_out(c, c->strain, 6 * (int64_t)n, strain);
@@ -1470,7 +1601,7 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
     if (c->profile) c->timer_used = 0;
     const int n = c->n;
     const bool lag = c->c.lagrangian;
-    const bool push = lag && c->c.finite_strain;
+    const int fm = (lag && c->c.finite_strain) ? F_INCR : F_NONE;
     HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         const bool first = (c->step == 0);
@@ -1479,17 +1610,17 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
         if (first) {
             // LINIT call of HANDELER with zero increments (MAIN.F90:320-334)
             HIPCHK(hipMemsetAsync(c->d, 0, sizeof(double) * 3 * n, c->stream));
-            r = sync_d32(c);
+            r = sync_d4(c);
             if (!r) r = search_step(c, c->x, 1);
             if (!r) r = do_shapes(c, lag ? c->X0 : c->x);
-            if (!r) r = do_fint(c, push, false, -1);
+            if (!r) r = do_fint(c, fm == F_INCR ? F_DIRECT : F_NONE, false, -1);
             if (r) return r;
         }
         // the predictor is fused into the previous k_force within one call
         if (!c->predicted) r = do_predict(c);
         if (r) return r;
         if (lag) {
-            r = do_fint(c, push, true, last ? 1 : 2);   // DO_INTERP fused into the gather
+            r = do_fint(c, fm, true, last ? 1 : 2);   // DO_INTERP fused into the gather
         } else {
             Params p = params(c);
             {
@@ -1499,7 +1630,7 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
             }
             r = search_step(c, c->x, first);
             if (!r) r = do_shapes(c, c->x);
-            if (!r) r = do_fint(c, false, false, last ? 1 : 2);
+            if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2);
         }
         if (r) return r;
         c->predicted = last ? 0 : 1;
@@ -1574,7 +1705,8 @@ int mega_upload(mega_ctx *c, int32_t field, const double *host)
         double *dst = field_ptr(c, field, &w);
         if (!dst || field == MEGA_F_WAVE_VEL) return MEGA_ERR_ARG;
         HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * w * n, hipMemcpyHostToDevice, c->stream));
-        if (field == MEGA_F_D || field == MEGA_F_DTOT) r = sync_d32(c);
+        if (field == MEGA_F_D || field == MEGA_F_DTOT) r = sync_d4(c);
+        if (field == MEGA_F_DTOT) c->F_valid = 0;
     }
     }
     if (r) return r;
@@ -1585,12 +1717,15 @@ int mega_upload(mega_ctx *c, int32_t field, const double *host)
 int mega_energies(mega_ctx *c, double *e_int, double *e_kin, double *e_ext)
 {
     if (!c) return MEGA_ERR_ARG;
-    double acc[4];
-    HIPCHK(hipMemcpyAsync(acc, c->acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (e_int) *e_int = c->c.finite_strain ? acc[0] : acc[2];
-    if (e_kin) *e_kin = acc[1];
-    if (e_ext) *e_ext = acc[3];
+    int err = 0;
+    const double ke = host_sum(c, c->epart, c->nb_force, 3, 0, &err);
+    const double ie = host_sum(c, c->epart, c->nb_force, 3, 1, &err);
+    const double ee = host_sum(c, c->epart, c->nb_force, 3, 2, &err);
+    const double wk = host_sum(c, c->wpart, c->nb_node, 1, 0, &err);
+    if (err) return err;
+    if (e_int) *e_int = c->c.finite_strain ? wk : ie;
+    if (e_kin) *e_kin = ke;
+    if (e_ext) *e_ext = ee;
     return MEGA_OK;
 }
 
