@@ -4,19 +4,23 @@ Workload: the Taylor bar of 25.4 mm x 3.8 mm radius on a cubic lattice of
 spacing 0.088 mm (jitter 5 % of the spacing, seed 2; ~1.75M nodes, ~58
 neighbors per node), copper J2 with linear hardening, 190 m/s onto a fixed
 plane, dt = 0.3 h / c_p, Lagrangian finite strain with direct nodal
-integration.  Synthetic data; no model weights are involved.
+integration.  Synthetic deck.
 
 A "step" is one explicit step of the whole model (predictor, interpolation,
 internal force with the constitution, corrector).  The timed region is K
-steps after W warm-up steps, with all inputs resident in HBM; it is run in
-FP64 (plain) and in mixed precision (FP32 shape data, stress, return map and
-force; FP64 positions, F, F^-1 and rotation).  value = mixed-precision
-node-steps/s; the FP64 rate and the ratio are reported beside it.
+steps after W warm-up steps, with all inputs resident in HBM, run in three
+modes: fp64 (plain), mixed (FP32 shape data, stress, return map and force;
+FP64 positions, F, F^-1 and rotation) and surrogate (mixed, with the learned
+J2 return map of taylor_amd/data/j2_copper.npz and the exact return map as
+fallback).  value = surrogate node-steps/s; the other rates are beside it.
 
-Unless --no-validate, both precisions are also run to 80 us simulated time
-and the final bar length (max z above the wall plane) and mushroom radius
-(max radius over the impact-face nodes) of the mixed run are compared with
-the FP64 run (gate: length within 0.5 %).
+Unless --no-validate: (1) the three modes run the benchmark bar for
+--stable-steps steps (the window in which the reference formulation stays
+finite at this resolution; it loses stability later in FP64 too) and their
+bar length, mushroom radius and eqps field are compared with FP64; (2) the
+whole 80 us CPU-runnable bar (spacing 0.5 mm) runs in the three modes and is
+compared with FP64 and with the reference program's final state
+(tests/golden/taylor_h05_final.npz).  Gate: length within 0.5 %.
 
 roofline: the dominant kernel's algorithmic bytes per launch (compulsory
 HBM traffic: every pair and node array it must touch read or written once,
@@ -30,7 +34,8 @@ a 2.54 mm slice of the bar, ~181k nodes): node-steps/s from the difference of
 a 12-step and a 2-step run (setup cancels).  Falls back to the oracle
 restatement when the reference build is absent.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-validate] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes fp64,mixed,surrogate]
+                    [--stable-steps S] [--no-validate] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -49,6 +54,7 @@ sys.path.insert(0, ROOT)
 
 from taylor_amd import deck  # noqa: E402
 from taylor_amd import mega  # noqa: E402
+from taylor_amd import surrogate  # noqa: E402
 
 H = 0.088e-3
 SEED = 2
@@ -100,35 +106,95 @@ def read_traffic(kernel: str, mixed: bool):
     return None
 
 
-def timed_run(prob, precision: int, steps: int, warmup: int):
-    g = mega.Mega(prob, precision=precision)
+MODES = {"fp64": mega.MEGA_FP64, "mixed": mega.MEGA_MIXED, "surrogate": mega.MEGA_MIXED}
+
+
+def make(prob, mode: str):
+    g = mega.Mega(prob, precision=MODES[mode])
+    if mode == "surrogate":
+        surrogate.attach(g, surrogate.DEFAULT_WEIGHTS)
+    return g
+
+
+def timed_run(prob, mode: str, steps: int, warmup: int):
+    g = make(prob, mode)
     g.step(max(warmup, 1))               # includes the first step (search, shapes)
+    if mode == "surrogate":
+        g.surrogate_stats(reset=True)
     g.profile(True)
     t0 = time.perf_counter()
     g.step(steps)                        # returns after the stream is synchronised
     t1 = time.perf_counter()
     kt = g.kernel_times()
     nnz, mx, _ = g.stats()
+    st = g.surrogate_stats() if mode == "surrogate" else None
     g.close()
-    return (t1 - t0), kt, nnz, mx
+    return (t1 - t0), kt, nnz, mx, st
 
 
-def full_run(prob, precision: int):
-    g = mega.Mega(prob, precision=precision)
-    nsteps = int(np.floor(T_END / prob.dt)) + 1      # MAIN stops once TIME > TIME_END
+def run_to(prob, mode: str, nsteps: int):
+    """Final length (max z above the wall plane), mushroom radius (max radius
+    over the impact-face nodes) and eqps field after nsteps."""
+    g = make(prob, mode)
     done = 0
     while done < nsteps:
         k = min(2000, nsteps - done)
         g.step(k)
         done += k
-        log(f"  precision {precision}: {done}/{nsteps} steps")
     x = g.download("x")
     eps = g.download("eps")
+    st = g.surrogate_stats() if mode == "surrogate" else None
     g.close()
     wall = prob.sets["WALL"] - 1
-    length = float(x[:, 2].max())                   # wall plane z = 0
+    length = float(x[:, 2].max())
     radius = float(np.sqrt(x[wall, 0] ** 2 + x[wall, 1] ** 2).max())
-    return length, radius, eps, nsteps
+    return {"length_m": length, "radius_m": radius, "eps": eps, "surrogate_stats": st,
+            "finite": bool(np.isfinite(x).all())}
+
+
+def compare(prob, runs: dict, ref_key: str = "fp64") -> dict:
+    ref = runs[ref_key]
+    vol = prob.vol
+    out = {}
+    for k, r in runs.items():
+        d = {"length_m": r["length_m"], "radius_m": r["radius_m"], "finite": r["finite"]}
+        if k != ref_key:
+            d["length_rel_err"] = abs(r["length_m"] - ref["length_m"]) / ref["length_m"]
+            d["radius_rel_err"] = abs(r["radius_m"] - ref["radius_m"]) / ref["radius_m"]
+            d["eqps_l2_rel_err"] = float(np.sqrt((vol * (r["eps"] - ref["eps"]) ** 2).sum() /
+                                                 (vol * ref["eps"] ** 2).sum()))
+            d["length_gate_0p5pct"] = bool(d["length_rel_err"] <= 0.005)
+        if r.get("surrogate_stats"):
+            d["surrogate_stats"] = r["surrogate_stats"]
+        out[k] = d
+    return out
+
+
+def validate(prob, stable_steps: int):
+    """(1) the benchmark bar over the window in which the formulation stays
+    finite at this resolution; (2) the whole CPU-runnable bar (spacing 0.5 mm,
+    80 us), also against the reference program's final state when the fixture
+    is present."""
+    res = {}
+    runs = {m: run_to(prob, m, stable_steps) for m in MODES}
+    res["taylor_1p75M_window"] = {"steps": stable_steps, "t_end_s": stable_steps * prob.dt,
+                                  **compare(prob, runs)}
+    p5 = deck.taylor_problem(0.5e-3, v0=V0, seed=0)
+    n5 = int(np.floor(T_END / p5.dt)) + 1
+    runs = {m: run_to(p5, m, n5) for m in MODES}
+    out = {"steps": n5, "t_end_s": T_END, "nodes": p5.n, **compare(p5, runs)}
+    fix = os.path.join(ROOT, "tests", "golden", "taylor_h05_final.npz")
+    if os.path.exists(fix):
+        z = np.load(fix)
+        wall = p5.sets["WALL"] - 1
+        L0 = float(z["x"][:, 2].max())
+        R0 = float(np.sqrt(z["x"][wall, 0] ** 2 + z["x"][wall, 1] ** 2).max())
+        out["reference_program"] = {"length_m": L0, "radius_m": R0}
+        for m in MODES:
+            out[m]["length_rel_err_vs_reference"] = abs(runs[m]["length_m"] - L0) / L0
+            out[m]["radius_rel_err_vs_reference"] = abs(runs[m]["radius_m"] - R0) / R0
+    res["taylor_h0p5_80us"] = out
+    return res
 
 
 def cpu_baseline_reference(sample_len: float, cores: int):
@@ -170,18 +236,39 @@ def cpu_baseline_port(sample_len: float, cores: int):
                       f"({prob.n} nodes), FP64 Lagrangian, 10 steps after the first"}
 
 
+def copy_calibration():
+    """Five device-to-device copies of 1 GiB with torch's copy kernel: under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE this gives the counters' reading
+    for a known 1 GiB read and 1 GiB write per dispatch."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--modes", default="fp64,mixed,surrogate")
+    ap.add_argument("--stable-steps", type=int, default=1000)
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 1 GiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    modes = [m for m in args.modes.split(",") if m]
 
     t = time.time()
     prob = deck.taylor_problem(H, v0=V0, seed=SEED)
@@ -189,34 +276,26 @@ def main():
     log(f"workload: {n} nodes, dt {prob.dt:.4e} s, deck {time.time() - t:.1f} s")
 
     res = {}
-    for name, prec in (("fp64", mega.MEGA_FP64), ("mixed", mega.MEGA_MIXED)):
-        wall, kt, nnz, mx = timed_run(prob, prec, args.steps, args.warmup)
-        res[name] = dict(wall=wall, kt=kt, nnz=nnz, max_neighbors=mx, rate=n * args.steps / wall)
+    for name in modes:
+        wall, kt, nnz, mx, st = timed_run(prob, name, args.steps, args.warmup)
+        res[name] = dict(wall=wall, kt=kt, nnz=nnz, max_neighbors=mx, rate=n * args.steps / wall, stats=st)
         log(f"{name}: {1e3 * wall / args.steps:.3f} ms/step, {res[name]['rate']:.4e} node-steps/s, "
-            + ", ".join(f"{k} {v[0]:.3f} ms" for k, v in kt.items()))
+            + ", ".join(f"{k} {v[0]:.3f} ms" for k, v in kt.items()) + (f", surrogate {st}" if st else ""))
 
-    mixed = res["mixed"]
-    nnz = mixed["nnz"]
-    # dominant kernel of the reported (mixed) step
-    dom = max(mixed["kt"].items(), key=lambda kv: kv[1][0] * kv[1][1])
-    kname, (kms, _) = dom
-    kb = kernel_bytes(kname, n, nnz, True)
+    head = "surrogate" if "surrogate" in res else modes[-1]
+    top = res[head]
+    mixed_prec = head != "fp64"
+    nnz = top["nnz"]
+    # dominant kernel of the reported step
+    kname, (kms, _) = max(top["kt"].items(), key=lambda kv: kv[1][0] * kv[1][1])
+    kb = kernel_bytes(kname, n, nnz, mixed_prec)
     achieved = kb / (kms * 1e-3)
-    traffic = read_traffic(kname, True)
+    traffic = read_traffic(kname, mixed_prec)
 
     validation = None
     if not args.no_validate:
-        l64, r64, e64, ns = full_run(prob, mega.MEGA_FP64)
-        l32, r32, e32, _ = full_run(prob, mega.MEGA_MIXED)
-        vol = prob.vol
-        validation = {
-            "t_end_s": T_END, "steps": ns,
-            "length_fp64_m": l64, "length_mixed_m": l32, "length_rel_err": abs(l32 - l64) / l64,
-            "radius_fp64_m": r64, "radius_mixed_m": r32, "radius_rel_err": abs(r32 - r64) / r64,
-            "eqps_l2_rel_err": float(np.sqrt((vol * (e32 - e64) ** 2).sum() / (vol * e64 ** 2).sum())),
-            "length_gate_0p5pct": bool(abs(l32 - l64) / l64 <= 0.005),
-        }
-        log("validation:", validation)
+        validation = validate(prob, args.stable_steps)
+        log("validation:", json.dumps(validation))
 
     cpu = None
     if not args.no_cpu:
@@ -229,36 +308,39 @@ def main():
             cpu = cpu_baseline_port(2.54e-3, cores)
         log("cpu_baseline:", cpu)
 
+    st = top["stats"]
     out = {
         "metric": "node-steps/sec, surrogate+FP32 vs plain FP64, final length err <=0.5%",
-        "value": mixed["rate"],
+        "value": top["rate"],
         "unit": "node-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * mixed["wall"] / args.steps,
+        "ms_per_step": 1e3 * top["wall"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64, bf16 network" if head == "surrogate" else ("f32+f64" if mixed_prec else "f64"),
         "data": "synthetic",
-        "config": {"workload": "Taylor bar 25.4 mm x r 3.8 mm, h 0.088 mm, jitter 5% seed 2, copper J2 linear "
-                               "hardening, 190 m/s, Lagrangian DNI, mixed precision (surrogate off)",
-                   "nodes": n, "pairs": nnz, "mean_neighbors": nnz / n, "max_neighbors": mixed["max_neighbors"],
-                   "dt_s": prob.dt, "surrogate": False},
-        "fp64_value": res["fp64"]["rate"],
-        "fp64_ms_per_step": 1e3 * res["fp64"]["wall"] / args.steps,
-        "mixed_over_fp64": mixed["rate"] / res["fp64"]["rate"],
-        "step_roofline_frac": mixed["rate"] * node_step_bytes(n, nnz, True) / PEAK_HBM,
-        "step_roofline_frac_fp64": res["fp64"]["rate"] * node_step_bytes(n, nnz, False) / PEAK_HBM,
-        "kernel_ms": {k: v[0] for k, v in mixed["kt"].items()},
-        "kernel_ms_fp64": {k: v[0] for k, v in res["fp64"]["kt"].items()},
+        "config": {"workload": f"Taylor bar 25.4 mm x r 3.8 mm, h 0.088 mm, jitter 5% seed 2, copper J2 linear "
+                               f"hardening, 190 m/s, Lagrangian DNI, {head}",
+                   "nodes": n, "pairs": nnz, "mean_neighbors": nnz / n, "max_neighbors": top["max_neighbors"],
+                   "dt_s": prob.dt, "surrogate": head == "surrogate"},
+        "rates": {m: r["rate"] for m, r in res.items()},
+        "ms_per_step_by_mode": {m: 1e3 * r["wall"] / args.steps for m, r in res.items()},
+        "kernel_ms": {m: {k: v[0] for k, v in r["kt"].items()} for m, r in res.items()},
+        "surrogate_stats": st,
+        "surrogate_answered_frac": ((st["evaluated"] - st["rejected"]) / max(1, st["plastic"])) if st else None,
+        "step_roofline_frac": {m: r["rate"] * node_step_bytes(n, nnz, m != "fp64") / PEAK_HBM
+                               for m, r in res.items()},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "algorithmic_bytes": kb},
         "validation": validation,
         "cpu_baseline": cpu,
     }
+    if "fp64" in res:
+        out["value_over_fp64"] = top["rate"] / res["fp64"]["rate"]
     print(json.dumps(out))
 
 

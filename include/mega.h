@@ -139,7 +139,9 @@ int mega_upload(mega_ctx *ctx, int32_t field, const double *host);
 
 /* energies after the last step (MAIN.F90:524-553): internal, kinetic, external */
 int mega_energies(mega_ctx *ctx, double *e_int, double *e_kin, double *e_ext);
-/* mean/max neighbor count of the current lists (k-bar used by the byte model) */
+/* nnz and max neighbor count of the current lists (k-bar of the byte model);
+ * return_map_fallbacks: learned-return-map answers rejected so far (exact
+ * return map used instead), 0 without the surrogate. */
 int mega_stats(mega_ctx *ctx, int64_t *nnz, int32_t *max_neighbors, int64_t *return_map_fallbacks);
 
 /* Enable (1) or disable (0) per-kernel HIP-event timing of mega_step. */
@@ -151,6 +153,56 @@ int mega_profile(mega_ctx *ctx, int32_t enable);
 int mega_kernel_times(mega_ctx *ctx, const char **names, double *ms_avg, int32_t *launches, int32_t cap);
 
 const char *mega_status_string(int status);
+
+/* ---- learned J2 return map ------------------------------------------------
+ * No counterpart in the reference.  A 13-64-64-64-7 SiLU network (bf16
+ * weights on the matrix cores, fp32 accumulation) stands in for VON_MISES
+ * (CONSTITUTE_VONMISES.F90:21-158) at the CONSTITUTION seam (FINT.F90:800).
+ * Input x = (strain increment[6], rotated stress[6], eqps_n), normalised as
+ * xhat = (x - in_mean) * in_scale; output y maps to the plastic correction
+ * t = (sigma_trial - sigma[6], eqps - eqps_n) = y * out_scale, where
+ * sigma_trial = rotated stress + C:strain increment (FINT.F90:789-794).
+ * params[MEGA_MLP_PARAMS]: per layer W[out][in] row-major then b[out].
+ * norm[MEGA_MLP_NORM]: in_mean[13], in_scale[13], out_scale[7], envelope
+ * lo[13], hi[13] (min/max of the training inputs). */
+#define MEGA_MLP_PARAMS 9671
+#define MEGA_MLP_NORM 59
+typedef struct mega_mlp mega_mlp;
+/* trainer with Glorot-uniform weights from seed (splitmix64), zero biases */
+int mega_mlp_create(uint64_t seed, mega_mlp **out);
+int mega_mlp_destroy(mega_mlp *m);
+int mega_mlp_set_params(mega_mlp *m, const float *params);   /* resets the Adam moments */
+int mega_mlp_get_params(mega_mlp *m, float *params);
+int mega_mlp_get_norm(mega_mlp *m, float *norm);             /* after mega_mlp_set_data */
+/* training set: tuples[20 count] as written by mega_harvest_take, for the
+ * material of property row prop[30]; computes norm, normalises, shuffles. */
+int mega_mlp_set_data(mega_mlp *m, const float *tuples, int64_t count, const double *prop);
+/* iters Adam steps on batches of the (cyclically traversed) training set,
+ * learning rate cosine-decayed from lr0 to lr1; loss_history[iters] or NULL
+ * receives the mean squared error of each batch (normalised units). */
+int mega_mlp_train(mega_mlp *m, int32_t iters, int32_t batch, double lr0, double lr1, float *loss_history);
+/* y[7n] = network(xhat[13n]) */
+int mega_mlp_forward(mega_mlp *m, const float *xhat, float *y, int64_t n);
+/* gradient[MEGA_MLP_PARAMS] and value of (1/n) sum ||network(xhat) - that||^2 */
+int mega_mlp_gradient(mega_mlp *m, const float *xhat, const float *that, int64_t n, float *grad, float *loss);
+
+/* Use the network in the context's steps (MEGA_MIXED only) for the J2 nodes
+ * whose property row equals prop[30]: a plastic trial state inside the
+ * envelope takes the network's answer when its yield residual
+ * |sqrt(3/2)||dev sigma|| - sigma_y(eqps)| <= tol sigma_y(eqps) and eqps does
+ * not decrease; otherwise the exact return map runs.  params NULL disables. */
+int mega_set_surrogate(mega_ctx *ctx, const float *params, const float *norm, const double *prop, double tol);
+/* counters since the last reset: [0] plastic trial states of the trained
+ * material, [1] outside the envelope, [2] evaluated by the network, [3] of
+ * those, rejected by the residual test (exact return map used) */
+int mega_surrogate_stats(mega_ctx *ctx, int64_t *stats, int32_t reset);
+/* Record, every `every` steps of mega_step, one tuple per plastic J2 trial
+ * state: (strain increment[6], rotated stress[6], eqps_n, stress[6], eqps)
+ * as 20 floats, up to capacity tuples; capacity 0 stops and frees. */
+int mega_harvest(mega_ctx *ctx, int64_t capacity, int32_t every);
+/* copy up to cap tuples to host[20 cap]; *count receives the number
+ * recorded (it can exceed the capacity); the buffer is then emptied */
+int mega_harvest_take(mega_ctx *ctx, float *host, int64_t cap, int64_t *count);
 
 #ifdef __cplusplus
 }

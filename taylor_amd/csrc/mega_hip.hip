@@ -50,6 +50,7 @@
 
 #include "mega.h"
 #include "mega_math.hpp"
+#include "mlp_mfma.hpp"
 
 using mega::Material;
 
@@ -80,6 +81,16 @@ struct Params {
     double *wpart;                                   // per-block cumulative work (k_node)
     double *epart;                                   // per-block [ke, ie, ee cumulative] (k_force)
     int32_t *flags;                                  // [0] return-map failures [1] bad material
+    // learned return map (k_node, mixed precision)
+    const uint16_t *surr_tiles;                      // packed bf16 forward tiles
+    const float *surr_bias, *surr_norm;
+    int surr_mat;                                    // material slot it was trained for
+    float surr_tol;                                  // yield residual / sigma_y accepted
+    unsigned long long *sstat;                       // [plastic, outside envelope, evaluated, rejected]
+    // tuple harvest (k_node): 20 floats per plastic point
+    float *hv;
+    unsigned long long *hv_count;
+    int64_t hv_cap;
     int ql;
     double ql_coef, ql_len;
     int shsup, finite_strain, has_gravity;
@@ -482,94 +493,243 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 }
 
 // ---------------------------------------------------------------------------
+// learned return map (mixed precision) and tuple harvest, called by k_node with
+// every lane of the wave active
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lanes_below(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+struct SurrLds {
+    mlp::bf16x8 tiles[mlp::kTilesFwd * 64];
+    float bias[mlp::kBiasPack];
+    float norm[mlp::kNorm];
+    float stage[kBlock / 64][64][16];   // per wave: normalised inputs, then outputs
+};
+
+// cand: this lane's J2 trial state is plastic and belongs to the trained
+// material.  Inside the training envelope the network gives the plastic
+// correction; the answer is kept when the residual of the return-map
+// equations (mega::return_residual) is within tol and eps does not decrease,
+// otherwise the caller runs the exact return map.
+__device__ bool surr_return(const Params &p, bool cand, const float *der, const float *sr, const float *pred,
+                            float &ep, float *s_out, const Material &mt)
+{
+    __shared__ SurrLds L;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float x[mlp::kIn];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { x[k] = der[k]; x[6 + k] = sr[k]; }
+    x[12] = ep;
+    bool inenv = cand;
+    if (cand)
+#pragma unroll
+        for (int k = 0; k < mlp::kIn; ++k)
+            inenv = inenv && x[k] >= p.surr_norm[mlp::kEnvLo + k] && x[k] <= p.surr_norm[mlp::kEnvHi + k];
+    const uint64_t mc = __ballot(cand), mu = __ballot(inenv);
+    if (lane == 0 && mc) {
+        atomicAdd(&p.sstat[0], (unsigned long long)__popcll(mc));
+        atomicAdd(&p.sstat[1], (unsigned long long)(__popcll(mc) - __popcll(mu)));
+        atomicAdd(&p.sstat[2], (unsigned long long)__popcll(mu));
+    }
+    if (!__syncthreads_or(inenv)) return false;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(p.surr_tiles);
+        uint4 *dst = reinterpret_cast<uint4 *>(L.tiles);
+        for (int k = threadIdx.x; k < mlp::kTilesFwd * 64; k += kBlock) dst[k] = src[k];
+        for (int k = threadIdx.x; k < mlp::kBiasPack; k += kBlock) L.bias[k] = p.surr_bias[k];
+        for (int k = threadIdx.x; k < mlp::kNorm; k += kBlock) L.norm[k] = p.surr_norm[k];
+    }
+    __syncthreads();
+    const int nc = __popcll(mu);
+    const int slot = lanes_below(mu);
+    float(*st)[16] = L.stage[wv];
+    if (inenv) {
+#pragma unroll
+        for (int k = 0; k < mlp::kIn; ++k) st[slot][k] = (x[k] - L.norm[mlp::kInMean + k]) * L.norm[mlp::kInScale + k];
+        st[slot][13] = st[slot][14] = st[slot][15] = 0.0f;
+    }
+    wave_sync();
+    const int g = lane >> 4;
+    for (int c0 = 0; c0 < nc; c0 += 16) {
+        const int s = c0 + (lane & 15);
+        mlp::bf16x8 x0 = {};
+        if (g < 2 && s < nc) {
+            const float4 a = *reinterpret_cast<const float4 *>(&st[s][8 * g]);
+            const float4 b = *reinterpret_cast<const float4 *>(&st[s][8 * g + 4]);
+            x0 = mlp::pack8(mlp::f32x4{a.x, a.y, a.z, a.w}, mlp::f32x4{b.x, b.y, b.z, b.w});
+        }
+        mlp::f32x4 z1[4], z2[4], z3[4], y;
+        mlp::forward16(L.tiles, L.bias, x0, lane, z1, z2, z3, y);
+        wave_sync();
+        if (g < 2 && s < nc) *reinterpret_cast<float4 *>(&st[s][4 * g]) = float4{y[0], y[1], y[2], y[3]};
+    }
+    wave_sync();
+    bool ok = false;
+    if (inenv) {
+        const float *os = L.norm + mlp::kOutScale;
+        float s[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] = pred[k] - os[k] * st[slot][k];
+        const float e = ep + os[6] * st[slot][6];
+        ok = e >= ep && mega::return_residual(pred, s, ep, e, mt) <= p.surr_tol;
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s_out[k] = s[k];
+            ep = e;
+        }
+    }
+    const uint64_t mf = __ballot(inenv && !ok);
+    if (lane == 0 && mf) atomicAdd(&p.sstat[3], (unsigned long long)__popcll(mf));
+    return ok;
+}
+
+// append (de[6], rotated stress[6], eqps_n, stress[6], eqps) of lanes with h set
+template <class R>
+__device__ void harvest_put(const Params &p, bool h, const R *der, const R *s_in, R ep_n, const R *s_out, R ep)
+{
+    const uint64_t m = __ballot(h);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(p.hv_count, (unsigned long long)__popcll(m));
+    const uint32_t lo = __shfl((uint32_t)base, leader, 64), hi = __shfl((uint32_t)(base >> 32), leader, 64);
+    base = ((unsigned long long)hi << 32) | lo;
+    if (h) {
+        const unsigned long long slot = base + (unsigned long long)lanes_below(m);
+        if (slot < (unsigned long long)p.hv_cap) {
+            float *o = p.hv + 20 * slot;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                o[k] = (float)der[k];
+                o[6 + k] = (float)s_in[k];
+                o[13 + k] = (float)s_out[k];
+            }
+            o[12] = (float)ep_n;
+            o[19] = (float)ep;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-node update of CONSTRUCT_FINT (FINT.F90:353-391, 758-870), one lane per
 // node: F and J = |det F|, push-forward by INV3 (adj F * det F, the
 // reference's own INV3), L, Hughes-Winget rotation (FP64), rotated
-// stress/strain, elastic trial and J2 return (FP64 or FP32), state
-// write-back, work, wave speed, and P = V J sigma F^-T for the force gather.
+// stress/strain, elastic trial and J2 return (FP64 or FP32; SURR: the learned
+// return map first), state write-back, work, wave speed, and P = V J sigma F^-T
+// for the force gather.  Lanes past the last node compute on a copy of it and
+// store nothing, so that every lane reaches the wave-wide sections.
 
-template <class R, int FM>
+template <class R, int FM, bool SURR>
 __global__ __launch_bounds__(kBlock) void k_node(Params p)
 {
     __shared__ double s_red[kBlock / 64];
     constexpr bool MIX = std::is_same<R, float>::value;
+    static_assert(!SURR || MIX, "the learned return map runs in mixed precision");
     constexpr int W = MIX ? 21 : 18;
     constexpr int FO = MIX ? 12 : 9;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < p.n;
+    const int i = valid ? i0 : p.n - 1;
     double work = 0.0;
-    if (i < p.n) {
-        const R *red = as<R>(p.red) + (int64_t)W * i;
-        double L0[9];
+    const R *red = as<R>(p.red) + (int64_t)W * i;
+    double L0[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) L0[k] = (double)red[k];
-        double S[3] = {0, 0, 0}, dI[3] = {0, 0, 0}, tI[3] = {0, 0, 0};
-        if (MIX) {   // add back S (x) own value: L0(k,l) += S_l d_ik
-            const float4 dv = p.d4[i];
-            S[0] = (double)red[9]; S[1] = (double)red[10]; S[2] = (double)red[11];
-            dI[0] = dv.x; dI[1] = dv.y; dI[2] = dv.z;
+    for (int k = 0; k < 9; ++k) L0[k] = (double)red[k];
+    double S[3] = {0, 0, 0}, dI[3] = {0, 0, 0}, tI[3] = {0, 0, 0};
+    if (MIX) {   // add back S (x) own value: L0(k,l) += S_l d_ik
+        const float4 dv = p.d4[i];
+        S[0] = (double)red[9]; S[1] = (double)red[10]; S[2] = (double)red[11];
+        dI[0] = dv.x; dI[1] = dv.y; dI[2] = dv.z;
 #pragma unroll
-            for (int l = 0; l < 3; ++l)
+        for (int l = 0; l < 3; ++l)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) L0[k + 3 * l] += S[l] * dI[k];
-        }
-        const Material &mt = p.mats[p.mat[i]];
-        const double vol = p.vol[i];
-        double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Lm[9];
-        if (FM != F_NONE) {
-            double Fm[9];
-            if (FM == F_DIRECT) {
+            for (int k = 0; k < 3; ++k) L0[k + 3 * l] += S[l] * dI[k];
+    }
+    const int mi = p.mat[i];
+    const Material &mt = p.mats[mi];
+    const double vol = p.vol[i];
+    double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Lm[9];
+    if (FM != F_NONE) {
+        double Fm[9];
+        if (FM == F_DIRECT) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) Fm[k] = (double)red[FO + k];
-                if (MIX) {
-                    const float4 tv = p.t4[i];
-                    tI[0] = tv.x; tI[1] = tv.y; tI[2] = tv.z;
+            for (int k = 0; k < 9; ++k) Fm[k] = (double)red[FO + k];
+            if (MIX) {
+                const float4 tv = p.t4[i];
+                tI[0] = tv.x; tI[1] = tv.y; tI[2] = tv.z;
 #pragma unroll
-                    for (int l = 0; l < 3; ++l)
+                for (int l = 0; l < 3; ++l)
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) Fm[k + 3 * l] += S[l] * tI[k];
-                }
-                Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
-            } else {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) Fm[k] = p.Fst[9 * (int64_t)i + k] + L0[k];
+                    for (int k = 0; k < 3; ++k) Fm[k + 3 * l] += S[l] * tI[k];
             }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) p.Fst[9 * (int64_t)i + k] = Fm[k];
-            det = mega::det3_abs(Fm);
-            mega::inv3_ref(Fm, IF);
-            mega::mm3(L0, IF, Lm);   // L = L0 F^-1 (FINT.F90:380-391 then 669-679)
+            Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
         } else {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Lm[k] = L0[k];
+            for (int k = 0; k < 9; ++k) Fm[k] = p.Fst[9 * (int64_t)i + k] + L0[k];
         }
-        R *stress = as<R>(p.stress) + 6 * (int64_t)i, *strain = as<R>(p.strain) + 6 * (int64_t)i;
-        double sig[6], e6[6], de[6], dd[6], old_s[6], old_e[6];
+        if (valid)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)stress[k]; e6[k] = old_e[k] = (double)strain[k]; }
-        if (p.finite_strain) {
-            double Rot[9];
-            mega::hughes_winget(Lm, Rot, de, dd);
-            mega::rotate_voigt(Rot, sig);
-            mega::rotate_voigt(Rot, e6);
-        } else {   // small strain (FINT.F90:772-783)
-            de[0] = Lm[0]; de[1] = Lm[4]; de[2] = Lm[8];
-            de[3] = Lm[5] + Lm[7]; de[4] = Lm[2] + Lm[6]; de[5] = Lm[1] + Lm[3];
+            for (int k = 0; k < 9; ++k) p.Fst[9 * (int64_t)i + k] = Fm[k];
+        det = mega::det3_abs(Fm);
+        mega::inv3_ref(Fm, IF);
+        mega::mm3(L0, IF, Lm);   // L = L0 F^-1 (FINT.F90:380-391 then 669-679)
+    } else {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) dd[k] = 0.0;
-        }
-        R sr[6], der[6], pred[6];
+        for (int k = 0; k < 9; ++k) Lm[k] = L0[k];
+    }
+    R *stress = as<R>(p.stress) + 6 * (int64_t)i, *strain = as<R>(p.strain) + 6 * (int64_t)i;
+    double sig[6], e6[6], de[6], dd[6], old_s[6], old_e[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) { e6[k] = e6[k] + de[k]; sr[k] = (R)sig[k]; der[k] = (R)de[k]; }
-        mega::elastic_trial(sr, der, mt, pred);
-        R ep = as<R>(p.eps)[i];
+    for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)stress[k]; e6[k] = old_e[k] = (double)strain[k]; }
+    if (p.finite_strain) {
+        double Rot[9];
+        mega::hughes_winget(Lm, Rot, de, dd);
+        mega::rotate_voigt(Rot, sig);
+        mega::rotate_voigt(Rot, e6);
+    } else {   // small strain (FINT.F90:772-783)
+        de[0] = Lm[0]; de[1] = Lm[4]; de[2] = Lm[8];
+        de[3] = Lm[5] + Lm[7]; de[4] = Lm[2] + Lm[6]; de[5] = Lm[1] + Lm[3];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dd[k] = 0.0;
+    }
+    R sr[6], der[6], pred[6], s_in[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { e6[k] = e6[k] + de[k]; sr[k] = s_in[k] = (R)sig[k]; der[k] = (R)de[k]; }
+    mega::elastic_trial(sr, der, mt, pred);
+    R ep = as<R>(p.eps)[i];
+    const R ep_n = ep;
+    bool done = false;
+    if constexpr (SURR) {
+        const bool cand = valid && mt.type == 2 && mi == p.surr_mat && mega::trial_excess(pred, ep, mt) > 0.0f;
+        float s_new[6];
+        done = surr_return(p, cand, der, sr, pred, ep, s_new, mt);
+        if (done)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sr[k] = s_new[k];
+    }
+    if (!done) {
         if (mt.type == 2) {
-            if (mega::von_mises(sr, pred, ep, mt) < 0) atomicAdd(&p.flags[0], 1);
+            if (mega::von_mises(sr, pred, ep, mt) < 0 && valid) atomicAdd(&p.flags[0], 1);
         } else if (mt.type == 1) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) sr[k] = pred[k];
-        } else {
+        } else if (valid) {
             atomicAdd(&p.flags[1], 1);
         }
+    }
+    if (p.hv) harvest_put<R>(p, valid && mt.type == 2 && mega::trial_excess(pred, ep_n, mt) > R(0), der, s_in, ep_n,
+                             sr, ep);
+    if (valid) {
         as<R>(p.eps)[i] = ep;
         double si[6], ei[6];
 #pragma unroll
@@ -951,6 +1111,17 @@ struct mega_ctx {
     double dt = 0.0;
     int has_gravity = 0;
     std::vector<Material> host_mats;
+    // learned return map
+    int surr_on = 0, surr_mat = -1;
+    float surr_tol = 1e-3f;
+    uint16_t *surr_tiles = nullptr;
+    float *surr_bias = nullptr, *surr_norm = nullptr;
+    unsigned long long *sstat = nullptr;
+    // tuple harvest
+    float *hv = nullptr;
+    unsigned long long *hv_count = nullptr;
+    int64_t hv_cap = 0;
+    int hv_every = 0, hv_active = 0;
     // profiling
     int profile = 0;
     struct Timer { const char *name; hipEvent_t a, b; };
@@ -996,6 +1167,9 @@ Params params(mega_ctx *c)
     p.shsup = c->c.shsup; p.finite_strain = c->c.finite_strain; p.has_gravity = c->has_gravity;
     for (int k = 0; k < 3; ++k) p.grav[k] = c->c.gravity[k];
     p.dt = c->dt;
+    p.surr_tiles = c->surr_tiles; p.surr_bias = c->surr_bias; p.surr_norm = c->surr_norm;
+    p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
+    p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
     return p;
 }
 
@@ -1172,7 +1346,12 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
     }
     {
         Scope s(c, "k_node");
-        k_node<R, FM><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+        if constexpr (std::is_same<R, float>::value) {
+            if (c->surr_on) k_node<R, FM, true><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+            else k_node<R, FM, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+        } else {
+            k_node<R, FM, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+        }
     }
     if (force_mode >= 0) {
         Scope s(c, "k_force");
@@ -1298,7 +1477,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         return MEGA_ERR_ARG;
     if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3 || ctrl->itype_int != 0) return MEGA_ERR_ARG;
     if (ctrl->precision != MEGA_FP64 && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
-    if (ctrl->surrogate) return MEGA_ERR_ARG;
+    if (ctrl->surrogate && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     *out = nullptr;
     mega_ctx *c = new mega_ctx();
     c->c = *ctrl;
@@ -1365,6 +1544,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     xcd_grid(n, &c->nb_force);
     e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
     e |= dalloc(&c->flags, 4);
+    e |= dalloc(&c->sstat, 4);
     if (e) { mega_destroy(c); return MEGA_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { mega_destroy(c); return MEGA_ERR_HIP; }
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -1379,7 +1559,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
          zero(c->eps, n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
-         zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n);
+         zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n) &&
+         zero(c->sstat, 32);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
     if (ok && c->mixed) ok = zero(c->d4, (size_t)n * 16) && zero(c->t4, (size_t)n * 16);
     if (ok && c->wave_vel) ok = zero(c->wave_vel, (size_t)n * 8);
@@ -1397,7 +1578,7 @@ int mega_destroy(mega_ctx *c)
                     c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->Fst, c->ebc, c->ebc_node, c->mat,
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
-                    c->cub_tmp};
+                    c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -1607,6 +1788,8 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
         const bool first = (c->step == 0);
         const bool last = (s == nsteps - 1);
         int r = 0;
+        // tuples of this step's return maps (the first call has zero increments)
+        c->hv_active = (c->hv && c->hv_every > 0 && c->step > 0 && c->step % c->hv_every == 0) ? 1 : 0;
         if (first) {
             // LINIT call of HANDELER with zero increments (MAIN.F90:320-334)
             HIPCHK(hipMemsetAsync(c->d, 0, sizeof(double) * 3 * n, c->stream));
@@ -1637,6 +1820,7 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
         HIPCHK(hipGetLastError());
         c->step++;
     }
+    c->hv_active = 0;
     return check_flags(c);
 }
 
@@ -1740,7 +1924,12 @@ int mega_stats(mega_ctx *c, int64_t *nnz, int32_t *max_neighbors, int64_t *fallb
     for (int i = 0; i < n; ++i) mx = std::max(mx, rp[i + 1] - rp[i]);
     if (nnz) *nnz = rp[n];
     if (max_neighbors) *max_neighbors = mx;
-    if (fallbacks) *fallbacks = 0;
+    if (fallbacks) {
+        unsigned long long h[4];
+        HIPCHK(hipMemcpyAsync(h, c->sstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *fallbacks = (int64_t)h[3];
+    }
     return MEGA_OK;
 }
 
@@ -1776,6 +1965,88 @@ int mega_kernel_times(mega_ctx *c, const char **names, double *ms_avg, int32_t *
         ++k;
     }
     return k;
+}
+
+int mega_set_surrogate(mega_ctx *c, const float *params, const float *norm, const double *prop, double tol)
+{
+    if (!c) return MEGA_ERR_ARG;
+    if (!params) { c->surr_on = 0; return MEGA_OK; }
+    if (!norm || !prop || !c->mixed || c->hv || !(tol > 0.0)) return MEGA_ERR_ARG;
+    Material want = make_material(prop, 2);
+    int slot = -1;
+    for (size_t k = 0; k < c->host_mats.size(); ++k) {
+        const Material &m = c->host_mats[k];
+        if (m.type == 2 && m.nu == want.nu && m.E == want.E && m.rho == want.rho && m.sig0 == want.sig0 &&
+            m.B == want.B && m.CE == want.CE && m.lam == want.lam && m.mu == want.mu && m.l2m == want.l2m)
+            slot = (int)k;
+    }
+    if (slot < 0) return MEGA_ERR_ARG;
+    std::vector<uint16_t> tiles(mlp::kTilesFwd * 512);
+    for (int t = 0; t < mlp::kTilesFwd; ++t)
+        for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < 8; ++j) tiles[(t * 64 + l) * 8 + j] = mlp::bf16_bits(mlp::fwd_tile_value(params, t, l, j));
+    std::vector<float> bias(mlp::kBiasPack);
+    for (int k = 0; k < mlp::kBiasPack; ++k) bias[k] = mlp::bias_pack_value(params, k);
+    if (!c->surr_tiles) {
+        int e = dalloc(&c->surr_tiles, tiles.size()) | dalloc(&c->surr_bias, bias.size()) |
+                dalloc(&c->surr_norm, mlp::kNorm);
+        if (e) return MEGA_ERR_NOMEM;
+    }
+    HIPCHK(hipMemcpyAsync(c->surr_tiles, tiles.data(), tiles.size() * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->surr_bias, bias.data(), bias.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->surr_norm, norm, mlp::kNorm * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->surr_mat = slot;
+    c->surr_tol = (float)tol;
+    c->surr_on = 1;
+    return MEGA_OK;
+}
+
+int mega_surrogate_stats(mega_ctx *c, int64_t *stats, int32_t reset)
+{
+    if (!c || !stats) return MEGA_ERR_ARG;
+    unsigned long long h[4];
+    HIPCHK(hipMemcpyAsync(h, c->sstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(hipMemsetAsync(c->sstat, 0, sizeof(h), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) stats[k] = (int64_t)h[k];
+    return MEGA_OK;
+}
+
+int mega_harvest(mega_ctx *c, int64_t capacity, int32_t every)
+{
+    if (!c || capacity < 0 || (capacity > 0 && every < 1)) return MEGA_ERR_ARG;
+    if (capacity > 0 && c->surr_on) return MEGA_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->hv) (void)hipFree(c->hv);
+    if (c->hv_count) (void)hipFree(c->hv_count);
+    c->hv = nullptr;
+    c->hv_count = nullptr;
+    c->hv_cap = 0;
+    c->hv_every = 0;
+    if (capacity == 0) return MEGA_OK;
+    if (dalloc(&c->hv, 20 * (size_t)capacity) || dalloc(&c->hv_count, 1)) return MEGA_ERR_NOMEM;
+    HIPCHK(hipMemsetAsync(c->hv_count, 0, sizeof(unsigned long long), c->stream));
+    c->hv_cap = capacity;
+    c->hv_every = every;
+    return MEGA_OK;
+}
+
+int mega_harvest_take(mega_ctx *c, float *host, int64_t cap, int64_t *count)
+{
+    if (!c || !c->hv || !count) return MEGA_ERR_ARG;
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->hv_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *count = (int64_t)h;   // may exceed the capacity: only the first capacity tuples were kept
+    const int64_t m = std::min<int64_t>((int64_t)h, std::min<int64_t>(cap, c->hv_cap));
+    if (host && m > 0) {
+        HIPCHK(hipMemcpyAsync(host, c->hv, sizeof(float) * 20 * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(hipMemsetAsync(c->hv_count, 0, sizeof(h), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MEGA_OK;
 }
 
 }  // extern "C"

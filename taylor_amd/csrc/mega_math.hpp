@@ -223,6 +223,18 @@ MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
     return iters;
 }
 
+// trial yield value ||dev pred|| - sqrt(2/3) sigma_0 (1 + B eps)^CE, the test
+// that opens the return map (CONSTITUTE_VONMISES.F90:87-100)
+template <class T>
+MEGA_HD T trial_excess(const T *pred, T eps, const Material &m)
+{
+    const T kk = pred[0] + pred[1] + pred[2];
+    const T p3 = kk / T(3);
+    const T d0 = pred[0] - p3, d1 = pred[1] - p3, d2 = pred[2] - p3;
+    const T tn = sqrt(d0 * d0 + d1 * d1 + d2 * d2 + T(2) * (pred[3] * pred[3] + pred[4] * pred[4] + pred[5] * pred[5]));
+    return tn - T(sqrt(2.0 / 3.0)) * T(m.sig0) * hard_pow(T(1) + T(m.B) * eps, T(m.CE));
+}
+
 // elastic trial: pred = stress + C:de with props 28-30 (FINT.F90:789-794, FORM_CMAT)
 template <class T>
 MEGA_HD void elastic_trial(const T *stress, const T *de, const Material &m, T *pred)
@@ -245,6 +257,34 @@ template <class T>
 MEGA_HD T tnorm_strain(const T *v)
 {
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
+}
+
+// Residual of the radial-return equations that VON_MISES solves
+// (CONSTITUTE_VONMISES.F90:100-157) at a candidate answer (s, eps) for the
+// trial state (pred, eps_n), in units of sigma_y(eps):
+//   yield condition     sqrt(3/2) ||dev s|| - sigma_y(eps)
+//   flow direction      dev s - (||dev s|| / ||dev pred||) dev pred
+//   pressure            tr(s - pred) / 3
+//   plastic multiplier  ||dev pred|| - ||dev s|| - 2 mu sqrt(3/2) (eps - eps_n)
+// the largest magnitude of the four.  The exact answer has all four zero.
+template <class T>
+MEGA_HD T return_residual(const T *pred, const T *s, T eps_n, T eps, const Material &m)
+{
+    const T pp = (pred[0] + pred[1] + pred[2]) / T(3), ps = (s[0] + s[1] + s[2]) / T(3);
+    const T a[6] = {pred[0] - pp, pred[1] - pp, pred[2] - pp, pred[3], pred[4], pred[5]};
+    const T b[6] = {s[0] - ps, s[1] - ps, s[2] - ps, s[3], s[4], s[5]};
+    const T na = tnorm(a), nb = tnorm(b);
+    const T sy = T(m.sig0) * hard_pow(T(1) + T(m.B) * eps, T(m.CE));
+    const T r = nb / na;
+    T c[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = b[k] - r * a[k];
+    const T ryield = fabs(sqrt(T(1.5)) * nb - sy);
+    const T rdir = tnorm(c);
+    const T rpres = fabs(ps - pp);
+    const T rmult = fabs(na - nb - T(2) * T(m.mu) * sqrt(T(1.5)) * (eps - eps_n));
+    const T res = fmax(fmax(ryield, rdir), fmax(rpres, rmult)) / sy;
+    return res == res ? res : T(1e30);   // NaN never passes
 }
 
 // GMAX_WVEL from secant moduli (FINT.F90:850-870, ESTIMATE_MODULI.F90:16-86)

@@ -10,6 +10,10 @@ The methods follow the reference's stages and argument meanings:
     Mega.step            MAIN time loop                MAIN.F90:304-658
     constitute           HUGHES_WINGET/ROTATE/VON_MISES per point
                                                        FINT.F90:758-801
+    Mlp                  trainer/inference of the learned J2 return map
+                         (no reference counterpart; stands in for VON_MISES,
+                         CONSTITUTE_VONMISES.F90:21-158)
+    Mega.set_surrogate / surrogate_stats / harvest / harvest_take
 
 Errors come back as MegaError carrying the library status (the reference
 stops the program instead).  The library must be built (taylor_amd/build.py);
@@ -26,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmega_hip.so")
 
 MEGA_FP64, MEGA_MIXED = 0, 1
+MLP_PARAMS, MLP_NORM = 9671, 59
 FIELDS = {"x": 0, "v": 1, "a": 2, "d": 3, "dtot": 4, "uphy": 5, "fint": 6, "fext": 7, "stress": 8,
           "strain": 9, "eps": 10, "strain_eq": 11, "wave_vel": 12, "vphy": 13}
 WIDTH = {"stress": 6, "strain": 6, "eps": 1, "strain_eq": 1, "wave_vel": 1}
@@ -73,6 +78,19 @@ def load_library():
         "mega_stats": [vp, C.POINTER(i64), ip, C.POINTER(i64)],
         "mega_profile": [vp, i32],
         "mega_kernel_times": [vp, C.POINTER(C.c_char_p), dp, ip, i32],
+        "mega_mlp_create": [C.c_uint64, C.POINTER(vp)],
+        "mega_mlp_destroy": [vp],
+        "mega_mlp_set_params": [vp, vp],
+        "mega_mlp_get_params": [vp, vp],
+        "mega_mlp_get_norm": [vp, vp],
+        "mega_mlp_set_data": [vp, vp, i64, vp],
+        "mega_mlp_train": [vp, i32, i32, C.c_double, C.c_double, vp],
+        "mega_mlp_forward": [vp, vp, vp, i64],
+        "mega_mlp_gradient": [vp, vp, vp, i64, vp, C.POINTER(C.c_float)],
+        "mega_set_surrogate": [vp, vp, vp, vp, C.c_double],
+        "mega_surrogate_stats": [vp, C.POINTER(i64), i32],
+        "mega_harvest": [vp, i64, i32],
+        "mega_harvest_take": [vp, vp, i64, C.POINTER(i64)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -224,6 +242,33 @@ class Mega:
     def profile(self, on: bool = True):
         _chk(self.lib.mega_profile(self.ctx, int(on)), "mega_profile")
 
+    def set_surrogate(self, params, norm, props, tol: float = 1e-3):
+        """Enable the learned return map for J2 nodes of property row props[30]
+        (params None disables)."""
+        if params is None:
+            _chk(self.lib.mega_set_surrogate(self.ctx, None, None, None, 0.0), "mega_set_surrogate")
+            return
+        self._surr = (np.ascontiguousarray(params, np.float32), np.ascontiguousarray(norm, np.float32),
+                      _f64(props))
+        _chk(self.lib.mega_set_surrogate(self.ctx, _p(self._surr[0]), _p(self._surr[1]), _p(self._surr[2]),
+                                         float(tol)), "mega_set_surrogate")
+
+    def surrogate_stats(self, reset: bool = False) -> dict:
+        s = (C.c_int64 * 4)()
+        _chk(self.lib.mega_surrogate_stats(self.ctx, s, int(reset)), "mega_surrogate_stats")
+        return {"plastic": s[0], "outside_envelope": s[1], "evaluated": s[2], "rejected": s[3]}
+
+    def harvest(self, capacity: int, every: int = 1):
+        _chk(self.lib.mega_harvest(self.ctx, int(capacity), int(every)), "mega_harvest")
+
+    def harvest_take(self, cap: int) -> tuple[np.ndarray, int]:
+        """(tuples[m, 20] float32, number recorded since the last take)"""
+        out = np.zeros((max(int(cap), 1), 20), np.float32)
+        cnt = C.c_int64()
+        _chk(self.lib.mega_harvest_take(self.ctx, _p(out), int(cap), C.byref(cnt)), "mega_harvest_take")
+        m = min(cnt.value, int(cap))
+        return out[:m].copy(), cnt.value
+
     def kernel_times(self) -> dict:
         cap = 32
         names = (C.c_char_p * cap)()
@@ -244,3 +289,67 @@ def constitute(lmat, stress, strain, state, props, mat_type: int = 2, finite_str
     st = lib.mega_constitute(n, _p(_f64(lmat)), _p(stress), _p(strain), _p(state), _p(_f64(props)),
                              int(mat_type), int(finite_strain), int(precision))
     return _chk(st, "mega_constitute", allow=(1,))
+
+
+class Mlp:
+    """Trainer / batch inference of the 13-64-64-64-7 network on the GPU."""
+
+    def __init__(self, seed: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _chk(self.lib.mega_mlp_create(C.c_uint64(seed), C.byref(h)), "mega_mlp_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mega_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _chk(self.lib.mega_mlp_get_params(self.h, _p(out)), "mega_mlp_get_params")
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        _chk(self.lib.mega_mlp_set_params(self.h, _p(p)), "mega_mlp_set_params")
+
+    @property
+    def norm(self) -> np.ndarray:
+        out = np.zeros(MLP_NORM, np.float32)
+        _chk(self.lib.mega_mlp_get_norm(self.h, _p(out)), "mega_mlp_get_norm")
+        return out
+
+    def set_data(self, tuples, props):
+        t = np.ascontiguousarray(tuples, np.float32).reshape(-1, 20)
+        _chk(self.lib.mega_mlp_set_data(self.h, _p(t), t.shape[0], _p(_f64(props))), "mega_mlp_set_data")
+
+    def train(self, iters: int, batch: int = 8192, lr0: float = 2e-3, lr1: float = 2e-5) -> np.ndarray:
+        hist = np.zeros(max(iters, 1), np.float32)
+        _chk(self.lib.mega_mlp_train(self.h, int(iters), int(batch), float(lr0), float(lr1), _p(hist)),
+             "mega_mlp_train")
+        return hist[:iters]
+
+    def forward(self, xhat) -> np.ndarray:
+        x = np.ascontiguousarray(xhat, np.float32).reshape(-1, 13)
+        y = np.zeros((x.shape[0], 7), np.float32)
+        _chk(self.lib.mega_mlp_forward(self.h, _p(x), _p(y), x.shape[0]), "mega_mlp_forward")
+        return y
+
+    def gradient(self, xhat, that):
+        x = np.ascontiguousarray(xhat, np.float32).reshape(-1, 13)
+        t = np.ascontiguousarray(that, np.float32).reshape(-1, 7)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _chk(self.lib.mega_mlp_gradient(self.h, _p(x), _p(t), x.shape[0], _p(g), C.byref(loss)),
+             "mega_mlp_gradient")
+        return g, loss.value

@@ -191,7 +191,34 @@ def mega_run(prob, mesh, nsteps, frames, workdir):
     return out
 
 
+def final_state_vectors(workdir):
+    """The CPU-runnable Taylor bar (spacing 0.5 mm, seed 0, 190 m/s, 80 us,
+    Lagrangian FP64) run to the end by the reference program with one thread;
+    the last frame's positions, displacement and plastic strain."""
+    mesh = deck.taylor_bar_mesh(0.5e-3, seed=0)
+    prob = deck.taylor_problem(0.5e-3, seed=0)
+    nsteps = int(np.floor(80e-6 / prob.dt)) + 1
+    kfile = os.path.join(workdir, "bar.k")
+    deck.write_k(kfile, mesh)
+    prob.time_end = prob.dt * (nsteps - 0.5)
+    deck.write_control(os.path.join(workdir, "Control.dat"), kfile, prob, ["BAR", "WALL"],
+                       time_output=prob.dt * (nsteps - 1.5),
+                       out_vars="unformatted\ndisplacement\neps")
+    with open(os.path.join(workdir, "mega.out"), "w") as fo:
+        subprocess.run([REF_MEGA, "-j", "1"], cwd=workdir, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
+                       check=True, timeout=7200)
+    last = sorted(glob.glob(os.path.join(workdir, "MEGAOUT_*.vtk")))[-1]
+    fr = read_unformatted_vtk(last)
+    return {"nsteps": np.array([nsteps]), "frame": np.array([int(last[-8:-4])]), "x": fr["points"],
+            "u": np.stack([fr["displacement_x"], fr["displacement_y"], fr["displacement_z"]], 1),
+            "eps": fr["eps"]}
+
+
 def main():
+    if "--final" in sys.argv:
+        with tempfile.TemporaryDirectory() as wd:
+            np.savez_compressed(os.path.join(HERE, "taylor_h05_final.npz"), **final_state_vectors(wd))
+        return
     ref = load_ref()
     rng = np.random.default_rng(20260101)
     np.savez_compressed(os.path.join(HERE, "dense.npz"), **dense_vectors(ref, rng))
