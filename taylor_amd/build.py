@@ -2,6 +2,8 @@
 
 * taylor_amd/libmega_hip.so  -- the product: HIP kernels for gfx950 + C-ABI
   (include/mega.h), compiled with hipcc.
+* taylor_amd/host/mega_drive -- the Fortran host (bind(C) interfaces and the
+  time loop), compiled with flang against libmega_hip.so.
 * oracle/liboracle.so       -- the CPU checker (test infrastructure).
 * oracle/_ref/              -- the reference program compiled from its own
   sources, only where those sources exist (this build container).
@@ -47,6 +49,29 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host(verbose: bool = False) -> str | None:
+    """The Fortran host (taylor_amd/host/mega_drive): the bind(C) interface
+    module and the time-loop driver, linked against libmega_hip.so."""
+    fc = shutil.which("flang") or ("/opt/rocm/lib/llvm/bin/flang" if os.path.exists("/opt/rocm/lib/llvm/bin/flang") else None)
+    if not fc:
+        return None
+    import tempfile
+    host = os.path.join(HERE, "host")
+    out = os.path.join(host, "mega_drive")
+    with tempfile.TemporaryDirectory() as td:
+        cmds = [[fc, "-O2", "-c", "-module-dir", td, os.path.join(host, "mega_iface.f90"), "-o",
+                 os.path.join(td, "mega_iface.o")],
+                [fc, "-O2", "-module-dir", td, "-I", td, os.path.join(host, "mega_drive.f90"),
+                 os.path.join(td, "mega_iface.o"), "-L", HERE, "-lmega_hip", "-Wl,-rpath,$ORIGIN/..",
+                 "-o", out + ".tmp"]]
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(verbose: bool = False) -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
     if os.path.isdir(REF_SRC) and shutil.which("flang"):
@@ -55,6 +80,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def build_all(verbose: bool = False) -> None:
     build_hip(verbose)
+    build_host(verbose)
     build_oracle(verbose)
 
 

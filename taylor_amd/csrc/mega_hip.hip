@@ -881,30 +881,57 @@ __global__ void k_tcount(int64_t nnz, const int32_t *col, int32_t *cnt)
     if (q < nnz) atomicAdd(&cnt[col[q]], 1);
 }
 
-__global__ void k_tfill(int n, const int32_t *row_ptr, const int32_t *col, int32_t *cursor, int32_t *tsrc, int32_t *tq)
+// kGroup lanes per list row append the row's entries to the transposed rows
+__global__ __launch_bounds__(kBlock) void k_tfill(int n, const int32_t *row_ptr, const int32_t *col, int32_t *cursor,
+                                                  int32_t *tsrc, int32_t *tq)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kGroup), sub = (int)(t % kGroup);
     if (i >= n) return;
-    for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
-        const int t = atomicAdd(&cursor[col[q]], 1);
-        tsrc[t] = i;
-        tq[t] = q;
+    for (int q = row_ptr[i] + sub; q < row_ptr[i + 1]; q += kGroup) {
+        const int k = atomicAdd(&cursor[col[q]], 1);
+        tsrc[k] = i;
+        tq[k] = q;
     }
 }
 
-__global__ void k_tsort(int n, const int32_t *trow, int32_t *tsrc, int32_t *tq, int32_t *tpos)
+// Order each transposed row by source node (the order of the reference's own
+// scatter over i, FINT.F90:997-1116): one wave per row ranks its entries in
+// LDS; sources within a row are distinct, so an entry's rank is its slot.
+constexpr int kTsortMax = 256;
+
+__global__ __launch_bounds__(kBlock) void k_tsort(int n, const int32_t *trow, int32_t *tsrc, int32_t *tq,
+                                                  int32_t *tpos)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int s = trow[j], e = trow[j + 1];
-    for (int a = s + 1; a < e; ++a) {
-        const int vs = tsrc[a], vq = tq[a];
-        int b = a - 1;
-        while (b >= s && tsrc[b] > vs) { tsrc[b + 1] = tsrc[b]; tq[b + 1] = tq[b]; --b; }
-        tsrc[b + 1] = vs;
-        tq[b + 1] = vq;
+    __shared__ int2 buf[kBlock / 64][kTsortMax];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (j >= n) return;   // whole waves leave together
+    const int s = trow[j], L = trow[j + 1] - s;
+    if (L > kTsortMax) {  // long row: one lane, insertion sort in place
+        if (lane == 0) {
+            for (int a = s + 1; a < s + L; ++a) {
+                const int vs = tsrc[a], vq = tq[a];
+                int b = a - 1;
+                while (b >= s && tsrc[b] > vs) { tsrc[b + 1] = tsrc[b]; tq[b + 1] = tq[b]; --b; }
+                tsrc[b + 1] = vs;
+                tq[b + 1] = vq;
+            }
+            for (int a = s; a < s + L; ++a) tpos[tq[a]] = a;
+        }
+        return;
     }
-    for (int a = s; a < e; ++a) tpos[tq[a]] = a;
+    int2 *row = buf[w];
+    for (int a = lane; a < L; a += 64) row[a] = make_int2(tsrc[s + a], tq[s + a]);
+    wave_sync();
+    for (int a = lane; a < L; a += 64) {
+        const int2 v = row[a];
+        int r = 0;
+        for (int b = 0; b < L; ++b) r += row[b].x < v.x;
+        tsrc[s + r] = v.x;
+        tq[s + r] = v.y;
+        tpos[v.y] = s + r;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1270,8 +1297,8 @@ int do_transpose(mega_ctx *c)
     int r = cub_scan(c, c->counts, c->trow, n + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(c->counts, c->trow, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
-    k_tfill<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->row_ptr, c->col, c->counts, c->tsrc, c->tq);
-    k_tsort<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->trow, c->tsrc, c->tq, c->tpos);
+    k_tfill<<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(n, c->row_ptr, c->col, c->counts, c->tsrc, c->tq);
+    k_tsort<<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(n, c->trow, c->tsrc, c->tq, c->tpos);
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }

@@ -1,0 +1,52 @@
+! mega_drive.f90 -- Fortran host of the explicit step on the device: the
+! time loop of MAIN (MAIN.F90:304-658) with HANDELER / CONSTRUCT_FINT /
+! CONSTITUTION replaced by the C-ABI (mega_iface).  It reads the model
+! arrays MAIN would hold after init (GCOO, GWIN, GVOL, LOCAL_MASS, LOCAL_EBC,
+! GEBC_NODES, GMAT_TYPE, GPROP, LOCAL_VEL; MAIN.F90:130-302) from a binary
+! stream file, advances nsteps steps in chunks, and writes the final
+! positions and plastic strain.
+!
+!   mega_drive <model.bin> <result.bin>
+!
+! model.bin (stream, native endianness): int32 n, nsteps, chunk; real64 dt;
+! mega_control; real64 X0(3n), win(3n), vol(n), mass(3n); int32 ebc(3n),
+! ebc_node(n), mat_type(n); real64 prop(30n), vinit(3n).
+program mega_drive
+  use iso_c_binding
+  use mega_iface
+  implicit none
+  type(mega_control) :: ctrl
+  type(c_ptr) :: ctx
+  integer(c_int32_t) :: n, nsteps, chunk, k
+  real(c_double) :: dt, e_int, e_kin, e_ext
+  real(c_double), allocatable :: x0(:), win(:), vol(:), mass(:), prop(:), vinit(:), x(:), eps(:)
+  integer(c_int32_t), allocatable :: ebc(:), ebc_node(:), mat_type(:)
+  character(len=4096) :: fin, fout
+  integer :: done
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) n, nsteps, chunk, dt
+  read(10) ctrl
+  allocate(x0(3*n), win(3*n), vol(n), mass(3*n), ebc(3*n), ebc_node(n), mat_type(n), prop(30*n), vinit(3*n))
+  read(10) x0, win, vol, mass, ebc, ebc_node, mat_type, prop, vinit
+  close(10)
+
+  call mega_check(mega_create(ctrl, n, x0, win, vol, mass, ebc, ebc_node, mat_type, prop, vinit, ctx), 'mega_create')
+  done = 0
+  do while (done < nsteps)
+     k = min(chunk, nsteps - done)
+     call mega_check(mega_step(ctx, dt, k), 'mega_step')
+     done = done + k
+     call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
+     write(*, '(A,I8,A,3ES14.6)') 'step ', done, '  energies (int, kin, ext)', e_int, e_kin, e_ext
+  end do
+  allocate(x(3*n), eps(n))
+  call mega_check(mega_download(ctx, MEGA_F_X, x), 'mega_download')
+  call mega_check(mega_download(ctx, MEGA_F_EPS, eps), 'mega_download')
+  call mega_check(mega_destroy(ctx), 'mega_destroy')
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(11) x, eps
+  close(11)
+end program mega_drive

@@ -353,3 +353,19 @@ class Mlp:
         _chk(self.lib.mega_mlp_gradient(self.h, _p(x), _p(t), x.shape[0], _p(g), C.byref(loss)),
              "mega_mlp_gradient")
         return g, loss.value
+
+
+def write_model(path: str, prob, nsteps: int, chunk: int = 500, precision: int = MEGA_FP64) -> None:
+    """Binary model file of the Fortran host (taylor_amd/host/mega_drive.f90):
+    int32 n, nsteps, chunk; real64 dt; the mega_control struct; then the arrays
+    mega_create takes, in its argument order."""
+    import struct
+    ctrl = make_control(prob, precision, 0)
+    props = prob.props_table()[prob.mat_id]
+    mtype = prob.mat_types()[prob.mat_id]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iiid", prob.n, int(nsteps), int(chunk), float(prob.dt)))
+        f.write(bytes(ctrl))
+        for a in (_f64(prob.coo), _f64(prob.win), _f64(prob.vol), _f64(prob.mass), _i32(prob.ebc),
+                  _i32(prob.ebc_node), _i32(mtype), _f64(props), _f64(prob.vinit)):
+            f.write(a.tobytes())
