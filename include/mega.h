@@ -59,6 +59,8 @@ typedef struct mega_control {
     double  dist_max[3];     /* XDIST_MAX, YDIST_MAX, ZDIST_MAX (bin size)     */
     int32_t precision;       /* MEGA_FP64 or MEGA_MIXED                        */
     int32_t surrogate;       /* 1: learned return map with exact fallback      */
+    int32_t peridynamics;    /* PERIDYNAMICS: CONSTRUCT_FINT_PD (Lagrangian)   */
+    int32_t reserved;        /* 0                                              */
 } mega_control;
 
 typedef struct mega_ctx mega_ctx;
@@ -91,10 +93,18 @@ int mega_shapes(mega_ctx *ctx, const double *x_eval);
 /* Export stored shapes: shp[nnz], dshp[3 nnz], in neighbor-list order. */
 int mega_get_shapes(mega_ctx *ctx, double *shp, double *dshp, int64_t cap);
 
-/* DO_INTERP: phys[3n] = sum_j phi_ij coef[3j+k].  HANDELER.F90:114-133. */
+/* With peridynamics on (FINT_PD.F90:14-759): UDFM_SHAPE_TENSOR results,
+ * influence weights omega[nnz] in neighbor-list order (GSTACK_SHP) and the
+ * inverse shape tensors invk[9n] (GINVK, column-major).  RK.f90:579-726. */
+int mega_get_pd_shape(mega_ctx *ctx, double *omega, double *invk, int64_t cap);
+
+/* DO_INTERP: phys[3n] = sum_j phi_ij coef[3j+k].  HANDELER.F90:114-133.
+ * With peridynamics, phys = coef (HANDELER.F90:134-137). */
 int mega_interp(mega_ctx *ctx, const double *coef, double *phys);
 
-/* One CONSTRUCT_FINT call on host arrays (FINT.F90:16-1247), constitution
+/* One CONSTRUCT_FINT call on host arrays (FINT.F90:16-1247), or with
+ * peridynamics one CONSTRUCT_FINT_PD call (FINT_PD.F90:14-759; fext is then
+ * zero and int_work unchanged, as the reference passes neither), constitution
  * included: inputs dinc[3n] (GDINC), dinc_tot[3n] (GDINC_TOT), x_cur[3n]
  * (GCOO_CUURENT); in/out stress[6n], strain[6n], state[20n]; out fint[3n],
  * fext[3n]; int_work accumulates (GINT_WORK).  linit=1 computes and stores

@@ -699,6 +699,175 @@ int or_construct_fint(or_fint_io *io, const or_control *c)
 }
 
 /* ---------------------------------------------------------------------- */
+/* Correspondence peridynamics.                                           */
+
+/* UDFM_SHAPE_TENSOR (RK.f90:579-726): w_J = prod_k w(|X_J,k - x_k| / a_J,k),
+ * K = sum_J xi xi^T w_J V_J with xi = X_J - x, K^-1 by INVERSE.  The
+ * singular-kernel branch there is switched off (IF (.FALSE.)). */
+void or_udfm(const double *x, int n, const double *coo, const double *vol, const double *win,
+             const int32_t *lstack, int ln, double *shp, double *invk)
+{
+    (void)n;
+    double K[9] = {0};
+    for (int I = 0; I < ln; ++I) {
+        const int ii = lstack[I];
+        const double h[3] = {-(x[0] - coo[3 * ii]), -(x[1] - coo[3 * ii + 1]), -(x[2] - coo[3 * ii + 2])};
+        double px, py, pz, d;
+        or_kernel(fabs(-(x[0] - coo[3 * ii]) / win[3 * ii]), &px, &d);
+        or_kernel(fabs(-(x[1] - coo[3 * ii + 1]) / win[3 * ii + 1]), &py, &d);
+        or_kernel(fabs(-(x[2] - coo[3 * ii + 2]) / win[3 * ii + 2]), &pz, &d);
+        const double phi = px * py * pz;
+        for (int J = 0; J < 3; ++J)
+            for (int L = 0; L < 3; ++L) CM3(K, J, L) = CM3(K, J, L) + h[J] * h[L] * phi * vol[ii];
+        shp[I] = phi;
+    }
+    or_inverse(K, 3, invk);
+}
+
+/* DFM_SHAPE_TENSOR (RK.f90:730-807) */
+void or_dfm(const double *x0, const double *xt, const double *coo, const double *vol, const int32_t *lstack,
+            int ln, const double *ycur, const double *shp, double *S)
+{
+    for (int k = 0; k < 9; ++k) S[k] = 0.0;
+    for (int I = 0; I < ln; ++I) {
+        const int ii = lstack[I];
+        const double h[3] = {coo[3 * ii] - x0[0], coo[3 * ii + 1] - x0[1], coo[3 * ii + 2] - x0[2]};
+        const double hs[3] = {ycur[3 * I] - xt[0], ycur[3 * I + 1] - xt[1], ycur[3 * I + 2] - xt[2]};
+        for (int J = 0; J < 3; ++J)
+            for (int L = 0; L < 3; ++L) CM3(S, J, L) = CM3(S, J, L) + hs[J] * h[L] * shp[I] * vol[ii];
+    }
+}
+
+/* CONSTRUCT_FINT_PD (FINT_PD.F90:14-759), Lagrangian.  At LINIT the influence
+ * weights and K^-1 are formed and stored (:227-290, :418-432), F = I;
+ * afterwards F = S K^-1 from the current positions (:383-403) with |det F|
+ * and the pivoting INVERSE.  L = (sum w V dd (x) xi) K^-1 F^-1 (:446-470);
+ * Hughes-Winget, rotation, trial and CONSTITUTION as in CONSTRUCT_FINT;
+ * bond force t = J sigma F^-T K^-1 (X_i - X_j) added to i with V_j w V_i and
+ * taken from j with V_i w V_j (:619-655).  The wave-speed estimate keeps the
+ * reference's STRAIN_INC = LSTRAIN - GSTRESS (:509). */
+int or_construct_fint_pd(or_fint_io *io, const or_control *c)
+{
+    const int n = io->n;
+    const int nth = io->nthreads > 0 ? io->nthreads : 1;
+    if (!c->lagrangian) return -3;   /* semi-Lagrangian PD is "not implemented" in the reference */
+    for (int64_t i = 0; i < 3 * (int64_t)n; ++i) io->fint[i] = 0.0;
+    int64_t maxln = 0;
+    for (int i = 0; i < n; ++i) {
+        int64_t l = io->row_ptr[i + 1] - io->row_ptr[i];
+        if (l > maxln) maxln = l;
+    }
+    if (io->linit) {
+        int32_t *lst = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxln + 1));
+        for (int i = 0; i < n; ++i) {
+            const int64_t p0 = io->row_ptr[i];
+            const int ln = (int)(io->row_ptr[i + 1] - p0);
+            for (int J = 0; J < ln; ++J) lst[J] = io->col[p0 + J];
+            or_udfm(io->coo0 + 3 * i, n, io->coo0, io->vol, io->win, lst, ln, io->shp + p0, io->invk + 9 * (int64_t)i);
+        }
+        free(lst);
+    }
+    int64_t fails = 0;
+    int bad_type = 0;
+#pragma omp parallel num_threads(nth) reduction(+ : fails, bad_type)
+    {
+        double *ycur = (double *)malloc(sizeof(double) * 3 * (size_t)(maxln + 1));
+        int32_t *lst = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxln + 1));
+        const int atomic = nth > 1;
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            const int64_t p0 = io->row_ptr[i];
+            const int ln = (int)(io->row_ptr[i + 1] - p0);
+            const int mid = io->mat_id[i];
+            const double *lprop = io->props + 30 * mid;
+            const int mtype = io->mat_type[mid];
+            const double *shp = io->shp + p0;
+            const double *invk = io->invk + 9 * (int64_t)i;
+            double lstress[6], lstrain[6], lstate[20];
+            for (int k = 0; k < 6; ++k) { lstress[k] = io->stress[6 * i + k]; lstrain[k] = io->strain[6 * i + k]; }
+            for (int k = 0; k < 20; ++k) lstate[k] = io->state[20 * i + k];
+            for (int J = 0; J < ln; ++J) lst[J] = io->col[p0 + J];
+            double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, det = 1.0;
+            if (!io->linit && c->finite_strain) {
+                double S[9];
+                for (int J = 0; J < ln; ++J)
+                    for (int k = 0; k < 3; ++k) ycur[3 * J + k] = io->coo_cur[3 * lst[J] + k];
+                or_dfm(io->coo0 + 3 * i, io->coo_cur + 3 * i, io->coo0, io->vol, lst, ln, ycur, shp, S);
+                mm3(S, invk, F);
+                det = or_determinant(F);
+                or_inverse(F, 3, IF);
+            }
+            double L0[9], L1[9], lmat[9];
+            for (int J = 0; J < ln; ++J)
+                for (int k = 0; k < 3; ++k) ycur[3 * J + k] = io->dinc[3 * lst[J] + k];
+            or_dfm(io->coo0 + 3 * i, io->dinc + 3 * i, io->coo0, io->vol, lst, ln, ycur, shp, L0);
+            mm3(L0, invk, L1);
+            if (!io->linit) mm3(L1, IF, lmat);
+            else memcpy(lmat, L1, sizeof(L1));
+            double old_s[6];
+            for (int k = 0; k < 6; ++k) old_s[k] = io->stress[6 * i + k];
+            int it = or_constitute_point(lmat, lstress, lstrain, lstate, lprop, mtype, 1, NULL);
+            if (it == -1) fails++;
+            if (it == -2) bad_type++;
+            for (int k = 0; k < 20; ++k) io->state[20 * i + k] = lstate[k];
+            double si[6], ei[6];
+            for (int k = 0; k < 6; ++k) {
+                si[k] = lstress[k] - old_s[k];
+                ei[k] = lstrain[k] - old_s[k];
+                io->stress[6 * i + k] = lstress[k];
+                io->strain[6 * i + k] = lstrain[k];
+            }
+            if (io->wave_vel) {
+                double poiss = lprop[0], young = lprop[1], rho = lprop[2];
+                double bulk = young / (3.0 * (1.0 - 2.0 * poiss));
+                double shear = young / (2.0 * (1.0 + poiss));
+                if (mtype > 1) or_estimate_moduli(si, ei, &shear, &bulk);
+                double pmod = bulk + 4.0 * shear / 3.0;
+                io->wave_vel[i] = sqrt(fmax(pmod, shear) / rho);
+                if (mtype == 3) io->wave_vel[i] = io->wave_vel[i] / 0.15;
+            }
+            /* P = det * sigma IF^T, T = P K^-1 */
+            const double sg[9] = {lstress[0], lstress[5], lstress[4], lstress[5], lstress[1], lstress[3],
+                                  lstress[4], lstress[3], lstress[2]};
+            double IFt[9], PK[9], T[9];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) CM3(IFt, a, b) = CM3(IF, b, a);
+            mm3(sg, IFt, PK);
+            for (int k = 0; k < 9; ++k) PK[k] = det * PK[k];
+            mm3(PK, invk, T);
+            const double vi = io->vol[i];
+            for (int J = 0; J < ln; ++J) {
+                const int jj = lst[J];
+                const double dx[3] = {io->coo0[3 * i] - io->coo0[3 * jj], io->coo0[3 * i + 1] - io->coo0[3 * jj + 1],
+                                      io->coo0[3 * i + 2] - io->coo0[3 * jj + 2]};
+                double f3[3];
+                for (int a = 0; a < 3; ++a) {
+                    double s = 0.0;
+                    for (int b = 0; b < 3; ++b) s += CM3(T, a, b) * dx[b];
+                    f3[a] = s;
+                }
+                const double vj = io->vol[jj];
+                for (int k = 0; k < 3; ++k) {
+                    const double tj = f3[k] * vi * shp[J] * vj, ti = f3[k] * vj * shp[J] * vi;
+                    if (atomic) {
+#pragma omp atomic
+                        io->fint[3 * jj + k] -= tj;
+#pragma omp atomic
+                        io->fint[3 * i + k] += ti;
+                    } else {
+                        io->fint[3 * jj + k] = io->fint[3 * jj + k] - tj;
+                        io->fint[3 * i + k] = io->fint[3 * i + k] + ti;
+                    }
+                }
+            }
+        }
+        free(ycur); free(lst);
+    }
+    io->ret_fail = fails;
+    return bad_type ? -2 : (fails ? -1 : 0);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Whole run: MAIN.F90:304-658 with the HANDELER calls it makes.          */
 
 static int model_search(or_model *m, int first)
@@ -733,8 +902,8 @@ static int model_fint(or_model *m, int linit)
     io.row_ptr = m->row_ptr; io.col = m->col; io.shp = m->shp; io.dshp = m->dshp;
     io.stress = m->stress; io.strain = m->strain; io.state = m->state;
     io.fint = m->fint; io.fext = m->fext; io.wave_vel = m->wave_vel; io.strain_eq = m->strain_eq;
-    io.int_work = m->int_work; io.linit = linit; io.nthreads = m->nthreads;
-    int r = or_construct_fint(&io, &m->c);
+    io.int_work = m->int_work; io.linit = linit; io.nthreads = m->nthreads; io.invk = m->invk;
+    int r = m->c.peridynamics ? or_construct_fint_pd(&io, &m->c) : or_construct_fint(&io, &m->c);
     m->int_work = io.int_work;
     m->ret_fail += io.ret_fail;
     return r;
@@ -746,6 +915,8 @@ int or_model_init(or_model *m)
     m->row_ptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
     m->ijk = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)n);
     m->col = NULL; m->shp = NULL; m->dshp = NULL; m->nnz = 0; m->nnz_cap = 0;
+    m->invk = m->c.peridynamics ? (double *)calloc(9 * (size_t)n, sizeof(double)) : NULL;
+    if (m->c.peridynamics && !m->invk) return -1;
     m->step = 0; m->time = 0.0; m->int_work = 0.0; m->ke = m->ie = m->ee = 0.0;
     m->search_count = 0; m->ret_fail = 0;
     if (!m->row_ptr || !m->ijk) return -1;
@@ -781,9 +952,17 @@ int or_model_step(or_model *m)
         m->v[q] = m->v[q] + dt * 0.5 * m->a[q];
     }
     /* DO_INTERP (HANDELER.F90:114-133) and coordinate update (MAIN.F90:377-386) */
-    or_interp(n, m->row_ptr, m->col, m->shp, m->d, m->dphy);
-    if (m->vphy) or_interp(n, m->row_ptr, m->col, m->shp, m->v, m->vphy);
-    if (m->aphy) or_interp(n, m->row_ptr, m->col, m->shp, m->a, m->aphy);
+    if (m->c.peridynamics) {   /* HANDELER.F90:134-137: physical = generalized */
+        for (int64_t q = 0; q < 3 * (int64_t)n; ++q) {
+            m->dphy[q] = m->d[q];
+            if (m->vphy) m->vphy[q] = m->v[q];
+            if (m->aphy) m->aphy[q] = m->a[q];
+        }
+    } else {
+        or_interp(n, m->row_ptr, m->col, m->shp, m->d, m->dphy);
+        if (m->vphy) or_interp(n, m->row_ptr, m->col, m->shp, m->v, m->vphy);
+        if (m->aphy) or_interp(n, m->row_ptr, m->col, m->shp, m->a, m->aphy);
+    }
     for (int64_t q = 0; q < 3 * (int64_t)n; ++q) {
         m->dphy_tot[q] = m->dphy_tot[q] + m->dphy[q];
         m->x[q] = m->coo0[q] + m->dphy_tot[q];
@@ -825,6 +1004,6 @@ int or_model_step(or_model *m)
 
 void or_model_free(or_model *m)
 {
-    free(m->row_ptr); free(m->ijk); free(m->col); free(m->shp); free(m->dshp);
-    m->row_ptr = NULL; m->ijk = NULL; m->col = NULL; m->shp = NULL; m->dshp = NULL;
+    free(m->row_ptr); free(m->ijk); free(m->col); free(m->shp); free(m->dshp); free(m->invk);
+    m->row_ptr = NULL; m->ijk = NULL; m->col = NULL; m->shp = NULL; m->dshp = NULL; m->invk = NULL;
 }

@@ -30,6 +30,7 @@ typedef struct or_control {
     int lagrangian;     /* LLAGRANGIAN                           */
     int finite_strain;  /* LFINITE_STRAIN                        */
     double gravity[3];  /* IGRAVITY                              */
+    int peridynamics;   /* PERIDYNAMICS: CONSTRUCT_FINT_PD        */
 } or_control;
 
 /* ---- small dense math (INVERSE.F90, DETERMINANT.F90, KERNEL.F90) ---- */
@@ -78,12 +79,23 @@ typedef struct or_fint_io {
     double *stress, *strain, *state;             /* 6n, 6n, 20n (in/out)                */
     double *fint, *fext;                         /* 3n out                              */
     double *wave_vel, *strain_eq;                /* n out (may be NULL)                 */
+    double *invk;                                /* 9n inverse PD shape tensors (in/out) */
     double int_work;                             /* accumulated (in/out)                */
     int linit;
     int nthreads;                                /* 1 = reference summation order       */
     int64_t ret_fail;                            /* count of non-converged return maps  */
 } or_fint_io;
 int or_construct_fint(or_fint_io *io, const or_control *c);
+
+/* ---- correspondence peridynamics (FINT_PD.F90, RK.f90:579-807) ---- */
+/* UDFM_SHAPE_TENSOR: influence weights shp[ln] and K^-1 (3x3) at x */
+void or_udfm(const double *x, int n, const double *coo, const double *vol, const double *win,
+             const int32_t *lstack, int ln, double *shp, double *invk);
+/* DFM_SHAPE_TENSOR: S = sum w V (y_J - x_t) (x) (X_J - x_0) with y given per neighbor */
+void or_dfm(const double *x0, const double *xt, const double *coo, const double *vol, const int32_t *lstack,
+            int ln, const double *ycur, const double *shp, double *S);
+/* CONSTRUCT_FINT_PD: fint only (the reference passes no FEXT and no work) */
+int or_construct_fint_pd(or_fint_io *io, const or_control *c);
 
 /* ---- interpolation (HANDELER.F90:114-139) ---- */
 void or_interp(int n, const int64_t *row_ptr, const int32_t *col, const double *shp,
@@ -109,6 +121,7 @@ typedef struct or_model {
     double int_work, ke, ie, ee, time;
     int64_t step, nnz, nnz_cap;
     int64_t *row_ptr; int32_t *col; double *shp, *dshp;
+    double *invk;                      /* 9n (peridynamics)                    */
     double geom[6]; int32_t nb[3]; int32_t *ijk; int search_count;
     int64_t ret_fail;
 } or_model;

@@ -51,6 +51,7 @@ contains
     LFINITE_STRAIN = (ifinite /= 0)
     IGRAVITY = grav
     PERIDYNAMICS = .false.
+    AUTO_TS = .false.
     NCORES_INPUT = 1
     PDSEARCH = 1
     call omp_set_num_threads(1)
@@ -258,6 +259,67 @@ contains
         fint, fext, dlt_fint, chard, gmax_wvel, dxs, dys, dzs, xm, ym, zm, bodyforce, gint_work, &
         body_id, gstrain_eq, dl)
     deallocate(sm_len, sm_area, sm_vol, nsni_fac, h_s, s_s, ddshp, ginvk, dxs, dys, dzs, xm, ym, zm, chard)
+    deallocate(ghost, lebc)
+  end subroutine
+
+  ! UDFM_SHAPE_TENSOR at one point (RK.f90:579-726): influence weights and the
+  ! inverse of the undeformed peridynamic shape tensor.
+  subroutine ref_udfm(x, gnump, gcoo, gvol, gwin, lstack, ln, ebcs, self_ebc, shp, invk) bind(C, name="ref_udfm")
+    integer(c_int), value :: gnump, ln, self_ebc
+    real(c_double), intent(in) :: x(3), gcoo(3,gnump), gvol(gnump), gwin(3,gnump)
+    integer(c_int), intent(in) :: lstack(ln), ebcs(gnump)
+    real(c_double), intent(out) :: shp(ln), invk(3,3)
+    logical, allocatable :: lebc(:)
+    logical :: lself
+    integer :: n, l
+    allocate(lebc(gnump))
+    lebc = (ebcs /= 0)
+    lself = (self_ebc /= 0)
+    n = gnump
+    l = ln
+    call UDFM_SHAPE_TENSOR(x, RK_DEGREE, RK_PSIZE, RK_CONT, RK_IMPL, gcoo, gvol, gwin, n, lstack, l, l, lebc, lself, &
+                           QL, QL_COEF, QL_LEN, shp, invk)
+    deallocate(lebc)
+  end subroutine
+
+  ! One CONSTRUCT_FINT_PD call (FINT_PD.F90:14-759) with harness-owned
+  ! neighbor lists, influence weights (gshp) and inverse shape tensors (ginvk).
+  subroutine ref_construct_fint_pd(gnump, gwin, gvol, gcoo, gcoo_cur, gprop, gstate, gstress, gstrain, &
+      gdinc, gdinc_tot, gmat_type, gebc, gn, gstart, dim, gstack, gshp, ginvk, gmaxn, linit, &
+      fint, gmax_wvel, dlt) bind(C, name="ref_construct_fint_pd")
+    integer(c_int), value :: gnump, dim, gmaxn, linit
+    real(c_double), intent(in) :: gwin(3,gnump), gvol(gnump), gcoo(3,gnump), gcoo_cur(3,gnump)
+    real(c_double), intent(in) :: gprop(30,gnump), gdinc(3*gnump), gdinc_tot(3*gnump)
+    real(c_double), intent(inout) :: gstate(20,gnump), gstress(6,gnump), gstrain(6,gnump)
+    integer(c_int), intent(in) :: gmat_type(gnump), gebc(gnump), gn(gnump), gstart(gnump), gstack(dim)
+    real(c_double), intent(inout) :: gshp(dim), ginvk(3,3,gnump)
+    real(c_double), intent(out) :: fint(3*gnump), gmax_wvel(gnump)
+    real(c_double), value :: dlt
+    double precision, allocatable :: sm_len(:,:), sm_area(:,:), sm_vol(:), nsni_fac(:,:), h_s(:,:), s_s(:,:)
+    double precision, allocatable :: dshp(:,:), ddshp(:,:), dxs(:,:), dys(:,:), dzs(:,:)
+    double precision, allocatable :: xm(:), ym(:), zm(:), chard(:), bod(:,:)
+    integer, allocatable :: ghost(:)
+    logical, allocatable :: lebc(:)
+    logical :: lin
+    double precision :: dlt_fint, dl, work
+    integer :: n, d, mx
+    n = gnump; d = dim; mx = gmaxn
+    allocate(sm_len(6,n), sm_area(3,n), sm_vol(n), nsni_fac(3,n), h_s(6,n), s_s(6,n))
+    allocate(dshp(3,d), ddshp(6,d), dxs(6,n), dys(6,n), dzs(6,n), xm(n), ym(n), zm(n), chard(n), bod(3,n))
+    allocate(ghost(n), lebc(n))
+    sm_len = 0; sm_area = 0; sm_vol = 0; nsni_fac = 0; h_s = 0; s_s = 0
+    dshp = 0; ddshp = 0; dxs = 0; dys = 0; dzs = 0; xm = 0; ym = 0; zm = 0; chard = 0; bod = 0
+    ghost = 0
+    lebc = (gebc /= 0)
+    lin = (linit /= 0)
+    dlt_fint = 0.0d0
+    dl = dlt
+    work = 0.0d0
+    call CONSTRUCT_FINT_PD(gwin, gvol, n, gcoo, gcoo_cur, sm_len, sm_area, sm_vol, nsni_fac, &
+        ghost, gprop, gstate, gstress, gstrain, h_s, s_s, gdinc, gdinc_tot, gmat_type, lebc, &
+        gn, gstart, d, gstack, gshp, dshp, ddshp, mx, ginvk, lin, &
+        fint, dlt_fint, chard, gmax_wvel, dxs, dys, dzs, xm, ym, zm, bod, work, dl)
+    deallocate(sm_len, sm_area, sm_vol, nsni_fac, h_s, s_s, dshp, ddshp, dxs, dys, dzs, xm, ym, zm, chard, bod)
     deallocate(ghost, lebc)
   end subroutine
 

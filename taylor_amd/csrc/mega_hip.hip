@@ -96,6 +96,10 @@ struct Params {
     int shsup, finite_strain, has_gravity;
     double grav[3];
     double dt;
+    // correspondence peridynamics
+    int pd, pd_linit;
+    double *Kinv;                                    // 9 per node, inverse shape tensor
+    double *gvec;                                    // 3 per node, sum_j w V_j (X_i - X_j)
 };
 
 template <class R> __device__ __forceinline__ R *as(void *p) { return static_cast<R *>(p); }
@@ -151,6 +155,11 @@ __device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
     p.d[q] = d;
     p.dtot[q] = t;
     p.v[q] = v + dt * 0.5 * a;
+    if (p.pd) {   // peridynamics: physical = generalized, so x moves with d
+        const double u = p.uphy[q] + d;
+        p.uphy[q] = u;
+        p.x[q] = p.X0[q] + u;
+    }
     if (p.d4) {
         const int64_t i = q / 3, k = q - 3 * i;
         reinterpret_cast<float *>(p.d4)[4 * i + k] = (float)d;
@@ -780,6 +789,40 @@ __device__ __forceinline__ void load_P(const R *Pb, int i, R *P)
     }
 }
 
+// one degree of freedom after its force is known: fint, EBC (fixity 1: v = a = 0,
+// reaction into fext), damping and the corrector (MAIN.F90:432-488), the
+// energy terms (MAIN.F90:524-553) and, fused, the next step's predictor
+template <bool CORRECT, bool PREDICT>
+__device__ __forceinline__ void dof_update(const Params &p, int node, int64_t q, double f, double fx, double &ke,
+                                           double &ie, double &ee)
+{
+    p.fint[q] = f;
+    if (!CORRECT) {
+        p.fext[q] = fx;
+        return;
+    }
+    const double fx_old = p.has_gravity ? p.fext_nmo[q] : 0.0;
+    double v = p.v[q];
+    if (p.ebc[q] == 1) {
+        v = 0.0;
+        p.a[q] = 0.0;
+        fx = -f;
+    } else {
+        const double mass = p.mass[q];
+        const double c = p.mats[p.mat[node]].damping * mass;
+        const double acc = (fx - f - c * v) / (mass + 0.5 * p.dt * c);
+        p.a[q] = acc;
+        v = v + 0.5 * p.dt * acc;
+    }
+    p.v[q] = v;
+    p.fext[q] = fx;
+    if (p.has_gravity) p.fext_nmo[q] = fx;
+    ke = 0.5 * p.mass[q] * v * v;
+    if (!p.finite_strain) ie = 0.5 * f * p.dtot[q];
+    ee = 0.5 * (fx + fx_old) * p.d[q];
+    if (PREDICT) predict_dof(p, q);
+}
+
 template <class R, bool CORRECT, bool PREDICT>
 __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
 {
@@ -830,36 +873,225 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
     if (p.has_gravity) fe = group_sum<kGroup>(fe);
     double ke = 0.0, ie = 0.0, ee = 0.0;
     if (live && lane < 3) {
-        const int64_t q = 3 * (int64_t)node + lane;
         const double f = (double)(lane == 0 ? f0 : (lane == 1 ? f1 : f2));
-        p.fint[q] = f;
-        double fx = p.has_gravity ? (double)fe * p.grav[lane] : 0.0;
-        if (!CORRECT) {
-            p.fext[q] = fx;
-        } else {
-            const double fx_old = p.has_gravity ? p.fext_nmo[q] : 0.0;
-            double v = p.v[q];
-            if (p.ebc[q] == 1) {
-                v = 0.0;
-                p.a[q] = 0.0;
-                fx = -f;
-            } else {
-                const double mass = p.mass[q];
-                const double c = p.mats[p.mat[node]].damping * mass;
-                const double acc = (fx - f - c * v) / (mass + 0.5 * p.dt * c);
-                p.a[q] = acc;
-                v = v + 0.5 * p.dt * acc;
-            }
-            p.v[q] = v;
-            p.fext[q] = fx;
-            if (p.has_gravity) p.fext_nmo[q] = fx;
-            ke = 0.5 * p.mass[q] * v * v;
-            if (!p.finite_strain) ie = 0.5 * f * p.dtot[q];
-            ee = 0.5 * (fx + fx_old) * p.d[q];
-            if (PREDICT) predict_dof(p, q);
-        }
+        const double fx = p.has_gravity ? (double)fe * p.grav[lane] : 0.0;
+        dof_update<CORRECT, PREDICT>(p, node, 3 * (int64_t)node + lane, f, fx, ke, ie, ee);
     }
     if (CORRECT) {   // per-block partials: no same-address atomics, deterministic
+        const double sk = block_sum(ke, s_red);
+        const double si = block_sum(ie, s_red);
+        const double se = block_sum(ee, s_red);
+        if (threadIdx.x == 0) {
+            p.epart[3 * blk] = sk;
+            p.epart[3 * blk + 1] = si;
+            p.epart[3 * blk + 2] += se;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// correspondence peridynamics (FINT_PD.F90:14-759, RK.f90:579-807), Lagrangian.
+//
+//   k_pd_shape   once: influence weights w_ij = prod_k w(|X_j,k - X_i,k| / a_j,k)
+//                (list and transposed order), K = sum w V_j xi xi^T and its inverse
+//                by the reference's pivoting INVERSE, g_i = sum w V_j (X_i - X_j)
+//   k_pd_gather  S = sum w V_j (x_j - x_i) (x) xi, L0 = sum w V_j (d_j - d_i) (x) xi
+//   k_pd_node    F = S K^-1, |det F|, F^-1 (pivoting INVERSE), L = L0 K^-1 F^-1,
+//                Hughes-Winget, rotation, trial, return map; T = J sigma F^-T K^-1
+//   k_pd_force   f_i = V_i (T_i g_i - sum_{k: i in N(k)} w_ki V_k T_k (X_k - X_i)):
+//                the reference's +/- bond scatter (FINT_PD.F90:629-655) as one
+//                deterministic gather over the transposed bond list, then the
+//                corrector as for the RK force
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int node = (int)(t / kGroup), lane = (int)(t % kGroup);
+    const bool live = node < p.n;
+    double K[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};   // K: 00 01 02 11 12 22
+    if (live) {
+        const double xi0 = p.X0[3 * node], xi1 = p.X0[3 * node + 1], xi2 = p.X0[3 * node + 2];
+        R *w = as<R>(p.shp), *tw = as<R>(p.tshp);
+        for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
+            const int j = p.col[q];
+            const double h0 = -(xi0 - p.X0[3 * j]), h1 = -(xi1 - p.X0[3 * j + 1]), h2 = -(xi2 - p.X0[3 * j + 2]);
+            double px, py, pz, dd;
+            mega::cubic_kernel(fabs(h0 / p.win[3 * j]), px, dd);
+            mega::cubic_kernel(fabs(h1 / p.win[3 * j + 1]), py, dd);
+            mega::cubic_kernel(fabs(h2 / p.win[3 * j + 2]), pz, dd);
+            const double om = px * py * pz, vj = p.vol[j];
+            w[q] = (R)om;
+            tw[p.tpos[q]] = (R)om;
+            K[0] += h0 * h0 * om * vj; K[1] += h0 * h1 * om * vj; K[2] += h0 * h2 * om * vj;
+            K[3] += h1 * h1 * om * vj; K[4] += h1 * h2 * om * vj; K[5] += h2 * h2 * om * vj;
+            g[0] -= om * vj * h0; g[1] -= om * vj * h1; g[2] -= om * vj * h2;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) K[k] = group_sum<kGroup>(K[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[k] = group_sum<kGroup>(g[k]);
+    if (live && lane == 0) {
+        const double Km[9] = {K[0], K[1], K[2], K[1], K[3], K[4], K[2], K[4], K[5]};
+        double Ki[9];
+        if (!mega::inverse_gj3(Km, Ki)) atomicAdd(&p.flags[2], 1);
+        double *out = p.Kinv + 9 * (int64_t)node;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) out[k] = Ki[k];
+        double *go = p.gvec + 3 * (int64_t)node;
+        go[0] = g[0]; go[1] = g[1]; go[2] = g[2];
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
+{
+    constexpr int W = std::is_same<R, float>::value ? 21 : 18;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int lane = threadIdx.x & (kGroup - 1);
+    const bool live = node < p.n;
+    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (live) {
+        const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
+        const double x[3] = {p.x[3 * node], p.x[3 * node + 1], p.x[3 * node + 2]};
+        const double d[3] = {p.d[3 * node], p.d[3 * node + 1], p.d[3 * node + 2]};
+        const R *w = as<R>(p.shp);
+        for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
+            const int j = p.col[q];
+            const double wv = (double)w[q] * p.vol[j];
+            const double h[3] = {p.X0[3 * j] - X[0], p.X0[3 * j + 1] - X[1], p.X0[3 * j + 2] - X[2]};
+            const double hs[3] = {p.x[3 * j] - x[0], p.x[3 * j + 1] - x[1], p.x[3 * j + 2] - x[2]};
+            const double hd[3] = {p.d[3 * j] - d[0], p.d[3 * j + 1] - d[1], p.d[3 * j + 2] - d[2]};
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    S[a + 3 * b] += hs[a] * h[b] * wv;
+                    L[a + 3 * b] += hd[a] * h[b] * wv;
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { S[k] = group_sum<kGroup>(S[k]); L[k] = group_sum<kGroup>(L[k]); }
+    if (live && lane == 0) {
+        R *red = as<R>(p.red) + (int64_t)W * node;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { red[k] = (R)S[k]; red[9 + k] = (R)L[k]; }
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pd_node(Params p)
+{
+    constexpr int W = std::is_same<R, float>::value ? 21 : 18;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const R *red = as<R>(p.red) + (int64_t)W * i;
+    double S[9], L0[9], Ki[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { S[k] = (double)red[k]; L0[k] = (double)red[9 + k]; Ki[k] = p.Kinv[9 * (int64_t)i + k]; }
+    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, det = 1.0;
+    if (!p.pd_linit && p.finite_strain) {
+        mega::mm3(S, Ki, F);
+        det = mega::det3_abs(F);
+        if (!mega::inverse_gj3(F, IF)) atomicAdd(&p.flags[2], 1);
+    }
+    double L1[9], Lm[9];
+    mega::mm3(L0, Ki, L1);
+    if (p.pd_linit) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Lm[k] = L1[k];
+    } else {
+        mega::mm3(L1, IF, Lm);
+    }
+    const Material &mt = p.mats[p.mat[i]];
+    R *stress = as<R>(p.stress) + 6 * (int64_t)i, *strain = as<R>(p.strain) + 6 * (int64_t)i;
+    double sig[6], e6[6], de[6], dd[6], old_s[6], Rot[9];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)stress[k]; e6[k] = (double)strain[k]; }
+    mega::hughes_winget(Lm, Rot, de, dd);   // FINT_PD.F90:475 (always, also for small strain)
+    mega::rotate_voigt(Rot, sig);
+    mega::rotate_voigt(Rot, e6);
+    R sr[6], der[6], pred[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { e6[k] = e6[k] + de[k]; sr[k] = (R)sig[k]; der[k] = (R)de[k]; }
+    mega::elastic_trial(sr, der, mt, pred);
+    R ep = as<R>(p.eps)[i];
+    if (mt.type == 2) {
+        if (mega::von_mises(sr, pred, ep, mt) < 0) atomicAdd(&p.flags[0], 1);
+    } else if (mt.type == 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sr[k] = pred[k];
+    } else {
+        atomicAdd(&p.flags[1], 1);
+    }
+    as<R>(p.eps)[i] = ep;
+    double si[6], ei[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        stress[k] = sr[k];
+        strain[k] = (R)e6[k];
+        sig[k] = (double)sr[k];
+        si[k] = sig[k] - old_s[k];
+        ei[k] = e6[k] - old_s[k];   // the reference's STRAIN_INC = LSTRAIN - GSTRESS (FINT_PD.F90:509)
+    }
+    if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
+    // T = det sigma IF^T K^-1 (FINT_PD.F90:620-624)
+    const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
+    double IFt[9], PK[9], T[9];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) IFt[a + 3 * b] = IF[b + 3 * a];
+    mega::mm3(Sg, IFt, PK);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) PK[k] = det * PK[k];
+    mega::mm3(PK, Ki, T);
+    R *P = as<R>(p.P) + pstride<R>() * (int64_t)i;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P[k] = (R)T[k];
+    P[9] = (R)p.vol[i];
+}
+
+template <class R, bool CORRECT, bool PREDICT>
+__global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
+{
+    __shared__ double s_red[kBlock / 64];
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int lane = threadIdx.x & (kGroup - 1);
+    const bool live = node < p.n;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const R *Pb = as<R>(p.P);
+    if (live) {
+        const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
+        const R *tw = as<R>(p.tshp);
+        for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
+            const int k = p.tsrc[t];
+            R T[10];
+            load_P<R>(Pb, k, T);
+            const double c = (double)tw[t] * (double)T[9];
+            const double dx[3] = {p.X0[3 * k] - X[0], p.X0[3 * k + 1] - X[1], p.X0[3 * k + 2] - X[2]};
+            a0 += c * ((double)T[0] * dx[0] + (double)T[3] * dx[1] + (double)T[6] * dx[2]);
+            a1 += c * ((double)T[1] * dx[0] + (double)T[4] * dx[1] + (double)T[7] * dx[2]);
+            a2 += c * ((double)T[2] * dx[0] + (double)T[5] * dx[1] + (double)T[8] * dx[2]);
+        }
+    }
+    a0 = group_sum<kGroup>(a0); a1 = group_sum<kGroup>(a1); a2 = group_sum<kGroup>(a2);
+    double ke = 0.0, ie = 0.0, ee = 0.0;
+    if (live && lane < 3) {
+        R T[10];
+        load_P<R>(Pb, node, T);
+        const double *g = p.gvec + 3 * (int64_t)node;
+        const double own = (double)T[lane] * g[0] + (double)T[lane + 3] * g[1] + (double)T[lane + 6] * g[2];
+        const double f = (double)T[9] * (own - (lane == 0 ? a0 : (lane == 1 ? a1 : a2)));
+        dof_update<CORRECT, PREDICT>(p, node, 3 * (int64_t)node + lane, f, 0.0, ke, ie, ee);
+    }
+    if (CORRECT) {
         const double sk = block_sum(ke, s_red);
         const double si = block_sum(ie, s_red);
         const double se = block_sum(ee, s_red);
@@ -1149,6 +1381,9 @@ struct mega_ctx {
     unsigned long long *hv_count = nullptr;
     int64_t hv_cap = 0;
     int hv_every = 0, hv_active = 0;
+    // correspondence peridynamics
+    int pd = 0, pd_shapes = 0;
+    double *Kinv = nullptr, *gvec = nullptr;
     // profiling
     int profile = 0;
     struct Timer { const char *name; hipEvent_t a, b; };
@@ -1197,6 +1432,7 @@ Params params(mega_ctx *c)
     p.surr_tiles = c->surr_tiles; p.surr_bias = c->surr_bias; p.surr_norm = c->surr_norm;
     p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
+    p.pd = c->pd; p.pd_linit = 0; p.Kinv = c->Kinv; p.gvec = c->gvec;
     return p;
 }
 
@@ -1252,10 +1488,11 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
     int e = 0;
     e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tq, cap); e |= dalloc(&c->tpos, cap);
     e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r); e |= dalloc(&c->tdshp, 3 * cap, r);
-    if (c->has_gravity) e |= dalloc(&c->tshp, cap, r);
+    if (c->has_gravity || c->pd) e |= dalloc(&c->tshp, cap, r);
     if (e) { free_pairs(c); return MEGA_ERR_NOMEM; }
     c->nnz_cap = cap;
     c->has_shapes = 0;
+    c->pd_shapes = 0;
     return MEGA_OK;
 }
 
@@ -1410,6 +1647,51 @@ int do_fint(mega_ctx *c, int fm, bool update_x, int force_mode)
                     : fint_kernels<double>(c, fm, update_x, force_mode);
 }
 
+// UDFM_SHAPE_TENSOR for all nodes (FINT_PD.F90:227-290)
+int do_pd_shapes(mega_ctx *c)
+{
+    if (!c->has_lists) return MEGA_ERR_ARG;
+    Params p = params(c);
+    Scope s(c, "k_pd_shape");
+    if (c->mixed) k_pd_shape<float><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p);
+    else k_pd_shape<double><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p);
+    HIPCHK(hipGetLastError());
+    c->pd_shapes = 1;
+    return MEGA_OK;
+}
+
+template <class R>
+int pd_kernels(mega_ctx *c, int linit, int force_mode)
+{
+    Params p = params(c);
+    p.pd_linit = linit;
+    int nb = 0;
+    const unsigned g = xcd_grid(c->n, &nb);
+    {
+        Scope s(c, "k_pd_gather");
+        k_pd_gather<R><<<g, kBlock, 0, c->stream>>>(p, nb);
+    }
+    {
+        Scope s(c, "k_pd_node");
+        k_pd_node<R><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+    }
+    if (force_mode >= 0) {
+        Scope s(c, "k_pd_force");
+        if (force_mode == 0) k_pd_force<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else if (force_mode == 1) k_pd_force<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        else k_pd_force<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+    }
+    HIPCHK(hipGetLastError());
+    return MEGA_OK;
+}
+
+// CONSTRUCT_FINT_PD on the device state; force_mode as do_fint
+int do_pd_fint(mega_ctx *c, int linit, int force_mode)
+{
+    if (!c->pd_shapes) return MEGA_ERR_ARG;
+    return c->mixed ? pd_kernels<float>(c, linit, force_mode) : pd_kernels<double>(c, linit, force_mode);
+}
+
 double host_sum(mega_ctx *c, const double *dev, int64_t count, int64_t stride, int64_t offset, int *err)
 {
     std::vector<double> h((size_t)count * stride);
@@ -1443,11 +1725,12 @@ int sync_d4(mega_ctx *c)
 
 int check_flags(mega_ctx *c)
 {
-    int32_t f[2];
+    int32_t f[3];
     HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (f[1]) return MEGA_ERR_MATERIAL;
     if (f[0]) return MEGA_ERR_RETURN_MAP;
+    if (f[2]) return MEGA_ERR_SINGULAR;   // a warning, as INVERSE.F90:93-95
     return MEGA_OK;
 }
 
@@ -1505,6 +1788,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3 || ctrl->itype_int != 0) return MEGA_ERR_ARG;
     if (ctrl->precision != MEGA_FP64 && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     if (ctrl->surrogate && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
+    // the reference's CONSTRUCT_FINT_PD is Lagrangian only (FINT_PD.F90:408-412)
+    if (ctrl->peridynamics && (!ctrl->lagrangian || ctrl->surrogate)) return MEGA_ERR_ARG;
     *out = nullptr;
     mega_ctx *c = new mega_ctx();
     c->c = *ctrl;
@@ -1512,6 +1797,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     c->n = n;
     c->mixed = ctrl->precision == MEGA_MIXED;
     c->rsz = c->mixed ? 4 : 8;
+    c->pd = ctrl->peridynamics ? 1 : 0;
     c->has_gravity = (ctrl->gravity[0] != 0.0 || ctrl->gravity[1] != 0.0 || ctrl->gravity[2] != 0.0);
     // materials: nodes with identical property rows share a slot
     std::vector<int32_t> mid(n);
@@ -1559,6 +1845,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (c->mixed) { e |= dalloc(&c->d4, n); e |= dalloc(&c->t4, n); }
     else e |= dalloc(&c->wave_vel, n);
     e |= dalloc(&c->Fst, 9 * (size_t)n);
+    if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
     e |= dalloc(&c->win, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
     e |= dalloc(&c->mats, c->host_mats.size());
@@ -1605,7 +1892,8 @@ int mega_destroy(mega_ctx *c)
                     c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->Fst, c->ebc, c->ebc_node, c->mat,
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
-                    c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count};
+                    c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
+                    c->Kinv, c->gvec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -1657,6 +1945,13 @@ int mega_get_neighbors(mega_ctx *c, int32_t *gn, int32_t *gstart, int32_t *gstac
 int mega_shapes(mega_ctx *c, const double *x_eval)
 {
     if (!c || !c->has_lists) return MEGA_ERR_ARG;
+    if (c->pd) {   // UDFM_SHAPE_TENSOR at the initial positions
+        HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
+        int r = do_pd_shapes(c);
+        if (r) return r;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return check_flags(c);
+    }
     const double *xs = c->X0;
     if (x_eval) {
         HIPCHK(hipMemcpyAsync(c->scratch, x_eval, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
@@ -1699,10 +1994,36 @@ int mega_get_shapes(mega_ctx *c, double *shp, double *dshp, int64_t cap)
     return st;
 }
 
+int mega_get_pd_shape(mega_ctx *c, double *omega, double *invk, int64_t cap)
+{
+    if (!c || !c->pd || !c->pd_shapes || (omega && cap < c->nnz)) return MEGA_ERR_ARG;
+    const int64_t m = c->nnz;
+    if (omega) {
+        double *buf = nullptr;
+        if (dalloc(&buf, (size_t)m)) return MEGA_ERR_NOMEM;
+        if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)c->shp, buf);
+        else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)c->shp, buf);
+        const bool ok = hipMemcpyAsync(omega, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+                        hipStreamSynchronize(c->stream) == hipSuccess;
+        (void)hipFree(buf);
+        if (!ok) return MEGA_ERR_HIP;
+    }
+    if (invk) {
+        HIPCHK(hipMemcpyAsync(invk, c->Kinv, sizeof(double) * 9 * c->n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return MEGA_OK;
+}
+
 int mega_interp(mega_ctx *c, const double *coef, double *phys)
 {
-    if (!c || !coef || !phys || !c->has_shapes) return MEGA_ERR_ARG;
+    if (!c || !coef || !phys) return MEGA_ERR_ARG;
     const size_t b = sizeof(double) * 3 * c->n;
+    if (c->pd) {   // physical = generalized (HANDELER.F90:134-137)
+        std::memcpy(phys, coef, b);
+        return MEGA_OK;
+    }
+    if (!c->has_shapes) return MEGA_ERR_ARG;
     double *out = nullptr;
     if (dalloc(&out, 3 * (size_t)c->n)) return MEGA_ERR_NOMEM;
     int st = MEGA_OK;
@@ -1743,9 +2064,14 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     const double w0 = host_sum(c, c->wpart, c->nb_node, 1, 0, &err);
     if (err) return err;
     const bool lag = c->c.lagrangian;
-    if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x);
-    else if (!c->has_shapes) r = MEGA_ERR_ARG;
-    if (!r) r = do_fint(c, (lag && c->c.finite_strain) ? F_DIRECT : F_NONE, false, 0);
+    if (c->pd) {   // CONSTRUCT_FINT_PD (FINT_PD.F90:14-759)
+        if (linit) r = do_pd_shapes(c);
+        if (!r) r = do_pd_fint(c, linit ? 1 : 0, 0);
+    } else {
+        if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x);
+        else if (!c->has_shapes) r = MEGA_ERR_ARG;
+        if (!r) r = do_fint(c, (lag && c->c.finite_strain) ? F_DIRECT : F_NONE, false, 0);
+    }
     if (r) return r;
     const int st = check_flags(c);
     const double w = host_sum(c, c->wpart, c->nb_node, 1, 0, &err) - w0;
@@ -1822,14 +2148,21 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
             HIPCHK(hipMemsetAsync(c->d, 0, sizeof(double) * 3 * n, c->stream));
             r = sync_d4(c);
             if (!r) r = search_step(c, c->x, 1);
-            if (!r) r = do_shapes(c, lag ? c->X0 : c->x);
-            if (!r) r = do_fint(c, fm == F_INCR ? F_DIRECT : F_NONE, false, -1);
+            if (c->pd) {
+                if (!r) r = do_pd_shapes(c);
+                if (!r) r = do_pd_fint(c, 1, -1);
+            } else {
+                if (!r) r = do_shapes(c, lag ? c->X0 : c->x);
+                if (!r) r = do_fint(c, fm == F_INCR ? F_DIRECT : F_NONE, false, -1);
+            }
             if (r) return r;
         }
         // the predictor is fused into the previous k_force within one call
         if (!c->predicted) r = do_predict(c);
         if (r) return r;
-        if (lag) {
+        if (c->pd) {
+            r = do_pd_fint(c, 0, last ? 1 : 2);   // DO_INTERP is the identity (HANDELER.F90:134-137)
+        } else if (lag) {
             r = do_fint(c, fm, true, last ? 1 : 2);   // DO_INTERP fused into the gather
         } else {
             Params p = params(c);
@@ -1882,7 +2215,11 @@ int mega_download(mega_ctx *c, int32_t field, double *host)
         else k_strain_eq<double><<<grid_for(n), kBlock, 0, c->stream>>>(n, (const double *)c->strain, c->scratch);
         HIPCHK(hipMemcpyAsync(host, c->scratch, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         break;
-    case MEGA_F_VPHY: {   // sum_j phi_ij v_j of the current velocity
+    case MEGA_F_VPHY: {   // sum_j phi_ij v_j of the current velocity (v itself for peridynamics)
+        if (c->pd) {
+            HIPCHK(hipMemcpyAsync(host, c->v, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+            break;
+        }
         if (!c->has_shapes) return MEGA_ERR_ARG;
         Params p = params(c);
         if (c->mixed) k_interp<float, false><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, c->v, c->scratch);

@@ -86,6 +86,47 @@ MEGA_HD void mm3(const T *a, const T *b, T *c)
             c[i + 3 * k] = a[i] * b[3 * k] + a[i + 3] * b[1 + 3 * k] + a[i + 6] * b[2 + 3 * k];
 }
 
+// INVERSE (INVERSE.F90:17-99) for a 3x3 matrix: Gauss-Jordan with full
+// pivoting.  When the largest remaining pivot d has d + 1 == 1 it stops and
+// returns the partly reduced matrix (the reference then only warns).
+// Returns 1 on success, 0 on that early stop.
+MEGA_HD int inverse_gj3(const double *A, double *Ai)
+{
+#define G_(i, j) a[(i) + 3 * (j)]
+    double a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] = A[e];
+    int is[3] = {0, 0, 0}, js[3] = {0, 0, 0};
+    int ok = 1;
+    for (int k = 0; k < 3 && ok; ++k) {
+        double d = 0.0;
+        for (int i = k; i < 3; ++i)
+            for (int j = k; j < 3; ++j)
+                if (fabs(G_(i, j)) > d) { d = fabs(G_(i, j)); is[k] = i; js[k] = j; }
+        if (d + 1.0 == 1.0) { ok = 0; break; }
+        for (int j = 0; j < 3; ++j) { const double t = G_(k, j); G_(k, j) = G_(is[k], j); G_(is[k], j) = t; }
+        for (int i = 0; i < 3; ++i) { const double t = G_(i, k); G_(i, k) = G_(i, js[k]); G_(i, js[k]) = t; }
+        G_(k, k) = 1.0 / G_(k, k);
+        for (int j = 0; j < 3; ++j)
+            if (j != k) G_(k, j) = G_(k, j) * G_(k, k);
+        for (int i = 0; i < 3; ++i)
+            if (i != k)
+                for (int j = 0; j < 3; ++j)
+                    if (j != k) G_(i, j) = G_(i, j) - G_(i, k) * G_(k, j);
+        for (int i = 0; i < 3; ++i)
+            if (i != k) G_(i, k) = -G_(i, k) * G_(k, k);
+    }
+    if (ok)
+        for (int k = 2; k >= 0; --k) {
+            for (int j = 0; j < 3; ++j) { const double t = G_(k, j); G_(k, j) = G_(js[k], j); G_(js[k], j) = t; }
+            for (int i = 0; i < 3; ++i) { const double t = G_(i, k); G_(i, k) = G_(i, is[k]); G_(i, is[k]) = t; }
+        }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Ai[e] = a[e];
+#undef G_
+    return ok;
+}
+
 // 4x4 inverse by cofactors (M44INV, INVERSE.F90:185-223) on the symmetric
 // moment matrix given by its 10 upper entries m[0..9] =
 // (00,01,02,03,11,12,13,22,23,33).  Writes the full inverse (column-major).

@@ -258,6 +258,7 @@ class Problem:
     norm_win: float = 1.84
     gravity: np.ndarray = field(default_factory=lambda: np.zeros(3))
     elcon: np.ndarray | None = None
+    peridynamics: int = 0    # CONSTRUCT_FINT_PD instead of CONSTRUCT_FINT (*PERIDYNAMICS 1)
 
     @property
     def n(self) -> int:
@@ -273,14 +274,30 @@ class Problem:
 def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float = 1.84,
                    radius: float = 3.8e-3, length: float = 25.4e-3, time_end: float = 80e-6,
                    dt_frac: float = 0.3, lagrangian: int = 1, pdsearch_steps: int = 20,
-                   material: Material | None = None, ql: int = 1, ql_coef: float = 0.001) -> Problem:
+                   material: Material | None = None, ql: int = 1, ql_coef: float = 0.001,
+                   peridynamics: int = 0, mm_units: bool = False) -> Problem:
     """Taylor bar hitting a fixed plane: BAR gets v_z = -v0, WALL (impact face)
-    gets fixity 0 0 1, dt = dt_frac * h / c_p."""
+    gets fixity 0 0 1, dt = dt_frac * h / c_p.  peridynamics=1 selects the
+    correspondence peridynamic force (Lagrangian only in the reference).
+    Arguments are SI; mm_units=True writes the deck in the consistent
+    mm / tonne / s / N / MPa system instead (lengths x1e3, stresses x1e-6,
+    density x1e-12), which keeps the peridynamic shape tensor O(1): in SI at
+    sub-mm spacing it is ~1e-16 and the reference's INVERSE treats it as
+    singular (INVERSE.F90:41-45)."""
+    if mm_units:
+        h, radius, length, v0 = h * 1e3, radius * 1e3, length * 1e3, v0 * 1e3
+        base = material or copper_linear()
+        p0 = base.props
+        material = Material.j2(E=p0[1] * 1e-6, nu=p0[0], rho=p0[2] * 1e-12, sig0=p0[3] * 1e-6, B=p0[4], CE=p0[5],
+                               damping=p0[20]) if base.mat_type == 2 else Material.elastic(p0[1] * 1e-6, p0[0],
+                                                                                           p0[2] * 1e-12)
     mesh = taylor_bar_mesh(h, radius=radius, length=length, seed=seed)
     mat = material or copper_linear()
-    return assemble(mesh, {"BAR": (0, 0, 0)} | {"WALL": (0, 0, 1)}, {"BAR": (0.0, 0.0, -v0)}, mat,
+    prob = assemble(mesh, {"BAR": (0, 0, 0)} | {"WALL": (0, 0, 1)}, {"BAR": (0.0, 0.0, -v0)}, mat,
                     norm_win=norm_win, dt=dt_frac * h / p_wave_speed(mat), time_end=time_end,
                     lagrangian=lagrangian, pdsearch=(1 if lagrangian else pdsearch_steps), ql=ql, ql_coef=ql_coef)
+    prob.peridynamics = int(peridynamics)
+    return prob
 
 
 def assemble(mesh: Mesh, fixity: dict, velocity: dict, mat: Material, norm_win: float, dt: float,
@@ -383,7 +400,7 @@ def write_control(path: str, kfile: str, prob: Problem, set_names: list, time_ou
            "*TIME PARAMETERS", "#TIME_END", "  %.17e" % prob.time_end, "#IAUTO_TS", "  0", "#DT", "  %.17e" % prob.dt, "",
            "*QL PARAMETERS", "# USE, COEFFICIENT", "   %d          %.17g" % (prob.ql, prob.ql_coef), "",
            "*RK PARAMETERS", "# RK_DEGREE, RK_CONT, RK_IMPL", "   1          3        0", "",
-           "*PERIDYNAMICS", "# SWITCH", "  0", "",
+           "*PERIDYNAMICS", "# SWITCH", "  %d" % int(prob.peridynamics), "",
            "*HPC PARAMETERS", "#HPC_SCHEME", "1", "",
            "*GHOSTING PARAMETERS", "#GHOST_BUFFER", "0", "",
            "*INTEGRATION PARAMETERS", "#TYPE", "0", "",

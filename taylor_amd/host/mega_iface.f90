@@ -21,6 +21,7 @@ module mega_iface
      integer(c_int32_t) :: shsup, lagrangian, finite_strain, pdsearch
      real(c_double)     :: gravity(3), dist_max(3)
      integer(c_int32_t) :: precision, surrogate
+     integer(c_int32_t) :: peridynamics, reserved
   end type mega_control
 
   interface

@@ -46,7 +46,8 @@ class MegaControl(C.Structure):
     _fields_ = [("rk_degree", C.c_int32), ("rk_cont", C.c_int32), ("itype_int", C.c_int32), ("ql", C.c_int32),
                 ("ql_coef", C.c_double), ("ql_len", C.c_double), ("shsup", C.c_int32), ("lagrangian", C.c_int32),
                 ("finite_strain", C.c_int32), ("pdsearch", C.c_int32), ("gravity", C.c_double * 3),
-                ("dist_max", C.c_double * 3), ("precision", C.c_int32), ("surrogate", C.c_int32)]
+                ("dist_max", C.c_double * 3), ("precision", C.c_int32), ("surrogate", C.c_int32),
+                ("peridynamics", C.c_int32), ("reserved", C.c_int32)]
 
 
 _lib = None
@@ -68,6 +69,7 @@ def load_library():
         "mega_get_neighbors": [vp, vp, vp, vp, i64, C.POINTER(i64)],
         "mega_shapes": [vp, vp],
         "mega_get_shapes": [vp, vp, vp, i64],
+        "mega_get_pd_shape": [vp, vp, vp, i64],
         "mega_interp": [vp, vp, vp],
         "mega_fint": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, dp],
         "mega_constitute": [i32, vp, vp, vp, vp, vp, i32, i32, i32],
@@ -138,6 +140,7 @@ def make_control(prob, precision: int = MEGA_FP64, surrogate: int = 0) -> MegaCo
         c.gravity[k] = float(prob.gravity[k])
         c.dist_max[k] = float(prob.dist_max[k])
     c.precision, c.surrogate = int(precision), int(surrogate)
+    c.peridynamics = int(getattr(prob, "peridynamics", 0))
     return c
 
 
@@ -185,7 +188,7 @@ class Mega:
 
     def shapes(self, x_eval=None):
         xs = None if x_eval is None else _f64(x_eval)
-        _chk(self.lib.mega_shapes(self.ctx, _p(xs)), "mega_shapes")
+        return _chk(self.lib.mega_shapes(self.ctx, _p(xs)), "mega_shapes", allow=(2,))
 
     def get_shapes(self):
         gn, _, _ = self.neighbors()
@@ -194,6 +197,13 @@ class Mega:
         dshp = np.zeros((max(nnz, 1), 3))
         _chk(self.lib.mega_get_shapes(self.ctx, _p(shp), _p(dshp), nnz), "mega_get_shapes")
         return shp[:nnz], dshp[:nnz]
+
+    def pd_shape(self):
+        """(omega[nnz], invk[n, 9]) of the peridynamic shape tensors."""
+        nnz, _, _ = self.stats()
+        w, k = np.zeros(max(nnz, 1)), np.zeros((self.n, 9))
+        _chk(self.lib.mega_get_pd_shape(self.ctx, _p(w), _p(k), nnz), "mega_get_pd_shape")
+        return w[:nnz], k
 
     def interp(self, coef):
         coef = _f64(coef, (self.n, 3))
@@ -212,12 +222,12 @@ class Mega:
         st = self.lib.mega_fint(self.ctx, int(linit), _p(_f64(dinc)), _p(_f64(dinc_tot)),
                                 _p(None if x_cur is None else _f64(x_cur)), _p(stress), _p(strain), _p(state),
                                 _p(fint), _p(fext), C.byref(wk))
-        _chk(st, "mega_fint", allow=(1,))
+        _chk(st, "mega_fint", allow=(1, 2))
         return fint, fext, wk.value, st
 
     def step(self, nsteps: int = 1, dt: float | None = None):
         st = self.lib.mega_step(self.ctx, float(dt or self.prob.dt), int(nsteps))
-        _chk(st, "mega_step", allow=(1,))
+        _chk(st, "mega_step", allow=(1, 2))
         return st
 
     def download(self, name: str) -> np.ndarray:

@@ -6,7 +6,9 @@ through the bind(C) harness and runs the reference program MEGA on synthetic
 decks written by taylor_amd.deck, then stores inputs and outputs as small
 compressed .npz files.  The fixtures are data only.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # dense, geometry, search/RK/FINT, runs
+    python tests/golden/make_golden.py --pd       # peridynamics vectors and run
+    python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
 """
 from __future__ import annotations
 
@@ -169,6 +171,67 @@ def search_rk_fint_vectors(ref, rng):
     return out
 
 
+def pd_vectors(ref, rng):
+    """Peridynamics (CONSTRUCT_FINT_PD) on the small bar: UDFM_SHAPE_TENSOR at
+    every node in SI units (the shape tensor is ~1e-16 there, so INVERSE stops
+    early and returns it unreduced) and in mm units, and one CONSTRUCT_FINT_PD
+    call per LINIT value on a yielding random state in mm units."""
+    out = {}
+    for tag, mm in (("si", False), ("mm", True)):
+        prob = small(peridynamics=1, mm_units=mm)
+        ref_set_control(ref, prob)
+        n = prob.n
+        coo, win, vol = (np.ascontiguousarray(prob.coo), np.ascontiguousarray(prob.win),
+                         np.ascontiguousarray(prob.vol))
+        d = prob.dist_max
+        geo, nb = np.zeros(6), np.zeros(3, np.int32)
+        ref.ref_bin_geometry(n, coo, win, float(d[0]), float(d[1]), float(d[2]), geo, nb)
+        ijk = np.zeros((n, 3), np.int32)
+        ref.ref_get_bins(n, coo, ijk)
+        dim = n * 200
+        gn, gs, st = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(dim, np.int32)
+        mx, nz = C.c_int(), C.c_int()
+        ref.ref_soft_search(n, coo, win, np.zeros((n, 6)), float(d[0]), float(d[1]), float(d[2]), gn, gs, dim, st,
+                            C.byref(mx), C.byref(nz))
+        st = st[:nz.value]
+        ebc = np.ascontiguousarray(prob.ebc_node, dtype=np.int32)
+        w, invk = [], []
+        for i in range(n):
+            a, b = gs[i] - 1, gs[i] - 1 + gn[i]
+            s1, k1 = np.zeros(gn[i]), np.zeros(9)
+            ref.ref_udfm(coo[i].copy(), n, coo, vol, win, np.ascontiguousarray(st[a:b]), int(gn[i]), ebc, int(ebc[i]),
+                         s1, k1)
+            w.append(s1)
+            invk.append(k1)
+        w, invk = np.concatenate(w), np.array(invk)
+        out.update({f"{tag}_gn": gn, f"{tag}_gstack": st, f"{tag}_omega": w, f"{tag}_invk": invk})
+        if not mm:
+            continue
+        nnz = int(gn.sum())
+        props = np.ascontiguousarray(prob.props_table()[prob.mat_id])
+        L = 0.7   # lattice spacing in the deck's units
+        sig0 = prob.materials[0].props[3]
+        for linit in (1, 0):
+            dtot = rng.normal(size=(n, 3)) * 2e-3 * L
+            dinc = rng.normal(size=(n, 3)) * 4e-4 * L + np.array([0, 0, -1e-3 * L])
+            stress = rng.normal(size=(n, 6)) * 0.75 * sig0
+            strain = rng.normal(size=(n, 6)) * 1e-3
+            state = np.zeros((n, 20))
+            state[:, 2] = np.abs(rng.normal(size=n)) * 0.02
+            s, e, t = stress.copy(), strain.copy(), state.copy()
+            ww = np.zeros(nnz) if linit else w.copy()
+            kk = np.zeros((n, 9)) if linit else invk.copy()
+            f, wv = np.zeros((n, 3)), np.zeros(n)
+            ref.ref_construct_fint_pd(n, win, vol, coo, np.ascontiguousarray(coo + dtot), props, t, s, e,
+                                      np.ascontiguousarray(dinc), np.ascontiguousarray(dtot), np.full(n, 2, np.int32),
+                                      ebc, gn, gs, nnz, st, ww, kk, int(gn.max()), linit, f, wv, prob.dt)
+            out.update({f"pd{linit}_dtot": dtot, f"pd{linit}_dinc": dinc, f"pd{linit}_stress0": stress,
+                        f"pd{linit}_strain0": strain, f"pd{linit}_eps0": state[:, 2].copy(), f"pd{linit}_f": f,
+                        f"pd{linit}_stress": s, f"pd{linit}_strain": e, f"pd{linit}_eps": t[:, 2].copy(),
+                        f"pd{linit}_wave": wv})
+    return out
+
+
 def mega_run(prob, mesh, nsteps, frames, workdir):
     """Run the reference program on the deck; return the chosen frames."""
     kfile = os.path.join(workdir, "bar.k")
@@ -214,10 +277,32 @@ def final_state_vectors(workdir):
             "eps": fr["eps"]}
 
 
+def pd_main():
+    ref = load_ref()
+    rng = np.random.default_rng(20260203)
+    with tempfile.TemporaryDirectory() as wd:
+        cwd = os.getcwd()
+        os.chdir(wd)   # the reference's WARN appends to MEGA.log and WARN.log in the working directory
+        open("MEGA.log", "a").close()
+        open("WARN.log", "a").close()
+        try:
+            np.savez_compressed(os.path.join(HERE, "pd.npz"), **pd_vectors(ref, rng))
+        finally:
+            os.chdir(cwd)
+    prob = small(peridynamics=1, mm_units=True)
+    mesh = deck.taylor_bar_mesh(H * 1e3, radius=3.8e-3 * 1e3, length=LEN * 1e3, seed=0)
+    assert np.array_equal(mesh.coo, prob.coo)
+    with tempfile.TemporaryDirectory() as wd:
+        np.savez_compressed(os.path.join(HERE, "run_pd.npz"), **mega_run(prob, mesh, 40, (1, 10, 40), wd))
+
+
 def main():
     if "--final" in sys.argv:
         with tempfile.TemporaryDirectory() as wd:
             np.savez_compressed(os.path.join(HERE, "taylor_h05_final.npz"), **final_state_vectors(wd))
+        return
+    if "--pd" in sys.argv:
+        pd_main()
         return
     ref = load_ref()
     rng = np.random.default_rng(20260101)

@@ -21,7 +21,7 @@ lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 class OrControl(C.Structure):
     _fields_ = [("rk_degree", C.c_int), ("rk_cont", C.c_int), ("ql", C.c_int), ("ql_coef", C.c_double),
                 ("ql_len", C.c_double), ("shsup", C.c_int), ("lagrangian", C.c_int),
-                ("finite_strain", C.c_int), ("gravity", C.c_double * 3)]
+                ("finite_strain", C.c_int), ("gravity", C.c_double * 3), ("peridynamics", C.c_int)]
 
 
 class OrFintIO(C.Structure):
@@ -32,6 +32,7 @@ class OrFintIO(C.Structure):
                 ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("shp", C.c_void_p), ("dshp", C.c_void_p),
                 ("stress", C.c_void_p), ("strain", C.c_void_p), ("state", C.c_void_p),
                 ("fint", C.c_void_p), ("fext", C.c_void_p), ("wave_vel", C.c_void_p), ("strain_eq", C.c_void_p),
+                ("invk", C.c_void_p),
                 ("int_work", C.c_double), ("linit", C.c_int), ("nthreads", C.c_int), ("ret_fail", C.c_int64)]
 
 
@@ -48,7 +49,7 @@ class OrModel(C.Structure):
                 ("int_work", C.c_double), ("ke", C.c_double), ("ie", C.c_double), ("ee", C.c_double),
                 ("time", C.c_double), ("step", C.c_int64), ("nnz", C.c_int64), ("nnz_cap", C.c_int64),
                 ("row_ptr", C.POINTER(C.c_int64)), ("col", C.POINTER(C.c_int32)),
-                ("shp", C.POINTER(C.c_double)), ("dshp", C.POINTER(C.c_double)),
+                ("shp", C.POINTER(C.c_double)), ("dshp", C.POINTER(C.c_double)), ("invk", C.POINTER(C.c_double)),
                 ("geom", C.c_double * 6), ("nb", C.c_int32 * 3), ("ijk", C.POINTER(C.c_int32)),
                 ("search_count", C.c_int), ("ret_fail", C.c_int64)]
 
@@ -81,6 +82,10 @@ def load_oracle():
     lib.or_construct_fint.argtypes = [C.POINTER(OrFintIO), C.POINTER(OrControl)]
     lib.or_construct_fint.restype = C.c_int
     lib.or_interp.argtypes = [C.c_int, lp, ip, dp, dp, dp]
+    lib.or_udfm.argtypes = [dp, C.c_int, dp, dp, dp, ip, C.c_int, dp, dp]
+    lib.or_dfm.argtypes = [dp, dp, dp, dp, ip, C.c_int, dp, dp, dp]
+    lib.or_construct_fint_pd.argtypes = [C.POINTER(OrFintIO), C.POINTER(OrControl)]
+    lib.or_construct_fint_pd.restype = C.c_int
     lib.or_model_init.argtypes = [C.POINTER(OrModel)]
     lib.or_model_init.restype = C.c_int
     lib.or_model_step.argtypes = [C.POINTER(OrModel)]
@@ -114,6 +119,9 @@ def load_ref():
                                         C.c_int, ip, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, ip, dp,
                                         C.POINTER(C.c_double), C.c_double])
     lib.ref_preprocess.argtypes = [C.c_int, C.c_int, ip, dp, dp, dp, dp, dp]
+    lib.ref_udfm.argtypes = [dp, C.c_int, dp, dp, dp, ip, C.c_int, ip, C.c_int, dp, dp]
+    lib.ref_construct_fint_pd.argtypes = ([C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip, ip, ip,
+                                           C.c_int, ip, dp, dp, C.c_int, C.c_int, dp, dp, C.c_double])
     return lib
 
 
@@ -123,6 +131,7 @@ def control(prob) -> OrControl:
     c.shsup, c.lagrangian, c.finite_strain = int(prob.shsup), int(prob.lagrangian), int(prob.finite_strain)
     for k in range(3):
         c.gravity[k] = float(prob.gravity[k])
+    c.peridynamics = int(getattr(prob, "peridynamics", 0))
     return c
 
 
@@ -222,6 +231,47 @@ def oracle_shapes(lib, prob, rp, col, xs=None):
         shp[a:b] = s1
         dshp[a:b] = d1.reshape(ln, 3)
     return shp, dshp
+
+
+def oracle_udfm(lib, prob, rp, col):
+    """UDFM_SHAPE_TENSOR at every node: influence weights (nnz) and K^-1 (n, 9)."""
+    n = prob.n
+    coo = np.ascontiguousarray(prob.coo, dtype=np.float64)
+    vol, win = np.ascontiguousarray(prob.vol), np.ascontiguousarray(prob.win)
+    w = np.zeros(int(rp[-1]))
+    invk = np.zeros((n, 9))
+    for i in range(n):
+        a, b = int(rp[i]), int(rp[i + 1])
+        s1, k1 = np.zeros(b - a), np.zeros(9)
+        lib.or_udfm(coo[i].copy(), n, coo, vol, win, np.ascontiguousarray(col[a:b]), b - a, s1, k1)
+        w[a:b], invk[i] = s1, k1
+    return w, invk
+
+
+def oracle_fint_pd(lib, prob, rp, col, omega, invk, linit, dinc, dinc_tot, x_cur, stress, strain, state,
+                   nthreads=1):
+    """One or_construct_fint_pd call; omega/invk/stress/strain/state updated in
+    place.  Returns fint, wave_vel, status."""
+    n = prob.n
+    io = OrFintIO()
+    keep = dict(win=np.ascontiguousarray(prob.win), vol=np.ascontiguousarray(prob.vol),
+                coo0=np.ascontiguousarray(prob.coo, dtype=np.float64),
+                coo_cur=np.ascontiguousarray(prob.coo if x_cur is None else x_cur, dtype=np.float64),
+                props=np.ascontiguousarray(prob.props_table()), mat_id=np.ascontiguousarray(prob.mat_id, dtype=np.int32),
+                mat_type=prob.mat_types(), ebc_node=np.ascontiguousarray(prob.ebc_node, dtype=np.int32),
+                dinc=np.ascontiguousarray(dinc, dtype=np.float64), dinc_tot=np.ascontiguousarray(dinc_tot, dtype=np.float64),
+                row_ptr=np.ascontiguousarray(rp, dtype=np.int64), col=np.ascontiguousarray(col, dtype=np.int32))
+    fint, fext, wv = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    io.n = n
+    for k, a in keep.items():
+        setattr(io, k, _ptr(a))
+    for k, a in dict(shp=omega, invk=invk, stress=stress, strain=strain, state=state).items():
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        setattr(io, k, _ptr(a))
+    io.fint, io.fext, io.wave_vel = _ptr(fint), _ptr(fext), _ptr(wv)
+    io.linit, io.nthreads = int(linit), nthreads
+    st = lib.or_construct_fint_pd(C.byref(io), C.byref(control(prob)))
+    return fint, wv, st
 
 
 def oracle_fint(lib, prob, rp, col, shp, dshp, linit, dinc, dinc_tot, x_cur, stress, strain, state,

@@ -1,0 +1,117 @@
+"""Correspondence peridynamics on the GPU (k_pd_*) against the reference's
+CONSTRUCT_FINT_PD: golden vectors written from the compiled reference
+(tests/golden/pd.npz, run_pd.npz) and the oracle restatement, which is
+bit-exact to them.  Bond lists are the SOFT_SEARCH neighbor lists."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_bind import OracleRun
+from taylor_amd import deck, mega
+
+pytestmark = pytest.mark.gpu
+H, LEN = 0.7e-3, 6e-3
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def small(**kw):
+    return deck.taylor_problem(H, length=LEN, seed=0, pdsearch_steps=3, peridynamics=1, **kw)
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("tag,mm", [("si", False), ("mm", True)])
+def test_pd_bonds_and_shape_tensors(tag, mm):
+    g = load("pd.npz")
+    prob = small(mm_units=mm)
+    m = mega.Mega(prob)
+    m.search()
+    gn, gs, st = m.neighbors()
+    assert np.array_equal(gn, g[f"{tag}_gn"]) and np.array_equal(st, g[f"{tag}_gstack"])
+    m.shapes()
+    w, invk = m.pd_shape()
+    m.close()
+    ew, ek = rel_err(w, g[f"{tag}_omega"]), rel_err(invk, g[f"{tag}_invk"])
+    print(f"{tag}: influence weights {ew:.2e}, inverse shape tensors {ek:.2e}")
+    assert ew < 1e-14 and ek < 1e-12
+
+
+@pytest.mark.parametrize("linit", [1, 0])
+def test_construct_fint_pd_one_call(linit):
+    g = load("pd.npz")
+    prob = small(mm_units=True)
+    n = prob.n
+    m = mega.Mega(prob)
+    m.search()
+    if not linit:
+        m.shapes()
+    s, e = g[f"pd{linit}_stress0"].copy(), g[f"pd{linit}_strain0"].copy()
+    st = np.zeros((n, 20))
+    st[:, 2] = g[f"pd{linit}_eps0"]
+    dtot = g[f"pd{linit}_dtot"]
+    f, fx, w, status = m.construct_fint(linit, g[f"pd{linit}_dinc"], dtot, prob.coo + dtot, s, e, st)
+    m.close()
+    ef = rel_err(f, g[f"pd{linit}_f"])
+    es, ee, ep = rel_err(s, g[f"pd{linit}_stress"]), rel_err(e, g[f"pd{linit}_strain"]), rel_err(st[:, 2], g[f"pd{linit}_eps"])
+    print(f"linit {linit}: fint {ef:.2e} stress {es:.2e} strain {ee:.2e} eps {ep:.2e}")
+    assert status == 0
+    assert ef < 1e-10 and es < 1e-10 and ee < 1e-10 and ep < 1e-10
+    assert np.all(fx == 0.0) and w == 0.0
+
+
+def test_pd_steps_match_reference_program():
+    """40 device-resident steps against the reference program's frames."""
+    g = load("run_pd.npz")
+    prob = small(mm_units=True)
+    m = mega.Mega(prob)
+    done = 0
+    for k in (1, 10, 40):
+        m.step(k - done)
+        done = k
+        errs = {"x": rel_err(m.download("x"), g[f"step{k}_x"]),
+                "u": rel_err(m.download("uphy"), g[f"step{k}_u"]),
+                # the reference writes the velocity of DO_INTERP time, before the
+                # corrector's half step: v - dt/2 a after the step
+                "v": rel_err(m.download("v") - 0.5 * prob.dt * m.download("a"), g[f"step{k}_vphy"]),
+                "fint": rel_err(m.download("fint"), g[f"step{k}_fint"]),
+                "stress": rel_err(m.download("stress"), g[f"step{k}_stress"])}
+        print(f"step {k}: " + " ".join(f"{a} {b:.2e}" for a, b in errs.items()))
+        assert errs["x"] < 1e-12 and errs["u"] < 1e-10
+        assert errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+    m.close()
+
+
+def test_pd_si_units_degenerate_like_the_reference(oracle):
+    """In SI units at sub-mm spacing the shape tensor is below INVERSE's pivot
+    threshold: the reference warns and keeps the unreduced tensor; so does the
+    device (status MEGA_ERR_SINGULAR, a warning), and the steps still follow
+    the oracle."""
+    prob = small()
+    m = mega.Mega(prob)
+    st = m.step(5)
+    run = OracleRun(oracle, prob).step(5)
+    assert st == 2
+    assert rel_err(m.download("x"), run.arr["x"]) < 1e-12
+    assert rel_err(m.download("v"), run.arr["v"]) < 1e-9
+    m.close()
+
+
+def test_pd_mixed_precision_steps():
+    prob = small(mm_units=True)
+    out = {}
+    for prec in (mega.MEGA_FP64, mega.MEGA_MIXED):
+        m = mega.Mega(prob, precision=prec)
+        m.step(40)
+        out[prec] = {k: m.download(k) for k in ("x", "uphy", "v", "stress")}
+        m.close()
+    errs = {k: rel_err(out[mega.MEGA_MIXED][k], out[mega.MEGA_FP64][k]) for k in out[mega.MEGA_FP64]}
+    print("mixed vs FP64 after 40 steps:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["x"] < 1e-7 and errs["uphy"] < 1e-4 and errs["v"] < 1e-3 and errs["stress"] < 1e-3

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_bind import OracleRun, control, oracle_fint, oracle_search, oracle_shapes
+from oracle_bind import OracleRun, control, oracle_fint, oracle_fint_pd, oracle_search, oracle_shapes, oracle_udfm
 from taylor_amd import deck
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -18,8 +18,8 @@ def load(name):
     return np.load(os.path.join(GOLD, name))
 
 
-def small(lagrangian=1):
-    return deck.taylor_problem(H, length=LEN, seed=0, lagrangian=lagrangian, pdsearch_steps=3)
+def small(lagrangian=1, **kw):
+    return deck.taylor_problem(H, length=LEN, seed=0, lagrangian=lagrangian, pdsearch_steps=3, **kw)
 
 
 def test_dense_routines_bit_exact(oracle):
@@ -119,6 +119,61 @@ def test_whole_run_bit_exact(oracle, lag, name):
     """40 explicit steps of the reference program MEGA (frames 1, 10, 40)."""
     g = load(name)
     run = OracleRun(oracle, small(lagrangian=lag))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        assert np.array_equal(A["x"], g[f"step{k}_x"])
+        assert np.array_equal(A["dphy_tot"], g[f"step{k}_u"])
+        assert np.array_equal(A["vphy"], g[f"step{k}_vphy"])
+        assert np.array_equal(A["fint"], g[f"step{k}_fint"])
+        assert np.array_equal(A["stress"], g[f"step{k}_stress"])
+        assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
+
+
+@pytest.mark.parametrize("tag,mm", [("si", False), ("mm", True)])
+def test_pd_bonds_and_shape_tensor_bit_exact(oracle, tag, mm):
+    """Bond lists (SOFT_SEARCH) and UDFM_SHAPE_TENSOR per node.  In SI units the
+    shape tensor is ~1e-16, INVERSE stops at its first pivot and returns it
+    unreduced; the restatement does the same."""
+    g = load("pd.npz")
+    prob = small(peridynamics=1, mm_units=mm)
+    rp, col, _ = oracle_search(oracle, prob)
+    assert np.array_equal(np.diff(rp), g[f"{tag}_gn"]) and np.array_equal(col + 1, g[f"{tag}_gstack"])
+    w, invk = oracle_udfm(oracle, prob, rp, col)
+    assert np.array_equal(w, g[f"{tag}_omega"])
+    assert np.array_equal(invk, g[f"{tag}_invk"])
+
+
+@pytest.mark.parametrize("linit", [1, 0])
+def test_construct_fint_pd_bit_exact(oracle, linit):
+    g = load("pd.npz")
+    prob = small(peridynamics=1, mm_units=True)
+    n = prob.n
+    rp, col, _ = oracle_search(oracle, prob)
+    w = np.zeros(int(rp[-1])) if linit else g["mm_omega"].copy()
+    invk = np.zeros((n, 9)) if linit else np.ascontiguousarray(g["mm_invk"]).copy()
+    s, e = g[f"pd{linit}_stress0"].copy(), g[f"pd{linit}_strain0"].copy()
+    st = np.zeros((n, 20))
+    st[:, 2] = g[f"pd{linit}_eps0"]
+    dtot = g[f"pd{linit}_dtot"]
+    f, wv, status = oracle_fint_pd(oracle, prob, rp, col, w, invk, linit, g[f"pd{linit}_dinc"], dtot,
+                                   prob.coo + dtot, s, e, st)
+    assert status == 0
+    assert np.array_equal(f, g[f"pd{linit}_f"])
+    assert np.array_equal(s, g[f"pd{linit}_stress"])
+    assert np.array_equal(e, g[f"pd{linit}_strain"])
+    assert np.array_equal(st[:, 2], g[f"pd{linit}_eps"])
+    assert np.array_equal(wv, g[f"pd{linit}_wave"])
+    if linit:
+        assert np.array_equal(w, g["mm_omega"]) and np.array_equal(invk, g["mm_invk"])
+
+
+def test_whole_run_pd_bit_exact(oracle):
+    """40 explicit steps of the reference program with *PERIDYNAMICS 1 (mm deck)."""
+    g = load("run_pd.npz")
+    run = OracleRun(oracle, small(peridynamics=1, mm_units=True))
     done = 0
     for k in (1, 10, 40):
         run.step(k - done)
