@@ -56,9 +56,20 @@ using mega::Material;
 
 namespace {
 
+#ifndef MEGA_KGROUP
+#define MEGA_KGROUP 16
+#endif
+#ifndef MEGA_UNROLL_MIX
+#define MEGA_UNROLL_MIX 6
+#endif
+#ifndef MEGA_UNROLL_F64
+#define MEGA_UNROLL_F64 4
+#endif
 constexpr int kBlock = 256;
-constexpr int kGroup = 16;      // lanes cooperating on one node's neighbor list
-constexpr int kXcd = 8;         // XCDs on MI355X
+constexpr int kGroup = MEGA_KGROUP;   // lanes cooperating on one node's neighbor list
+constexpr int kXcd = 8;               // XCDs on MI355X
+// neighbor-list entries each lane keeps in flight in the gathers
+constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
 
 // ---------------------------------------------------------------------------
 // kernel arguments
@@ -401,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 {
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr bool DIR = FM == F_DIRECT;
-    constexpr int U = MIX ? 6 : 4;
+    constexpr int U = MIX ? kUnrollMix : kUnrollF64;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
     const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
@@ -828,7 +839,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
 {
     __shared__ double s_red[kBlock / 64];
     constexpr bool MIX = std::is_same<R, float>::value;
-    constexpr int U = MIX ? 6 : 4;
+    constexpr int U = MIX ? kUnrollMix : kUnrollF64;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
     const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
@@ -1201,15 +1212,17 @@ __global__ void k_bin_fill(int n, const int32_t *ijk, BinGeom g, int32_t *cursor
     cell_nodes[slot] = i;
 }
 
-__global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, int32_t *cell_nodes)
+// nodes of a cell in the caller's node order (orig: internal -> caller id), as
+// GET_BINS appends them (BIN_SEARCH.F90:45-58)
+__global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32_t *orig, int32_t *cell_nodes)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= ncell) return;
     const int s = cell_start[m], e = cell_start[m + 1];
     for (int a = s + 1; a < e; ++a) {   // cells hold a few tens of nodes
-        const int v = cell_nodes[a];
+        const int v = cell_nodes[a], ov = orig[v];
         int b = a - 1;
-        while (b >= s && cell_nodes[b] > v) { cell_nodes[b + 1] = cell_nodes[b]; --b; }
+        while (b >= s && orig[cell_nodes[b]] > ov) { cell_nodes[b + 1] = cell_nodes[b]; --b; }
         cell_nodes[b + 1] = v;
     }
 }
@@ -1384,6 +1397,9 @@ struct mega_ctx {
     // correspondence peridynamics
     int pd = 0, pd_shapes = 0;
     double *Kinv = nullptr, *gvec = nullptr;
+    // node numbering: internal node i is the caller's node perm[i]
+    std::vector<int32_t> perm, iperm;
+    int32_t *orig = nullptr;   // device copy of perm
     // profiling
     int profile = 0;
     struct Timer { const char *name; hipEvent_t a, b; };
@@ -1520,7 +1536,7 @@ int do_bins(mega_ctx *c, const double *x)
     if (r) return r;
     HIPCHK(hipMemcpyAsync(c->cell_cursor, c->cell_start, sizeof(int32_t) * c->ncell, hipMemcpyDeviceToDevice, c->stream));
     k_bin_fill<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->ijk, c->geom, c->cell_cursor, c->cell_nodes);
-    k_bin_sort<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->cell_nodes);
+    k_bin_sort<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->orig, c->cell_nodes);
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }
@@ -1757,6 +1773,84 @@ int state_in(mega_ctx *c, void *dst, int64_t m, const double *host)
     return MEGA_OK;
 }
 
+// Nodes are numbered internally in Morton (Z-curve) order of their initial
+// positions on a grid of about one node spacing, so that consecutive internal
+// ids -- one block of a pair kernel, one XCD's range -- are compact in space
+// and share their neighbors in cache.  Arrays at the C-ABI stay in the
+// caller's order; these move them across.
+uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+std::vector<int32_t> morton_order(int n, const double *X0, const double *vol)
+{
+    double vsum = 0.0, mn[3] = {X0[0], X0[1], X0[2]};
+    for (int i = 0; i < n; ++i) {
+        vsum += vol[i];
+        for (int k = 0; k < 3; ++k) mn[k] = std::min(mn[k], X0[3 * (int64_t)i + k]);
+    }
+    double h = std::cbrt(vsum / n);
+    if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            double q = std::floor((X0[3 * (int64_t)i + k] - mn[k]) / h + 0.5);
+            q = std::min(std::max(q, 0.0), 2097151.0);
+            code |= spread3((uint64_t)q) << k;
+        }
+        key[i] = {code, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+    return perm;
+}
+
+// list-order pair values (w per pair) of the internal rows, written out row by
+// row in the caller's node order
+int rows_to_caller(mega_ctx *c, const double *in, double *out, int w)
+{
+    std::vector<int32_t> rp(c->n + 1);
+    if (hipMemcpyAsync(rp.data(), c->row_ptr, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return MEGA_ERR_HIP;
+    int64_t q = 0;
+    for (int o = 0; o < c->n; ++o) {
+        const int i = c->iperm[o];
+        for (int t = rp[i]; t < rp[i + 1]; ++t, ++q)
+            for (int k = 0; k < w; ++k) out[(size_t)w * q + k] = in[(size_t)w * t + k];
+    }
+    return MEGA_OK;
+}
+
+// internal-order copy of a caller-order host array (w values per node)
+template <class T>
+std::vector<T> to_internal(const mega_ctx *c, const T *h, int w)
+{
+    std::vector<T> o((size_t)c->n * w);
+    for (int i = 0; i < c->n; ++i)
+        for (int k = 0; k < w; ++k) o[(size_t)w * i + k] = h[(size_t)w * c->perm[i] + k];
+    return o;
+}
+
+// in place: h holds internal order on entry, the caller's on return
+template <class T>
+void to_caller(const mega_ctx *c, T *h, int w)
+{
+    std::vector<T> t(h, h + (size_t)c->n * w);
+    for (int i = 0; i < c->n; ++i)
+        for (int k = 0; k < w; ++k) h[(size_t)w * c->perm[i] + k] = t[(size_t)w * i + k];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1799,6 +1893,9 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     c->rsz = c->mixed ? 4 : 8;
     c->pd = ctrl->peridynamics ? 1 : 0;
     c->has_gravity = (ctrl->gravity[0] != 0.0 || ctrl->gravity[1] != 0.0 || ctrl->gravity[2] != 0.0);
+    c->perm = morton_order(n, X0, vol);
+    c->iperm.assign(n, 0);
+    for (int i = 0; i < n; ++i) c->iperm[c->perm[i]] = i;
     // materials: nodes with identical property rows share a slot
     std::vector<int32_t> mid(n);
     std::map<std::vector<double>, int> seen;
@@ -1854,6 +1951,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
     e |= dalloc(&c->cell_nodes, n);
+    e |= dalloc(&c->orig, n);
     c->nb_node = (int)grid_for(n);
     xcd_grid(n, &c->nb_force);
     e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
@@ -1865,11 +1963,15 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
     auto zero = [&](void *dst, size_t bytes) { return hipMemset(dst, 0, bytes) == hipSuccess; };
-    bool ok = up(c->X0, X0, n3 * 8) && up(c->x, X0, n3 * 8) && up(c->win, win, n3 * 8) &&
-              up(c->vol, vol, (size_t)n * 8) && up(c->mass, mass, n3 * 8) && up(c->ebc, ebc, n3 * 4) &&
-              up(c->ebc_node, ebc_node, (size_t)n * 4) && up(c->mat, mid.data(), (size_t)n * 4) &&
+    const auto X0i = to_internal(c, X0, 3), wini = to_internal(c, win, 3), voli = to_internal(c, vol, 1);
+    const auto massi = to_internal(c, mass, 3);
+    const auto ebci = to_internal(c, ebc, 3), ebni = to_internal(c, ebc_node, 1), midi = to_internal(c, mid.data(), 1);
+    bool ok = up(c->X0, X0i.data(), n3 * 8) && up(c->x, X0i.data(), n3 * 8) && up(c->win, wini.data(), n3 * 8) &&
+              up(c->vol, voli.data(), (size_t)n * 8) && up(c->mass, massi.data(), n3 * 8) &&
+              up(c->ebc, ebci.data(), n3 * 4) && up(c->ebc_node, ebni.data(), (size_t)n * 4) &&
+              up(c->mat, midi.data(), (size_t)n * 4) && up(c->orig, c->perm.data(), (size_t)n * 4) &&
               up(c->mats, c->host_mats.data(), c->host_mats.size() * sizeof(Material));
-    ok = ok && (vinit ? up(c->v, vinit, n3 * 8) : zero(c->v, n3 * 8));
+    ok = ok && (vinit ? up(c->v, to_internal(c, vinit, 3).data(), n3 * 8) : zero(c->v, n3 * 8));
     ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
          zero(c->eps, n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
@@ -1893,7 +1995,7 @@ int mega_destroy(mega_ctx *c)
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
-                    c->Kinv, c->gvec};
+                    c->Kinv, c->gvec, c->orig};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -1906,8 +2008,10 @@ int mega_search(mega_ctx *c, const double *x_cur, int32_t rebuild_bins)
 {
     if (!c) return MEGA_ERR_ARG;
     const double *xs = c->x;
+    std::vector<double> xi;
     if (x_cur) {
-        HIPCHK(hipMemcpyAsync(c->scratch, x_cur, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
+        xi = to_internal(c, x_cur, 3);
+        HIPCHK(hipMemcpyAsync(c->scratch, xi.data(), sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
         xs = c->scratch;
     }
     if (rebuild_bins || c->search_count == 0) {
@@ -1929,15 +2033,24 @@ int mega_get_neighbors(mega_ctx *c, int32_t *gn, int32_t *gstart, int32_t *gstac
     HIPCHK(hipMemcpyAsync(rp.data(), c->row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (nnz) *nnz = rp[n];
-    if (gn)
-        for (int i = 0; i < n; ++i) gn[i] = rp[i + 1] - rp[i];
-    if (gstart)
-        for (int i = 0; i < n; ++i) gstart[i] = rp[i] + 1;
+    // rows and neighbor ids in the caller's node order, 1-based
+    int64_t s = 0;
+    for (int o = 0; o < n; ++o) {
+        const int i = c->iperm[o], len = rp[i + 1] - rp[i];
+        if (gn) gn[o] = len;
+        if (gstart) gstart[o] = (int32_t)(s + 1);
+        s += len;
+    }
     if (gstack) {
         if (cap < rp[n]) return MEGA_ERR_ARG;
-        HIPCHK(hipMemcpyAsync(gstack, c->col, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost, c->stream));
+        std::vector<int32_t> col((size_t)rp[n]);
+        HIPCHK(hipMemcpyAsync(col.data(), c->col, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        for (int64_t q = 0; q < rp[n]; ++q) gstack[q] += 1;
+        int64_t q = 0;
+        for (int o = 0; o < n; ++o) {
+            const int i = c->iperm[o];
+            for (int t = rp[i]; t < rp[i + 1]; ++t) gstack[q++] = c->perm[col[t]] + 1;
+        }
     }
     return MEGA_OK;
 }
@@ -1953,8 +2066,10 @@ int mega_shapes(mega_ctx *c, const double *x_eval)
         return check_flags(c);
     }
     const double *xs = c->X0;
+    std::vector<double> xi;
     if (x_eval) {
-        HIPCHK(hipMemcpyAsync(c->scratch, x_eval, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
+        xi = to_internal(c, x_eval, 3);
+        HIPCHK(hipMemcpyAsync(c->scratch, xi.data(), sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
         xs = c->scratch;
     }
     int r = do_shapes(c, xs);
@@ -1977,18 +2092,24 @@ int mega_get_shapes(mega_ctx *c, double *shp, double *dshp, int64_t cap)
         if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)src, dst);
         else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)src, dst);
         if (part == 0) {
-            if (shp) HIPCHK(hipMemcpyAsync(shp, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+            std::vector<double> h((size_t)m);
+            if (shp) HIPCHK(hipMemcpyAsync(h.data(), buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
+            if (shp) {
+                const int r = rows_to_caller(c, h.data(), shp, 1);
+                if (r) { (void)hipFree(buf); return r; }
+            }
         }
     }
     int st = MEGA_OK;
     if (dshp) {
-        std::vector<double> h(3 * (size_t)m);
+        std::vector<double> h(3 * (size_t)m), g(3 * (size_t)m);
         if (hipMemcpyAsync(h.data(), buf, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             st = MEGA_ERR_HIP;
         for (int64_t q = 0; q < m && !st; ++q)
-            for (int k = 0; k < 3; ++k) dshp[3 * q + k] = h[(size_t)(k * m + q)];
+            for (int k = 0; k < 3; ++k) g[3 * q + k] = h[(size_t)(k * m + q)];
+        if (!st) st = rows_to_caller(c, g.data(), dshp, 3);
     }
     (void)hipFree(buf);
     return st;
@@ -2003,14 +2124,18 @@ int mega_get_pd_shape(mega_ctx *c, double *omega, double *invk, int64_t cap)
         if (dalloc(&buf, (size_t)m)) return MEGA_ERR_NOMEM;
         if (c->mixed) k_widen<float><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const float *)c->shp, buf);
         else k_widen<double><<<grid_for(m), kBlock, 0, c->stream>>>(m, (const double *)c->shp, buf);
-        const bool ok = hipMemcpyAsync(omega, buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+        std::vector<double> h((size_t)m);
+        const bool ok = hipMemcpyAsync(h.data(), buf, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
                         hipStreamSynchronize(c->stream) == hipSuccess;
         (void)hipFree(buf);
         if (!ok) return MEGA_ERR_HIP;
+        const int r = rows_to_caller(c, h.data(), omega, 1);
+        if (r) return r;
     }
     if (invk) {
         HIPCHK(hipMemcpyAsync(invk, c->Kinv, sizeof(double) * 9 * c->n, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        to_caller(c, invk, 9);
     }
     return MEGA_OK;
 }
@@ -2027,7 +2152,8 @@ int mega_interp(mega_ctx *c, const double *coef, double *phys)
     double *out = nullptr;
     if (dalloc(&out, 3 * (size_t)c->n)) return MEGA_ERR_NOMEM;
     int st = MEGA_OK;
-    if (hipMemcpyAsync(c->scratch, coef, b, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = MEGA_ERR_HIP;
+    const auto ci = to_internal(c, coef, 3);
+    if (hipMemcpyAsync(c->scratch, ci.data(), b, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = MEGA_ERR_HIP;
     if (!st) {
         Params p = params(c);
         if (c->mixed) k_interp<float, false><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->scratch, out);
@@ -2037,6 +2163,7 @@ int mega_interp(mega_ctx *c, const double *coef, double *phys)
             st = MEGA_ERR_HIP;
     }
     (void)hipFree(out);
+    if (!st) to_caller(c, phys, 3);
     return st;
 }
 
@@ -2047,15 +2174,22 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
         return MEGA_ERR_ARG;
     const int n = c->n;
     const size_t b3 = sizeof(double) * 3 * n;
-    std::vector<double> eps(n);
-    for (int i = 0; i < n; ++i) eps[i] = state[20 * (int64_t)i + 2];
-    HIPCHK(hipMemcpyAsync(c->d, dinc, b3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->dtot, dinc_tot, b3, hipMemcpyHostToDevice, c->stream));
-    if (x_cur) HIPCHK(hipMemcpyAsync(c->x, x_cur, b3, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> eps0(n);
+    for (int i = 0; i < n; ++i) eps0[i] = state[20 * (int64_t)i + 2];
+    const auto di = to_internal(c, dinc, 3), ti = to_internal(c, dinc_tot, 3);
+    const auto si = to_internal(c, stress, 6), ei = to_internal(c, strain, 6);
+    auto eps = to_internal(c, eps0.data(), 1);
+    std::vector<double> xi;
+    HIPCHK(hipMemcpyAsync(c->d, di.data(), b3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dtot, ti.data(), b3, hipMemcpyHostToDevice, c->stream));
+    if (x_cur) {
+        xi = to_internal(c, x_cur, 3);
+        HIPCHK(hipMemcpyAsync(c->x, xi.data(), b3, hipMemcpyHostToDevice, c->stream));
+    }
     int r = sync_d4(c);
-    if (!r) r = state_in(c, c->stress, 6 * (int64_t)n, stress);
+    if (!r) r = state_in(c, c->stress, 6 * (int64_t)n, si.data());
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (!r) r = state_in(c, c->strain, 6 * (int64_t)n, strain);
+    if (!r) r = state_in(c, c->strain, 6 * (int64_t)n, ei.data());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_in(c, c->eps, n, eps.data());
     if (r) return r;
@@ -2085,6 +2219,11 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     HIPCHK(hipMemcpyAsync(fint, c->fint, b3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(fext, c->fext, b3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    to_caller(c, stress, 6);
+    to_caller(c, strain, 6);
+    to_caller(c, eps.data(), 1);
+    to_caller(c, fint, 3);
+    to_caller(c, fext, 3);
     for (int i = 0; i < n; ++i) state[20 * (int64_t)i + 2] = eps[i];
     if (int_work) *int_work += w;
     c->predicted = 0;
@@ -2184,6 +2323,15 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
     return check_flags(c);
 }
 
+static int field_width(int f)
+{
+    switch (f) {
+    case MEGA_F_STRESS: case MEGA_F_STRAIN: return 6;
+    case MEGA_F_EPS: case MEGA_F_STRAIN_EQ: case MEGA_F_WAVE_VEL: return 1;
+    default: return 3;
+    }
+}
+
 static double *field_ptr(mega_ctx *c, int f, int *width)
 {
     *width = 3;
@@ -2236,14 +2384,17 @@ int mega_download(mega_ctx *c, int32_t field, double *host)
     }
     if (r) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
+    to_caller(c, host, field_width(field));
     return MEGA_OK;
 }
 
-int mega_upload(mega_ctx *c, int32_t field, const double *host)
+int mega_upload(mega_ctx *c, int32_t field, const double *host_caller)
 {
-    if (!c || !host) return MEGA_ERR_ARG;
+    if (!c || !host_caller || field < 0 || field >= MEGA_F_NFIELDS) return MEGA_ERR_ARG;
     const int n = c->n;
     int r = MEGA_OK;
+    const auto hi = to_internal(c, host_caller, field_width(field));
+    const double *host = hi.data();
     switch (field) {
     case MEGA_F_STRESS: r = state_in(c, c->stress, 6 * (int64_t)n, host); break;
     case MEGA_F_STRAIN: r = state_in(c, c->strain, 6 * (int64_t)n, host); break;
