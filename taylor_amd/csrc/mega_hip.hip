@@ -21,7 +21,7 @@
 //   k_predict  predictor when it is not fused into the previous k_force
 //   k_bin_*    GET_BINS as a counting sort     BIN_SEARCH.F90:2-64
 //   k_search   SOFT_SEARCH (+BIN_INODE)        BIN_SEARCH.F90:69-133,153-328
-//   k_t*       transposed neighbor lists (built once per search)
+//              and, in the same two passes, the transposed neighbor lists
 //   k_rk       RK1 shape functions/gradients   RK.f90:16-453, KERNEL.F90:3-140
 //
 // Precision: R = double (MEGA_FP64) or float (MEGA_MIXED) for shape data,
@@ -32,7 +32,8 @@
 // Layout in HBM: node vectors interleaved (3*i+k); stress/strain 6 per node
 // (Voigt XX,YY,ZZ,YZ,XZ,XY); neighbor lists as 0-based CSR (row_ptr int32[n+1],
 // col int32[nnz]) with phi[nnz], grad phi[3 nnz] in list order; the transposed
-// lists (trow[n+1], tsrc[nnz]) carry grad phi[3 nnz] in transposed order.
+// lists (trow[n+1], tsrc[nnz], grouped by the source's bin) carry grad phi[3 nnz]
+// in transposed order.
 // Pair kernels map blocks to XCDs in contiguous node ranges so that each
 // XCD's L2 holds the neighbor data its blocks gather.
 #include <hip/hip_runtime.h>
@@ -1115,69 +1116,6 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
 }
 
 // ---------------------------------------------------------------------------
-// transposed lists: trow[j] .. trow[j+1] hold the pairs (i -> j) in increasing
-// source i; tpos maps a pair's list position to its transposed position.
-
-__global__ void k_tcount(int64_t nnz, const int32_t *col, int32_t *cnt)
-{
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nnz) atomicAdd(&cnt[col[q]], 1);
-}
-
-// kGroup lanes per list row append the row's entries to the transposed rows
-__global__ __launch_bounds__(kBlock) void k_tfill(int n, const int32_t *row_ptr, const int32_t *col, int32_t *cursor,
-                                                  int32_t *tsrc, int32_t *tq)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(t / kGroup), sub = (int)(t % kGroup);
-    if (i >= n) return;
-    for (int q = row_ptr[i] + sub; q < row_ptr[i + 1]; q += kGroup) {
-        const int k = atomicAdd(&cursor[col[q]], 1);
-        tsrc[k] = i;
-        tq[k] = q;
-    }
-}
-
-// Order each transposed row by source node (the order of the reference's own
-// scatter over i, FINT.F90:997-1116): one wave per row ranks its entries in
-// LDS; sources within a row are distinct, so an entry's rank is its slot.
-constexpr int kTsortMax = 256;
-
-__global__ __launch_bounds__(kBlock) void k_tsort(int n, const int32_t *trow, int32_t *tsrc, int32_t *tq,
-                                                  int32_t *tpos)
-{
-    __shared__ int2 buf[kBlock / 64][kTsortMax];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (j >= n) return;   // whole waves leave together
-    const int s = trow[j], L = trow[j + 1] - s;
-    if (L > kTsortMax) {  // long row: one lane, insertion sort in place
-        if (lane == 0) {
-            for (int a = s + 1; a < s + L; ++a) {
-                const int vs = tsrc[a], vq = tq[a];
-                int b = a - 1;
-                while (b >= s && tsrc[b] > vs) { tsrc[b + 1] = tsrc[b]; tq[b + 1] = tq[b]; --b; }
-                tsrc[b + 1] = vs;
-                tq[b + 1] = vq;
-            }
-            for (int a = s; a < s + L; ++a) tpos[tq[a]] = a;
-        }
-        return;
-    }
-    int2 *row = buf[w];
-    for (int a = lane; a < L; a += 64) row[a] = make_int2(tsrc[s + a], tq[s + a]);
-    wave_sync();
-    for (int a = lane; a < L; a += 64) {
-        const int2 v = row[a];
-        int r = 0;
-        for (int b = 0; b < L; ++b) r += row[b].x < v.x;
-        tsrc[s + r] = v.x;
-        tq[s + r] = v.y;
-        tpos[v.y] = s + r;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // bins: cell of each node (GET_BINS floor/clamp, 1-based), counting sort of
 // nodes into cells, node order restored inside each cell.
 
@@ -1227,29 +1165,48 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
     }
 }
 
-// SOFT_SEARCH: one wavefront per node.  The 27 surrounding cells are visited in
-// the reference's order (ix slowest, then iy, iz; BIN_SEARCH.F90:105-127), each
-// cell's nodes in node order; 64 candidates are tested at a time and the kept
-// ones are compacted with a ballot, so the list comes out in the reference's
-// order.  Criterion: x_i inside node j's support, |x_j - x_i|_k / a_jk < 1
-// (box) or the ellipsoidal norm < 1 (BIN_SEARCH.F90:255-291).
+// SOFT_SEARCH: one wavefront per non-empty bin.  All nodes of a bin share the
+// same candidates -- the nodes of the 27 surrounding bins, visited in the
+// reference's order (ix slowest, then iy, iz; BIN_SEARCH.F90:105-127), each
+// bin's nodes in node order -- so the wave loads them into registers, 256 at a
+// time (4 per lane), and tests every node of its bin against them; kept
+// candidates are compacted with a ballot, so each list comes out in the
+// reference's order.  Criterion: x_i inside node j's support,
+// |x_j - x_i|_k / a_jk < 1 (box) or the ellipsoidal norm < 1
+// (BIN_SEARCH.F90:255-291).  Every node belongs to exactly one bin, so one wave
+// owns its list.
+//
+// The same pass builds the transposed lists.  A candidate j stays in one lane
+// while the wave runs through its own nodes, so that lane counts how many of
+// them keep j.  The transposed row of j is grouped by the source's bin, in the
+// order of the 27 bins around j's bin (bin B is entry 26 - c there when j's bin
+// is entry c around B), each group in bin order.  tbin[27 j + e] is that
+// group's size after the COUNT pass and its first slot before the FILL pass;
+// FILL writes col from row_ptr[i] on, tsrc (source of each transposed slot)
+// and tpos (transposed slot of each list entry).
+constexpr int kCandPerLane = 4;
+
 template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_search(int n, const double *x, const double *win, const int32_t *ijk,
-                                                   BinGeom g, const int32_t *cell_start, const int32_t *cell_nodes,
-                                                   int shsup, int32_t *counts, const int32_t *row_ptr, int32_t *col)
+__global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *cells, const double *x,
+                                                   const double *win, BinGeom g, const int32_t *cell_start,
+                                                   const int32_t *cell_nodes, int shsup, int32_t *counts,
+                                                   const int32_t *row_ptr, int32_t *col, int32_t *tbin,
+                                                   int32_t *tsrc, int32_t *tpos)
 {
     __shared__ int s_beg[kBlock / 64][28], s_off[kBlock / 64][28];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (i >= n) return;
-    const int ci = ijk[3 * i], cj = ijk[3 * i + 1], ck = ijk[3 * i + 2];
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (w >= ncells) return;   // whole waves leave together
+    const int64_t m0 = cells[w];
+    const int nxy = g.nb[0] * g.nb[1];
+    const int ck = (int)(m0 / nxy) + 1, cj = (int)((m0 % nxy) / g.nb[0]) + 1, ci = (int)(m0 % g.nb[0]) + 1;
     int cnt = 0;
     if (lane < 27) {
         const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
         const int ix = ci + dx, iy = cj + dy, iz = ck + dz;
         int b = 0;
         if (ix >= 1 && ix <= g.nb[0] && iy >= 1 && iy <= g.nb[1] && iz >= 1 && iz <= g.nb[2]) {
-            const int64_t m = (int64_t)(iz - 1) * g.nb[0] * g.nb[1] + (int64_t)(iy - 1) * g.nb[0] + (ix - 1);
+            const int64_t m = (int64_t)(iz - 1) * nxy + (int64_t)(iy - 1) * g.nb[0] + (ix - 1);
             b = cell_start[m];
             cnt = cell_start[m + 1] - b;
         }
@@ -1265,33 +1222,108 @@ __global__ __launch_bounds__(kBlock) void k_search(int n, const double *x, const
     const int total = __shfl(incl, 26, 64);
     if (lane == 0) s_off[wv][27] = total;
     __builtin_amdgcn_wave_barrier();
-    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-    int out = FILL ? row_ptr[i] : 0;
-    int kept = 0;
-    for (int base = 0; base < total; base += 64) {
-        const int c = base + lane;
-        bool add = false;
-        int j = 0;
-        if (c < total) {
-            int cell = 0;
-            while (cell < 26 && s_off[wv][cell + 1] <= c) ++cell;
-            j = cell_nodes[s_beg[wv][cell] + (c - s_off[wv][cell])];
-            const double ax = fabs(x[3 * j] - xi) / win[3 * j];
-            const double ay = fabs(x[3 * j + 1] - yi) / win[3 * j + 1];
-            const double az = fabs(x[3 * j + 2] - zi) / win[3 * j + 2];
-            if (shsup) add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
-            else add = (ax < 1.0) && (ay < 1.0) && (az < 1.0);
+    const int own0 = cell_start[m0], nown = cell_start[m0 + 1] - own0;
+    for (int t0 = 0; t0 < nown; t0 += 64) {   // own nodes, 64 at a time: lane t keeps node t0+t's offset
+        const int tn = min(64, nown - t0);
+        const bool more = t0 + 64 < nown;
+        int myi = 0, mine = 0;
+        if (lane < tn) {
+            myi = cell_nodes[own0 + t0 + lane];
+            mine = FILL ? row_ptr[myi] : 0;
         }
-        const unsigned long long mask = __ballot(add);
-        if (FILL && add) {
-            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-            col[out + pos] = j;
+        for (int base = 0; base < total; base += 64 * kCandPerLane) {
+            int cj_[kCandPerLane], tix[kCandPerLane], kc[kCandPerLane];
+            double cx[kCandPerLane], cy[kCandPerLane], cz[kCandPerLane];
+            double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
+#pragma unroll
+            for (int r = 0; r < kCandPerLane; ++r) {
+                const int c = base + 64 * r + lane;
+                int j = -1, cell = 0;
+                if (c < total) {
+                    while (cell < 26 && s_off[wv][cell + 1] <= c) ++cell;
+                    j = cell_nodes[s_beg[wv][cell] + (c - s_off[wv][cell])];
+                }
+                cj_[r] = j;
+                const int jj = j < 0 ? 0 : j;
+                tix[r] = 27 * jj + (26 - cell);
+                // this bin's running count (COUNT) or next slot (FILL) in j's transposed row;
+                // an earlier chunk of own nodes left it in tbin, written by this same lane
+                kc[r] = (j >= 0 && (FILL || t0 > 0)) ? tbin[tix[r]] : 0;
+                cx[r] = x[3 * jj]; cy[r] = x[3 * jj + 1]; cz[r] = x[3 * jj + 2];
+                wx[r] = win[3 * jj]; wy[r] = win[3 * jj + 1]; wz[r] = win[3 * jj + 2];
+            }
+            for (int t = 0; t < tn; ++t) {
+                const int i = __shfl(myi, t, 64);
+                int out = __shfl(mine, t, 64);
+                const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+#pragma unroll
+                for (int r = 0; r < kCandPerLane; ++r) {
+                    if (base + 64 * r >= total) break;   // uniform
+                    bool add = false;
+                    if (cj_[r] >= 0) {
+                        const double ax = fabs(cx[r] - xi) / wx[r];
+                        const double ay = fabs(cy[r] - yi) / wy[r];
+                        const double az = fabs(cz[r] - zi) / wz[r];
+                        if (shsup) add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
+                        else add = (ax < 1.0) && (ay < 1.0) && (az < 1.0);
+                    }
+                    const unsigned long long mask = __ballot(add);
+                    if (FILL && add) {
+                        const int q = out + __popcll(mask & ((1ull << lane) - 1ull));
+                        col[q] = cj_[r];
+                        tpos[q] = kc[r];
+                        tsrc[kc[r]] = i;
+                    }
+                    kc[r] += add ? 1 : 0;
+                    out += __popcll(mask);
+                }
+                if (lane == t) mine = out;
+            }
+#pragma unroll
+            for (int r = 0; r < kCandPerLane; ++r)
+                if (cj_[r] >= 0 && (!FILL || more)) tbin[tix[r]] = kc[r];
         }
-        const int nadd = __popcll(mask);
-        out += nadd;
-        kept += nadd;
+        if (!FILL && lane < tn) counts[myi] = mine;
     }
-    if (!FILL && lane == 0) counts[i] = kept;
+}
+
+// transposed row lengths: sum of the 27 per-bin group sizes
+__global__ void k_trow_len(int n, const int32_t *tbin, int32_t *len)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) return;
+    int s = 0;
+    if (j < n)
+        for (int e = 0; e < 27; ++e) s += tbin[27 * (int64_t)j + e];
+    len[j] = s;
+}
+
+// group sizes -> first slot of each group
+__global__ void k_tbin_start(int n, const int32_t *trow, int32_t *tbin)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int run = trow[j];
+    for (int e = 0; e < 27; ++e) {
+        const int v = tbin[27 * (int64_t)j + e];
+        tbin[27 * (int64_t)j + e] = run;
+        run += v;
+    }
+}
+
+// non-empty bins, in bin order
+__global__ void k_cells_ne(int64_t ncell, const int32_t *cell_start, const int32_t *pos, int32_t *cells)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ncell) return;
+    if (cell_start[m + 1] > cell_start[m]) cells[pos[m]] = (int32_t)m;
+}
+
+__global__ void k_cell_flag(int64_t ncell, const int32_t *cell_start, int32_t *flag)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > ncell) return;
+    flag[m] = m < ncell ? (cell_start[m + 1] > cell_start[m] ? 1 : 0) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1363,13 +1395,16 @@ struct mega_ctx {
     void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr;
     void *shp = nullptr, *dshp = nullptr, *tdshp = nullptr, *tshp = nullptr;
     // lists
-    int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tq = nullptr, *tpos = nullptr;
+    int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr;
+    int32_t *tbin = nullptr;   // [27 n] transposed-row group sizes / first slots
     int32_t *counts = nullptr;
     int64_t nnz = 0, nnz_cap = 0;
     // bins
     BinGeom geom{};
     int64_t ncell = 0;
     int32_t *ijk = nullptr, *cell_start = nullptr, *cell_nodes = nullptr, *cell_cursor = nullptr;
+    int32_t *cells_ne = nullptr, *cell_flag = nullptr;   // non-empty bins, in bin order
+    int ncell_ne = 0;
     // misc
     double *wpart = nullptr, *epart = nullptr;      // per-block work / energy partial sums
     int nb_node = 0, nb_force = 0;
@@ -1486,10 +1521,10 @@ unsigned xcd_grid(int n, int *nblocks)
 
 void free_pairs(mega_ctx *c)
 {
-    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tq, c->tpos};
+    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tpos};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    c->col = c->tsrc = c->tq = c->tpos = nullptr;
+    c->col = c->tsrc = c->tpos = nullptr;
     c->shp = c->dshp = c->tdshp = c->tshp = nullptr;
     c->nnz_cap = 0;
 }
@@ -1502,7 +1537,7 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
     const int64_t cap = nnz + nnz / 8 + 1024;
     const size_t r = c->rsz;
     int e = 0;
-    e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tq, cap); e |= dalloc(&c->tpos, cap);
+    e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tpos, cap);
     e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r); e |= dalloc(&c->tdshp, 3 * cap, r);
     if (c->has_gravity || c->pd) e |= dalloc(&c->tshp, cap, r);
     if (e) { free_pairs(c); return MEGA_ERR_NOMEM; }
@@ -1537,21 +1572,14 @@ int do_bins(mega_ctx *c, const double *x)
     HIPCHK(hipMemcpyAsync(c->cell_cursor, c->cell_start, sizeof(int32_t) * c->ncell, hipMemcpyDeviceToDevice, c->stream));
     k_bin_fill<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->ijk, c->geom, c->cell_cursor, c->cell_nodes);
     k_bin_sort<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->orig, c->cell_nodes);
-    HIPCHK(hipGetLastError());
-    return MEGA_OK;
-}
-
-int do_transpose(mega_ctx *c)
-{
-    const int n = c->n;
-    Scope s(c, "k_transpose");
-    HIPCHK(hipMemsetAsync(c->counts, 0, sizeof(int32_t) * (n + 1), c->stream));
-    k_tcount<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->col, c->counts);
-    int r = cub_scan(c, c->counts, c->trow, n + 1);
+    k_cell_flag<<<grid_for(c->ncell + 1), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->cell_flag);
+    r = cub_scan(c, c->cell_flag, c->cell_cursor, c->ncell + 1);   // positions of the non-empty bins
     if (r) return r;
-    HIPCHK(hipMemcpyAsync(c->counts, c->trow, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
-    k_tfill<<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(n, c->row_ptr, c->col, c->counts, c->tsrc, c->tq);
-    k_tsort<<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(n, c->trow, c->tsrc, c->tq, c->tpos);
+    k_cells_ne<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->cell_cursor, c->cells_ne);
+    int32_t nne = 0;
+    HIPCHK(hipMemcpyAsync(&nne, c->cell_cursor + c->ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ncell_ne = nne;
     HIPCHK(hipGetLastError());
     return MEGA_OK;
 }
@@ -1559,10 +1587,12 @@ int do_transpose(mega_ctx *c)
 int do_search(mega_ctx *c, const double *x)
 {
     const int n = c->n;
+    HIPCHK(hipMemsetAsync(c->tbin, 0, sizeof(int32_t) * 27 * (size_t)n, c->stream));
     {
         Scope s(c, "k_search_count");
-        k_search<false><<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(
-            n, x, c->win, c->ijk, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts, nullptr, nullptr);
+        k_search<false><<<grid_for((int64_t)c->ncell_ne * 64), kBlock, 0, c->stream>>>(
+            c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts,
+            nullptr, nullptr, c->tbin, nullptr, nullptr);
     }
     HIPCHK(hipMemsetAsync(c->counts + n, 0, sizeof(int32_t), c->stream));
     int r = cub_scan(c, c->counts, c->row_ptr, n + 1);
@@ -1575,13 +1605,19 @@ int do_search(mega_ctx *c, const double *x)
     if (r) return r;
     c->nnz = nnz32;
     {
+        Scope s(c, "k_transpose");
+        k_trow_len<<<grid_for(n + 1), kBlock, 0, c->stream>>>(n, c->tbin, c->counts);
+        r = cub_scan(c, c->counts, c->trow, n + 1);
+        if (r) return r;
+        k_tbin_start<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->trow, c->tbin);
+    }
+    {
         Scope s(c, "k_search_fill");
-        k_search<true><<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(
-            n, x, c->win, c->ijk, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr, c->row_ptr, c->col);
+        k_search<true><<<grid_for((int64_t)c->ncell_ne * 64), kBlock, 0, c->stream>>>(
+            c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr,
+            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos);
     }
     HIPCHK(hipGetLastError());
-    r = do_transpose(c);
-    if (r) return r;
     c->has_lists = 1;
     c->has_shapes = 0;
     return MEGA_OK;
@@ -1952,6 +1988,9 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
     e |= dalloc(&c->cell_nodes, n);
     e |= dalloc(&c->orig, n);
+    e |= dalloc(&c->cells_ne, std::min<int64_t>(c->ncell, n));
+    e |= dalloc(&c->tbin, 27 * (int64_t)n);
+    e |= dalloc(&c->cell_flag, c->ncell + 1);
     c->nb_node = (int)grid_for(n);
     xcd_grid(n, &c->nb_force);
     e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
@@ -1995,7 +2034,7 @@ int mega_destroy(mega_ctx *c)
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
-                    c->Kinv, c->gvec, c->orig};
+                    c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }

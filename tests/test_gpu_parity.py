@@ -48,8 +48,13 @@ def test_constitute_matches_oracle(mega, oracle):
         assert rel_err(state[:, 2], st_ref[:, 2]) < 1e-11
 
 
-def test_search_lists_equal_reference_order(mega, oracle):
-    prob = small_problem()
+@pytest.mark.parametrize("norm_win,shsup", [(1.84, 0), (5.0, 0), (5.0, 1)])
+def test_search_lists_equal_reference_order(mega, oracle, norm_win, shsup):
+    """norm_win 5 makes bins of ~100 nodes with ~800 candidates: several
+    candidate chunks and own-node chunks per bin; shsup=1 is the ellipsoidal
+    support test."""
+    prob = small_problem(norm_win=norm_win)
+    prob.shsup = shsup
     g = mega.Mega(prob)
     g.search()
     gn, gs, st = g.neighbors()
