@@ -60,6 +60,13 @@ namespace {
 #ifndef MEGA_KGROUP
 #define MEGA_KGROUP 16
 #endif
+#ifndef MEGA_RK_GROUP
+#define MEGA_RK_GROUP 16
+#endif
+#ifndef MEGA_RK_CACHE
+#define MEGA_RK_CACHE 4
+#endif
+static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one neighbor per lane");
 #ifndef MEGA_UNROLL_MIX
 #define MEGA_UNROLL_MIX 6
 #endif
@@ -68,6 +75,8 @@ namespace {
 #endif
 constexpr int kBlock = 256;
 constexpr int kGroup = MEGA_KGROUP;   // lanes cooperating on one node's neighbor list
+constexpr int kRkGroup = MEGA_RK_GROUP;   // the same for the shape-function kernel
+constexpr int kRkCache = MEGA_RK_CACHE;   // kernel evaluations per lane kept between its two passes
 constexpr int kXcd = 8;               // XCDs on MI355X
 // neighbor-list entries each lane keeps in flight in the gathers
 constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
@@ -123,11 +132,39 @@ enum FMode { F_NONE = 0, F_DIRECT = 1, F_INCR = 2 };
 // ---------------------------------------------------------------------------
 // helpers
 
+// Sum over aligned groups of W lanes (W <= 16: groups inside one DPP row).
+// Partners per step: lane ^ 1, lane ^ 2 (quad_perm), then the mirrored lane in
+// the half row and in the row.  After the first two steps all lanes of a quad
+// hold the same value, after the third all lanes of a half row, so every step
+// adds the same two values in every lane and all lanes end with the same bits.
+namespace dpp {
+constexpr int kXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kHalfMirror = 0x141; // row_half_mirror
+constexpr int kMirror = 0x140;     // row_mirror
+template <int CTRL>
+__device__ __forceinline__ float mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double mov(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+}  // namespace dpp
+
 template <int W, class T>
 __device__ __forceinline__ T group_sum(T v)
 {
-#pragma unroll
-    for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16, "group of at most one DPP row");
+    if (W >= 2) v += dpp::mov<dpp::kXor1>(v);
+    if (W >= 4) v += dpp::mov<dpp::kXor2>(v);
+    if (W >= 8) v += dpp::mov<dpp::kHalfMirror>(v);
+    if (W >= 16) v += dpp::mov<dpp::kMirror>(v);
     return v;
 }
 
@@ -262,10 +299,12 @@ struct KernelEval {
     double phi, px, py, pz;
 };
 
-__device__ __forceinline__ KernelEval rk_window(const Params &p, const double *xi, const double *xs, int j)
+// window of neighbor j (position xj, window w, essential-BC flag ebc) at x_i
+__device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double *xi, const double *xj,
+                                                   const double *w, int ebc)
 {
-    const double dx = xi[0] - xs[3 * j], dy = xi[1] - xs[3 * j + 1], dz = xi[2] - xs[3 * j + 2];
-    const double w1 = p.win[3 * j], w2 = p.win[3 * j + 1], w3 = p.win[3 * j + 2];
+    const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+    const double w1 = w[0], w2 = w[1], w3 = w[2];
     const double zx = dx / w1, zy = dy / w2, zz = dz / w3;
     KernelEval e;
     if (p.shsup) {
@@ -286,7 +325,7 @@ __device__ __forceinline__ KernelEval rk_window(const Params &p, const double *x
         const double rz = zz == 0.0 ? 0.0 : (zz >= 0.0 ? iw : -iw);
         double ph = fx * fy * fz;
         double px = fxx * fy * fz * rx, py = fx * fyy * fz * ry, pz = fx * fy * fzz * rz;
-        if (p.ebc_node[j]) {   // singular kernel at essential-BC nodes (RK.f90:217-228)
+        if (ebc) {   // singular kernel at essential-BC nodes (RK.f90:217-228)
             const double wm = 1.0 / (sqrt(zx * zx + zy * zy + zz * zz) + mega::kSingularEps);
             const double pw3 = ph * wm * wm * wm;
             px = px * wm - pw3 * zx / w1;
@@ -299,37 +338,89 @@ __device__ __forceinline__ KernelEval rk_window(const Params &p, const double *x
     return e;
 }
 
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
+__device__ __forceinline__ KernelEval rk_window(const Params &p, const double *xi, const double *xs, int j)
 {
-    const int node = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup;
-    const int lane = threadIdx.x & (kGroup - 1);
-    if (node >= p.n) return;
+    const double xj[3] = {xs[3 * j], xs[3 * j + 1], xs[3 * j + 2]};
+    const double w[3] = {p.win[3 * j], p.win[3 * j + 1], p.win[3 * j + 2]};
+    return rk_window_at(p, xi, xj, w, p.ebc_node[j]);
+}
+
+// one neighbor's terms of the moment matrix M (m[0..9]), its derivatives M_a
+// (m[10a..10a+9]) and sum phi (m[40]); H = (1, x_i - x_j).
+//   M_a(r,c) += (dH_r H_c + H_r dH_c) phi + H_r H_c phi_a, dH the unit vector e_a;
+// the product terms of dH vanish except on row or column a and are left out
+// there (their zero addends do not change the sum).
+__device__ __forceinline__ void rk_moments(double *m, const double *H, const KernelEval &e)
+{
+    int t = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = r; c < 4; ++c, ++t) {
+            const double hh = H[r] * H[c];
+            m[t] += hh * e.phi;
+            const double pa[3] = {e.px, e.py, e.pz};
+#pragma unroll
+            for (int a = 1; a <= 3; ++a) {
+                double d;
+                if (r == a && c == a) d = (H[c] * e.phi + H[r] * e.phi) + hh * pa[a - 1];
+                else if (r == a) d = H[c] * e.phi + hh * pa[a - 1];
+                else if (c == a) d = H[r] * e.phi + hh * pa[a - 1];
+                else d = hh * pa[a - 1];
+                m[10 * a + t] += d;
+            }
+        }
+    m[40] += e.phi;
+}
+
+// Nodes are taken in bin order (order = the bins' node lists), blocks mapped to
+// XCDs in contiguous ranges, kRkGroup lanes per node.  A lane's first kRkCache
+// neighbors are loaded together, and their kernel values and H are kept in
+// registers for the second pass; longer rows continue in a plain loop.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
+{
+    constexpr int G = kRkGroup, U = kRkCache;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int g = blk * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (g >= p.n) return;
+    const int node = order ? order[g] : g;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
     const double xi[3] = {xs[3 * node], xs[3 * node + 1], xs[3 * node + 2]};
     double m[41];   // 10 unique entries each of M, Mx, My, Mz + sum of phi
 #pragma unroll
     for (int k = 0; k < 41; ++k) m[k] = 0.0;
-    for (int q = p0 + lane; q < p1; q += kGroup) {
+    int jv[U], tp[U];
+    double c_ph[U], c_px[U], c_py[U], c_pz[U], c_h[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = p0 + lane + u * G;
+        jv[u] = q < p1 ? p.col[q] : -1;
+        tp[u] = (q < p1 && p.tpos) ? p.tpos[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int jj = jv[u] < 0 ? node : jv[u];
+        const double xj[3] = {xs[3 * jj], xs[3 * jj + 1], xs[3 * jj + 2]};
+        const double w[3] = {p.win[3 * jj], p.win[3 * jj + 1], p.win[3 * jj + 2]};
+        const KernelEval e = rk_window_at(p, xi, xj, w, p.ebc_node[jj]);
+        c_ph[u] = e.phi; c_px[u] = e.px; c_py[u] = e.py; c_pz[u] = e.pz;
+        c_h[u][0] = xi[0] - xj[0]; c_h[u][1] = xi[1] - xj[1]; c_h[u][2] = xi[2] - xj[2];
+        if (jv[u] >= 0) {
+            const double H[4] = {1.0, c_h[u][0], c_h[u][1], c_h[u][2]};
+            rk_moments(m, H, e);
+        }
+    }
+    for (int q = p0 + lane + U * G; q < p1; q += G) {
         const int j = p.col[q];
         const KernelEval e = rk_window(p, xi, xs, j);
         const double H[4] = {1.0, xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
-        int t = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = r; c < 4; ++c, ++t) {
-                const double hh = H[r] * H[c];
-                m[t] += hh * e.phi;
-                // M_a(r,c) = (dH_r H_c + H_r dH_c) phi + H_r H_c phi_a with dH the unit vector e_a
-                m[10 + t] += (double)(r == 1) * H[c] * e.phi + (double)(c == 1) * H[r] * e.phi + hh * e.px;
-                m[20 + t] += (double)(r == 2) * H[c] * e.phi + (double)(c == 2) * H[r] * e.phi + hh * e.py;
-                m[30 + t] += (double)(r == 3) * H[c] * e.phi + (double)(c == 3) * H[r] * e.phi + hh * e.pz;
-            }
-        m[40] += e.phi;
+        rk_moments(m, H, e);
     }
 #pragma unroll
-    for (int k = 0; k < 41; ++k) m[k] = group_sum<kGroup>(m[k]);
+    for (int k = 0; k < 41; ++k) m[k] = group_sum<kRkGroup>(m[k]);
     double hs[4] = {0, 0, 0, 0};
     if (p.ql) {   // quasi-linear augmentation (RK.f90:251-303, 357-377)
         const double L = p.ql_len;
@@ -372,10 +463,9 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
             Bd[al][c] = -(u[0] * Mi[4 * c] + u[1] * Mi[1 + 4 * c] + u[2] * Mi[2 + 4 * c] + u[3] * Mi[3 + 4 * c]);
     }
     R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp), *tdshp = as<R>(p.tdshp), *tshp = as<R>(p.tshp);
-    for (int q = p0 + lane; q < p1; q += kGroup) {
-        const int j = p.col[q];
-        const KernelEval e = rk_window(p, xi, xs, j);
-        double H[4] = {1.0, xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
+    // phi_j = B.H phi_w, grad phi_j = (Bd_a.H + B_a) phi_w + (B.H) grad phi_w
+    auto emit = [&](int q, int t, const double *h, const KernelEval &e) {
+        double H[4] = {1.0, h[0], h[1], h[2]};
         if (p.ql)
 #pragma unroll
             for (int k = 0; k < 4; ++k) H[k] = H[k] + hs[k] * p.ql_coef;
@@ -388,10 +478,22 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs)
         shp[q] = f;
         dshp[q] = gx; dshp[p.cap + q] = gy; dshp[2 * p.cap + q] = gz;
         if (p.tpos) {
-            const int t = p.tpos[q];
             tdshp[t] = gx; tdshp[p.cap + t] = gy; tdshp[2 * p.cap + t] = gz;
             if (tshp) tshp[t] = f;
         }
+    };
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (jv[u] >= 0) {
+            KernelEval e;
+            e.phi = c_ph[u]; e.px = c_px[u]; e.py = c_py[u]; e.pz = c_pz[u];
+            emit(p0 + lane + u * G, tp[u], c_h[u], e);
+        }
+    for (int q = p0 + lane + U * G; q < p1; q += G) {
+        const int j = p.col[q];
+        const KernelEval e = rk_window(p, xi, xs, j);
+        const double h[3] = {xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
+        emit(q, p.tpos ? p.tpos[q] : 0, h, e);
     }
 }
 
@@ -1195,7 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
 {
     __shared__ int s_beg[kBlock / 64][28], s_off[kBlock / 64][28];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int w = (xcd_block() * blockDim.x + threadIdx.x) >> 6;   // neighboring bins on one XCD
     if (w >= ncells) return;   // whole waves leave together
     const int64_t m0 = cells[w];
     const int nxy = g.nb[0] * g.nb[1];
@@ -1261,11 +1363,14 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                     if (base + 64 * r >= total) break;   // uniform
                     bool add = false;
                     if (cj_[r] >= 0) {
-                        const double ax = fabs(cx[r] - xi) / wx[r];
-                        const double ay = fabs(cy[r] - yi) / wy[r];
-                        const double az = fabs(cz[r] - zi) / wz[r];
-                        if (shsup) add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
-                        else add = (ax < 1.0) && (ay < 1.0) && (az < 1.0);
+                        const double dx = fabs(cx[r] - xi), dy = fabs(cy[r] - yi), dz = fabs(cz[r] - zi);
+                        if (shsup) {
+                            const double ax = dx / wx[r], ay = dy / wy[r], az = dz / wz[r];
+                            add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
+                        } else {   // the box test, exactly as the quotients decide it
+                            add = mega::quotient_below_one(dx, wx[r]) & mega::quotient_below_one(dy, wy[r]) &
+                                  mega::quotient_below_one(dz, wz[r]);
+                        }
                     }
                     const unsigned long long mask = __ballot(add);
                     if (FILL && add) {
@@ -1584,13 +1689,20 @@ int do_bins(mega_ctx *c, const double *x)
     return MEGA_OK;
 }
 
+// one wave per non-empty bin, padded to a multiple of kXcd blocks
+unsigned search_grid(const mega_ctx *c)
+{
+    const unsigned nb = grid_for((int64_t)c->ncell_ne * 64);
+    return (nb + kXcd - 1) / kXcd * kXcd;
+}
+
 int do_search(mega_ctx *c, const double *x)
 {
     const int n = c->n;
     HIPCHK(hipMemsetAsync(c->tbin, 0, sizeof(int32_t) * 27 * (size_t)n, c->stream));
     {
         Scope s(c, "k_search_count");
-        k_search<false><<<grid_for((int64_t)c->ncell_ne * 64), kBlock, 0, c->stream>>>(
+        k_search<false><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts,
             nullptr, nullptr, c->tbin, nullptr, nullptr);
     }
@@ -1613,7 +1725,7 @@ int do_search(mega_ctx *c, const double *x)
     }
     {
         Scope s(c, "k_search_fill");
-        k_search<true><<<grid_for((int64_t)c->ncell_ne * 64), kBlock, 0, c->stream>>>(
+        k_search<true><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr,
             c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos);
     }
@@ -1642,8 +1754,10 @@ int do_shapes(mega_ctx *c, const double *xs)
     if (!c->has_lists) return MEGA_ERR_ARG;
     Params p = params(c);
     Scope s(c, "k_rk");
-    if (c->mixed) k_rk<float><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, xs);
-    else k_rk<double><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, xs);
+    const int nb = (int)(((int64_t)c->n * kRkGroup + kBlock - 1) / kBlock);
+    const unsigned grid = (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
+    if (c->mixed) k_rk<float><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+    else k_rk<double><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
     HIPCHK(hipGetLastError());
     c->has_shapes = 1;
     return MEGA_OK;

@@ -27,6 +27,22 @@ struct Material {       // one property row, with derived constants
     int pad;
 };
 
+// fl(a / b) < 1 for a >= 0, b > 0 (normal numbers), decided without the
+// division.  IEEE division rounds to nearest-even; the doubles next to 1 from
+// below are 1 - 2^-53 and 1, with midpoint 1 - 2^-54 (which is never a quotient
+// of two doubles: a 2^54 = b (2^54 - 1) has no solution in 53-bit
+// significands).  So fl(a/b) < 1 exactly when a/b < 1 - 2^-54, i.e.
+// (a - b) + b 2^-54 < 0.  For b/2 <= a < b, a - b is
+// exact (Sterbenz) and the fma rounds once, which keeps the sign of the exact
+// sum; outside that range the answer is plain.  This is the box test of
+// SOFT_SEARCH, |x_J,k - x_k| / a_J,k < 1 (BIN_SEARCH.F90:268-289).
+MEGA_HD bool quotient_below_one(double a, double b)
+{
+    if (a < 0.5 * b) return true;
+    if (!(a < b)) return false;
+    return fma(b, 0x1p-54, a - b) < 0.0;
+}
+
 // MLS_KERNEL0 cubic B-spline, value and d/d|z| (KERNEL.F90:57-71)
 template <class T>
 MEGA_HD void cubic_kernel(T x, T &phi, T &dphi)
