@@ -32,8 +32,9 @@
 // Layout in HBM: node vectors interleaved (3*i+k); stress/strain 6 per node
 // (Voigt XX,YY,ZZ,YZ,XZ,XY); neighbor lists as 0-based CSR (row_ptr int32[n+1],
 // col int32[nnz]) with phi[nnz], grad phi[3 nnz] in list order; the transposed
-// lists (trow[n+1], tsrc[nnz], grouped by the source's bin) carry grad phi[3 nnz]
-// in transposed order.
+// lists (trow[n+1], tsrc[nnz]; sorted by source when the lists stay for the
+// run, else grouped by the source's bin) carry grad phi[3 nnz] in transposed
+// order.
 // Pair kernels map blocks to XCDs in contiguous node ranges so that each
 // XCD's L2 holds the neighbor data its blocks gather.
 #include <hip/hip_runtime.h>
@@ -1284,8 +1285,9 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
 // order of the 27 bins around j's bin (bin B is entry 26 - c there when j's bin
 // is entry c around B), each group in bin order.  tbin[27 j + e] is that
 // group's size after the COUNT pass and its first slot before the FILL pass;
-// FILL writes col from row_ptr[i] on, tsrc (source of each transposed slot)
-// and tpos (transposed slot of each list entry).
+// FILL writes col from row_ptr[i] on, tsrc (source of each transposed slot),
+// tpos (transposed slot of each list entry) and, when tq is given, tq (list
+// position of each transposed slot, for k_tsort).
 constexpr int kCandPerLane = 4;
 
 template <bool FILL>
@@ -1293,7 +1295,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                                                    const double *win, BinGeom g, const int32_t *cell_start,
                                                    const int32_t *cell_nodes, int shsup, int32_t *counts,
                                                    const int32_t *row_ptr, int32_t *col, int32_t *tbin,
-                                                   int32_t *tsrc, int32_t *tpos)
+                                                   int32_t *tsrc, int32_t *tpos, int32_t *tq)
 {
     __shared__ int s_beg[kBlock / 64][28], s_off[kBlock / 64][28];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1378,6 +1380,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                         col[q] = cj_[r];
                         tpos[q] = kc[r];
                         tsrc[kc[r]] = i;
+                        if (tq) tq[kc[r]] = q;
                     }
                     kc[r] += add ? 1 : 0;
                     out += __popcll(mask);
@@ -1401,6 +1404,46 @@ __global__ void k_trow_len(int n, const int32_t *tbin, int32_t *len)
     if (j < n)
         for (int e = 0; e < 27; ++e) s += tbin[27 * (int64_t)j + e];
     len[j] = s;
+}
+
+// Lists that stay for the whole run (Lagrangian) get their transposed rows
+// ordered by source node: the force gather then reads the sources' P in
+// increasing address order.  One wave per row ranks its entries in LDS;
+// sources within a row are distinct, so an entry's rank is its slot.  tq holds
+// each slot's list position, so tpos follows the entries.
+constexpr int kTsortMax = 256;
+
+__global__ __launch_bounds__(kBlock) void k_tsort(int n, const int32_t *trow, int32_t *tsrc, int32_t *tq,
+                                                  int32_t *tpos)
+{
+    __shared__ int2 buf[kBlock / 64][kTsortMax];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (j >= n) return;   // whole waves leave together
+    const int s = trow[j], L = trow[j + 1] - s;
+    if (L > kTsortMax) {  // long row: one lane, insertion sort in place
+        if (lane == 0) {
+            for (int a = s + 1; a < s + L; ++a) {
+                const int vs = tsrc[a], vq = tq[a];
+                int b = a - 1;
+                while (b >= s && tsrc[b] > vs) { tsrc[b + 1] = tsrc[b]; tq[b + 1] = tq[b]; --b; }
+                tsrc[b + 1] = vs;
+                tq[b + 1] = vq;
+            }
+            for (int a = s; a < s + L; ++a) tpos[tq[a]] = a;
+        }
+        return;
+    }
+    int2 *row = buf[w];
+    for (int a = lane; a < L; a += 64) row[a] = make_int2(tsrc[s + a], tq[s + a]);
+    wave_sync();
+    for (int a = lane; a < L; a += 64) {
+        const int2 v = row[a];
+        int r = 0;
+        for (int b = 0; b < L; ++b) r += row[b].x < v.x;
+        tsrc[s + r] = v.x;
+        tpos[v.y] = s + r;
+    }
 }
 
 // group sizes -> first slot of each group
@@ -1500,7 +1543,7 @@ struct mega_ctx {
     void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr;
     void *shp = nullptr, *dshp = nullptr, *tdshp = nullptr, *tshp = nullptr;
     // lists
-    int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr;
+    int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr, *tq = nullptr;
     int32_t *tbin = nullptr;   // [27 n] transposed-row group sizes / first slots
     int32_t *counts = nullptr;
     int64_t nnz = 0, nnz_cap = 0;
@@ -1626,10 +1669,10 @@ unsigned xcd_grid(int n, int *nblocks)
 
 void free_pairs(mega_ctx *c)
 {
-    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tpos};
+    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tpos, c->tq};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    c->col = c->tsrc = c->tpos = nullptr;
+    c->col = c->tsrc = c->tpos = c->tq = nullptr;
     c->shp = c->dshp = c->tdshp = c->tshp = nullptr;
     c->nnz_cap = 0;
 }
@@ -1643,6 +1686,7 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
     const size_t r = c->rsz;
     int e = 0;
     e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tpos, cap);
+    if (c->c.lagrangian) e |= dalloc(&c->tq, cap);
     e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r); e |= dalloc(&c->tdshp, 3 * cap, r);
     if (c->has_gravity || c->pd) e |= dalloc(&c->tshp, cap, r);
     if (e) { free_pairs(c); return MEGA_ERR_NOMEM; }
@@ -1704,7 +1748,7 @@ int do_search(mega_ctx *c, const double *x)
         Scope s(c, "k_search_count");
         k_search<false><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts,
-            nullptr, nullptr, c->tbin, nullptr, nullptr);
+            nullptr, nullptr, c->tbin, nullptr, nullptr, nullptr);
     }
     HIPCHK(hipMemsetAsync(c->counts + n, 0, sizeof(int32_t), c->stream));
     int r = cub_scan(c, c->counts, c->row_ptr, n + 1);
@@ -1727,7 +1771,11 @@ int do_search(mega_ctx *c, const double *x)
         Scope s(c, "k_search_fill");
         k_search<true><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr,
-            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos);
+            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos, c->tq);
+    }
+    if (c->tq) {
+        Scope s(c, "k_tsort");
+        k_tsort<<<grid_for((int64_t)n * 64), kBlock, 0, c->stream>>>(n, c->trow, c->tsrc, c->tq, c->tpos);
     }
     HIPCHK(hipGetLastError());
     c->has_lists = 1;
