@@ -33,8 +33,8 @@
 // (Voigt XX,YY,ZZ,YZ,XZ,XY); neighbor lists as 0-based CSR (row_ptr int32[n+1],
 // col int32[nnz]) with phi[nnz], grad phi[3 nnz] in list order; the transposed
 // lists (trow[n+1], tsrc[nnz]; sorted by source when the lists stay for the
-// run, else grouped by the source's bin) carry grad phi[3 nnz] in transposed
-// order.
+// run, else grouped by the source's bin) carry {grad phi, source} records
+// (TPair) in transposed order.
 // Pair kernels map blocks to XCDs in contiguous node ranges so that each
 // XCD's L2 holds the neighbor data its blocks gather.
 #include <hip/hip_runtime.h>
@@ -85,12 +85,25 @@ constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
 // ---------------------------------------------------------------------------
 // kernel arguments
 
+// One transposed-list entry as the force gather reads it: grad phi_ij of the
+// pair (i -> j) and its source i, in one aligned record (16 B mixed, 32 B FP64).
+template <class R> struct TPair;
+template <> struct alignas(16) TPair<float> {
+    float gx, gy, gz;
+    int32_t src;
+};
+template <> struct alignas(32) TPair<double> {
+    double gx, gy, gz;
+    int32_t src, pad;
+};
+
 struct Params {
     int n;
     int64_t cap;                                     // component stride of the pair arrays
     // lists
     const int32_t *row_ptr, *col, *trow, *tsrc, *tpos;
-    void *shp, *dshp, *tdshp, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap
+    void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
+                                                     // tpair: TPair<R> per transposed slot
     // node arrays, FP64 masters
     const double *X0, *win, *vol, *mass;
     double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
@@ -463,7 +476,8 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
         for (int c = 0; c < 4; ++c)
             Bd[al][c] = -(u[0] * Mi[4 * c] + u[1] * Mi[1 + 4 * c] + u[2] * Mi[2 + 4 * c] + u[3] * Mi[3 + 4 * c]);
     }
-    R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp), *tdshp = as<R>(p.tdshp), *tshp = as<R>(p.tshp);
+    R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp), *tshp = as<R>(p.tshp);
+    TPair<R> *tpair = as<TPair<R>>(p.tpair);
     // phi_j = B.H phi_w, grad phi_j = (Bd_a.H + B_a) phi_w + (B.H) grad phi_w
     auto emit = [&](int q, int t, const double *h, const KernelEval &e) {
         double H[4] = {1.0, h[0], h[1], h[2]};
@@ -479,7 +493,10 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
         shp[q] = f;
         dshp[q] = gx; dshp[p.cap + q] = gy; dshp[2 * p.cap + q] = gz;
         if (p.tpos) {
-            tdshp[t] = gx; tdshp[p.cap + t] = gy; tdshp[2 * p.cap + t] = gz;
+            TPair<R> v;
+            v.gx = gx; v.gy = gy; v.gz = gz; v.src = node;
+            if (sizeof(TPair<R>) == 32) reinterpret_cast<int32_t *>(&v)[7] = 0;
+            tpair[t] = v;
             if (tshp) tshp[t] = f;
         }
     };
@@ -952,29 +969,25 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
     R f0 = R(0), f1 = R(0), f2 = R(0), fe = R(0);
     if (live) {
         const int t0 = p.trow[node], t1 = p.trow[node + 1];
-        const R *__restrict__ tx = as<R>(p.tdshp);
-        const R *__restrict__ ty = tx + p.cap;
-        const R *__restrict__ tz = ty + p.cap;
+        const TPair<R> *__restrict__ tp = as<const TPair<R>>(p.tpair);
         const R *__restrict__ Pb = as<R>(p.P);
         const R *__restrict__ ts = as<R>(p.tshp);
         for (int base = t0 + lane; base < t1; base += kGroup * U) {
+            R g0[U], g1[U], g2[U], P[U][10], sh[U];
             int src[U];
 #pragma unroll
             for (int e = 0; e < U; ++e) {
                 const int t = base + kGroup * e;
-                src[e] = t < t1 ? p.tsrc[t] : node;
-            }
-            R g0[U], g1[U], g2[U], P[U][10], sh[U];
-#pragma unroll
-            for (int e = 0; e < U; ++e) {
-                const int t = base + kGroup * e;
                 const bool ok = t < t1;
-                g0[e] = ok ? tx[t] : R(0);
-                g1[e] = ok ? ty[t] : R(0);
-                g2[e] = ok ? tz[t] : R(0);
+                const TPair<R> v = tp[ok ? t : t0];
+                g0[e] = ok ? v.gx : R(0);
+                g1[e] = ok ? v.gy : R(0);
+                g2[e] = ok ? v.gz : R(0);
+                src[e] = ok ? v.src : node;
                 sh[e] = (ok && p.has_gravity) ? ts[t] : R(0);
-                load_P<R>(Pb, src[e], P[e]);
             }
+#pragma unroll
+            for (int e = 0; e < U; ++e) load_P<R>(Pb, src[e], P[e]);
 #pragma unroll
             for (int e = 0; e < U; ++e) {
                 f0 += P[e][0] * g0[e] + P[e][3] * g1[e] + P[e][6] * g2[e];
@@ -1541,7 +1554,7 @@ struct mega_ctx {
     Material *mats = nullptr;
     // R arrays
     void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr;
-    void *shp = nullptr, *dshp = nullptr, *tdshp = nullptr, *tshp = nullptr;
+    void *shp = nullptr, *dshp = nullptr, *tpair = nullptr, *tshp = nullptr;
     // lists
     int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr, *tq = nullptr;
     int32_t *tbin = nullptr;   // [27 n] transposed-row group sizes / first slots
@@ -1616,7 +1629,7 @@ Params params(mega_ctx *c)
     p.n = c->n;
     p.cap = c->nnz_cap;
     p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
-    p.shp = c->shp; p.dshp = c->dshp; p.tdshp = c->tdshp; p.tshp = c->tshp;
+    p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
     p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
@@ -1669,11 +1682,11 @@ unsigned xcd_grid(int n, int *nblocks)
 
 void free_pairs(mega_ctx *c)
 {
-    void *ptrs[] = {c->col, c->shp, c->dshp, c->tdshp, c->tshp, c->tsrc, c->tpos, c->tq};
+    void *ptrs[] = {c->col, c->shp, c->dshp, c->tpair, c->tshp, c->tsrc, c->tpos, c->tq};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->col = c->tsrc = c->tpos = c->tq = nullptr;
-    c->shp = c->dshp = c->tdshp = c->tshp = nullptr;
+    c->shp = c->dshp = c->tpair = c->tshp = nullptr;
     c->nnz_cap = 0;
 }
 
@@ -1687,7 +1700,8 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
     int e = 0;
     e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tpos, cap);
     if (c->c.lagrangian) e |= dalloc(&c->tq, cap);
-    e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r); e |= dalloc(&c->tdshp, 3 * cap, r);
+    e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r);
+    e |= dalloc(&c->tpair, cap, c->mixed ? sizeof(TPair<float>) : sizeof(TPair<double>));
     if (c->has_gravity || c->pd) e |= dalloc(&c->tshp, cap, r);
     if (e) { free_pairs(c); return MEGA_ERR_NOMEM; }
     c->nnz_cap = cap;
