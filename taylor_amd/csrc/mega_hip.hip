@@ -1298,17 +1298,20 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
 // order of the 27 bins around j's bin (bin B is entry 26 - c there when j's bin
 // is entry c around B), each group in bin order.  tbin[27 j + e] is that
 // group's size after the COUNT pass and its first slot before the FILL pass;
+// COUNT also keeps each node's ballot masks of its first 64 kMaskCap candidates,
+// so FILL reads them instead of testing again.
 // FILL writes col from row_ptr[i] on, tsrc (source of each transposed slot),
 // tpos (transposed slot of each list entry) and, when tq is given, tq (list
 // position of each transposed slot, for k_tsort).
 constexpr int kCandPerLane = 4;
+constexpr int kMaskCap = 8;   // ballot masks kept per node by COUNT for FILL: its first 512 candidates
 
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *cells, const double *x,
                                                    const double *win, BinGeom g, const int32_t *cell_start,
                                                    const int32_t *cell_nodes, int shsup, int32_t *counts,
                                                    const int32_t *row_ptr, int32_t *col, int32_t *tbin,
-                                                   int32_t *tsrc, int32_t *tpos, int32_t *tq)
+                                                   int32_t *tsrc, int32_t *tpos, int32_t *tq, uint64_t *masks)
 {
     __shared__ int s_beg[kBlock / 64][28], s_off[kBlock / 64][28];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1349,6 +1352,8 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
             mine = FILL ? row_ptr[myi] : 0;
         }
         for (int base = 0; base < total; base += 64 * kCandPerLane) {
+            // does this chunk test candidates here (COUNT, or beyond the kept masks)?
+            const bool tests = !FILL || (base >> 6) + kCandPerLane > kMaskCap;
             int cj_[kCandPerLane], tix[kCandPerLane], kc[kCandPerLane];
             double cx[kCandPerLane], cy[kCandPerLane], cz[kCandPerLane];
             double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
@@ -1366,28 +1371,42 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 // this bin's running count (COUNT) or next slot (FILL) in j's transposed row;
                 // an earlier chunk of own nodes left it in tbin, written by this same lane
                 kc[r] = (j >= 0 && (FILL || t0 > 0)) ? tbin[tix[r]] : 0;
-                cx[r] = x[3 * jj]; cy[r] = x[3 * jj + 1]; cz[r] = x[3 * jj + 2];
-                wx[r] = win[3 * jj]; wy[r] = win[3 * jj + 1]; wz[r] = win[3 * jj + 2];
+                cx[r] = cy[r] = cz[r] = 0.0;
+                wx[r] = wy[r] = wz[r] = 1.0;
+                if (tests) {
+                    cx[r] = x[3 * jj]; cy[r] = x[3 * jj + 1]; cz[r] = x[3 * jj + 2];
+                    wx[r] = win[3 * jj]; wy[r] = win[3 * jj + 1]; wz[r] = win[3 * jj + 2];
+                }
             }
             for (int t = 0; t < tn; ++t) {
-                const int i = __shfl(myi, t, 64);
+                const int i = __builtin_amdgcn_readfirstlane(__shfl(myi, t, 64));
                 int out = __shfl(mine, t, 64);
-                const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+                double xi = 0.0, yi = 0.0, zi = 0.0;
+                if (tests) { xi = x[3 * i]; yi = x[3 * i + 1]; zi = x[3 * i + 2]; }
 #pragma unroll
                 for (int r = 0; r < kCandPerLane; ++r) {
                     if (base + 64 * r >= total) break;   // uniform
+                    const int c64 = (base >> 6) + r;
+                    uint64_t *mslot = masks + (int64_t)i * kMaskCap + c64;
                     bool add = false;
-                    if (cj_[r] >= 0) {
-                        const double dx = fabs(cx[r] - xi), dy = fabs(cy[r] - yi), dz = fabs(cz[r] - zi);
-                        if (shsup) {
-                            const double ax = dx / wx[r], ay = dy / wy[r], az = dz / wz[r];
-                            add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
-                        } else {   // the box test, exactly as the quotients decide it
-                            add = mega::quotient_below_one(dx, wx[r]) & mega::quotient_below_one(dy, wy[r]) &
-                                  mega::quotient_below_one(dz, wz[r]);
+                    unsigned long long mask;
+                    if (FILL && c64 < kMaskCap) {
+                        mask = *mslot;
+                        add = (mask >> lane) & 1ull;
+                    } else {
+                        if (cj_[r] >= 0) {
+                            const double dx = fabs(cx[r] - xi), dy = fabs(cy[r] - yi), dz = fabs(cz[r] - zi);
+                            if (shsup) {
+                                const double ax = dx / wx[r], ay = dy / wy[r], az = dz / wz[r];
+                                add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
+                            } else {   // the box test, exactly as the quotients decide it
+                                add = mega::quotient_below_one(dx, wx[r]) & mega::quotient_below_one(dy, wy[r]) &
+                                      mega::quotient_below_one(dz, wz[r]);
+                            }
                         }
+                        mask = __ballot(add);
+                        if (!FILL && c64 < kMaskCap && lane == 0) *mslot = mask;
                     }
-                    const unsigned long long mask = __ballot(add);
                     if (FILL && add) {
                         const int q = out + __popcll(mask & ((1ull << lane) - 1ull));
                         col[q] = cj_[r];
@@ -1558,6 +1577,7 @@ struct mega_ctx {
     // lists
     int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr, *tq = nullptr;
     int32_t *tbin = nullptr;   // [27 n] transposed-row group sizes / first slots
+    uint64_t *smask = nullptr; // [kMaskCap n] search ballot masks, COUNT -> FILL
     int32_t *counts = nullptr;
     int64_t nnz = 0, nnz_cap = 0;
     // bins
@@ -1762,7 +1782,7 @@ int do_search(mega_ctx *c, const double *x)
         Scope s(c, "k_search_count");
         k_search<false><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts,
-            nullptr, nullptr, c->tbin, nullptr, nullptr, nullptr);
+            nullptr, nullptr, c->tbin, nullptr, nullptr, nullptr, c->smask);
     }
     HIPCHK(hipMemsetAsync(c->counts + n, 0, sizeof(int32_t), c->stream));
     int r = cub_scan(c, c->counts, c->row_ptr, n + 1);
@@ -1785,7 +1805,7 @@ int do_search(mega_ctx *c, const double *x)
         Scope s(c, "k_search_fill");
         k_search<true><<<search_grid(c), kBlock, 0, c->stream>>>(
             c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr,
-            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos, c->tq);
+            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos, c->tq, c->smask);
     }
     if (c->tq) {
         Scope s(c, "k_tsort");
@@ -2166,6 +2186,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->orig, n);
     e |= dalloc(&c->cells_ne, std::min<int64_t>(c->ncell, n));
     e |= dalloc(&c->tbin, 27 * (int64_t)n);
+    e |= dalloc(&c->smask, kMaskCap * (int64_t)n);
     e |= dalloc(&c->cell_flag, c->ncell + 1);
     c->nb_node = (int)grid_for(n);
     xcd_grid(n, &c->nb_force);
@@ -2210,7 +2231,7 @@ int mega_destroy(mega_ctx *c)
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
-                    c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin};
+                    c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
