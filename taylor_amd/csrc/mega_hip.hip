@@ -1300,8 +1300,8 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
 // group's size after the COUNT pass and its first slot before the FILL pass;
 // COUNT also keeps each node's ballot masks of its first 64 kMaskCap candidates,
 // so FILL reads them instead of testing again.
-// FILL writes col from row_ptr[i] on, tsrc (source of each transposed slot),
-// tpos (transposed slot of each list entry) and, when tq is given, tq (list
+// FILL writes col from row_ptr[i] on, tpos (transposed slot of each list
+// entry) and, when given, tsrc (source of each transposed slot) and tq (list
 // position of each transposed slot, for k_tsort).
 constexpr int kCandPerLane = 4;
 constexpr int kMaskCap = 8;   // ballot masks kept per node by COUNT for FILL: its first 512 candidates
@@ -1411,7 +1411,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                         const int q = out + __popcll(mask & ((1ull << lane) - 1ull));
                         col[q] = cj_[r];
                         tpos[q] = kc[r];
-                        tsrc[kc[r]] = i;
+                        if (tsrc) tsrc[kc[r]] = i;
                         if (tq) tq[kc[r]] = q;
                     }
                     kc[r] += add ? 1 : 0;
@@ -1718,8 +1718,11 @@ int ensure_pairs(mega_ctx *c, int64_t nnz)
     const int64_t cap = nnz + nnz / 8 + 1024;
     const size_t r = c->rsz;
     int e = 0;
-    e |= dalloc(&c->col, cap); e |= dalloc(&c->tsrc, cap); e |= dalloc(&c->tpos, cap);
-    if (c->c.lagrangian) e |= dalloc(&c->tq, cap);
+    e |= dalloc(&c->col, cap); e |= dalloc(&c->tpos, cap);
+    if (c->c.lagrangian) {   // only lists that stay are sorted and feed the peridynamic force
+        e |= dalloc(&c->tsrc, cap);
+        e |= dalloc(&c->tq, cap);
+    }
     e |= dalloc(&c->shp, cap, r); e |= dalloc(&c->dshp, 3 * cap, r);
     e |= dalloc(&c->tpair, cap, c->mixed ? sizeof(TPair<float>) : sizeof(TPair<double>));
     if (c->has_gravity || c->pd) e |= dalloc(&c->tshp, cap, r);
