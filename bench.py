@@ -22,6 +22,9 @@ whole 80 us CPU-runnable bar (spacing 0.5 mm) runs in the three modes and is
 compared with FP64 and with the reference program's final state
 (tests/golden/taylor_h05_final.npz).  Gate: length within 0.5 %.
 
+semi_lagrangian (unless --no-semi): the same bar in the semi-Lagrangian
+formulation, FP64 and mixed rates beside the headline (not part of value).
+
 roofline: the dominant kernel's algorithmic bytes per launch (compulsory
 HBM traffic: every pair and node array it must touch read or written once,
 neighbor re-reads excluded) over its mean launch time from HIP events on the
@@ -35,7 +38,7 @@ a 12-step and a 2-step run (setup cancels).  Falls back to the oracle
 restatement when the reference build is absent.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes fp64,mixed,surrogate]
-                    [--stable-steps S] [--no-validate] [--no-cpu]
+                    [--stable-steps S] [--no-validate] [--no-cpu] [--no-semi]
 """
 from __future__ import annotations
 
@@ -84,6 +87,10 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         return nnz * (4 + 3 * r) + n * (10 * r + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 + (24 if mixed else 0))
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
+    if name == "k_rk":
+        # pairs: col + tpos read, phi + grad phi written in list order and
+        # {grad phi, source} in transposed order; nodes: own x, window, EBC flag, row
+        return nnz * (4 + 4 + r + 3 * r + 3 * r + 4) + n * (24 + 24 + 4 + 8)
     return 0.0
 
 
@@ -92,6 +99,29 @@ def node_step_bytes(n: int, nnz: int, mixed: bool) -> float:
     pass over the pairs, index + phi + grad phi, and each node array once)."""
     kbar = nnz / n
     return kbar * (20 if mixed else 36) + (368 if mixed else 468)
+
+
+def semi_step_bytes(n: int, nnz: int, mixed: bool) -> float:
+    """The same for the semi-Lagrangian step: the search writes the indices,
+    the force pass reads them and writes phi, the next interpolation reads
+    index + phi (RK in registers, bins amortised over 20 steps)."""
+    kbar = nnz / n
+    return kbar * (20 if mixed else 28) + (400 if mixed else 500)
+
+
+def semi_lagrangian(steps: int, warmup: int) -> dict:
+    """The same bar in the semi-Lagrangian formulation (SOFT_SEARCH and RK1
+    every step, GET_BINS every 20 steps): FP64 and mixed rates."""
+    prob = deck.taylor_problem(H, v0=V0, seed=SEED, lagrangian=0, pdsearch_steps=20)
+    out = {"workload": "same bar, semi-Lagrangian, bins every 20 steps", "steps": steps}
+    for name in ("fp64", "mixed"):
+        wall, kt, nnz, mx, _ = timed_run(prob, name, steps, warmup)
+        rate = prob.n * steps / wall
+        out[name] = {"rate": rate, "ms_per_step": 1e3 * wall / steps,
+                     "kernel_ms": {k: v[0] for k, v in kt.items()},
+                     "step_roofline_frac": rate * semi_step_bytes(prob.n, nnz, name != "fp64") / PEAK_HBM}
+        log(f"semi-Lagrangian {name}: {1e3 * wall / steps:.3f} ms/step, {rate:.4e} node-steps/s")
+    return out
 
 
 def read_traffic(kernel: str, mixed: bool):
@@ -258,6 +288,7 @@ def main():
     ap.add_argument("--stable-steps", type=int, default=1000)
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-semi", action="store_true", help="skip the semi-Lagrangian side measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -291,6 +322,8 @@ def main():
     kb = kernel_bytes(kname, n, nnz, mixed_prec)
     achieved = kb / (kms * 1e-3)
     traffic = read_traffic(kname, mixed_prec)
+
+    semi = None if args.no_semi else semi_lagrangian(min(args.steps, 50), 3)
 
     validation = None
     if not args.no_validate:
@@ -336,6 +369,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "algorithmic_bytes": kb},
+        "semi_lagrangian": semi,
         "validation": validation,
         "cpu_baseline": cpu,
     }
