@@ -121,6 +121,20 @@ int mega_fint(mega_ctx *ctx, int32_t linit, const double *dinc, const double *di
 int mega_constitute(int32_t n, const double *lmat, double *stress, double *strain, double *state,
                     const double *prop, int32_t mat_type, int32_t finite_strain, int32_t precision);
 
+/* One-time geometry preprocessing of the model on the device, without a
+ * context: GET_ADIAL, GET_VOLUMES and GET_SMOOTHING_LENGTHS as PRE_MODEL calls
+ * them (PRE_MODEL.F90:248-262; PRE_MODEL_ADIAL.F90:16-158,
+ * PRE_MODEL_VOLUMES.F90:16-121, PRE_MODEL_VOL2.F90:36-52,
+ * PRE_MODEL_SMOOTHING_LENGTHS.F90:16-102).  In: nump nodes coo[3 nump]
+ * (GCOO layout), numel elements elcon[8 numel] (MODEL_ELCON, 1-based node
+ * ids), hex[numel] = 1 for a hexahedron (block type 2), 0 for a tetrahedron
+ * (its first 4 ids); hex may be NULL (all hexahedra).  Out: win[3 nump]
+ * (ADIAL, the unscaled window), vol[nump], sm_len[6 nump] (may be NULL),
+ * dist_max[3] (XDIST_MAX, YDIST_MAX, ZDIST_MAX).  MEGA_ERR_ARG if an element
+ * names a node outside 1..nump. */
+int mega_preprocess(int32_t nump, const double *coo, int32_t numel, const int32_t *elcon, const int32_t *hex,
+                    double *win, double *vol, double *sm_len, double *dist_max);
+
 /* nsteps explicit steps of size dt, device resident: predictor, DO_INTERP,
  * coordinate update, (search + shapes when semi-Lagrangian), CONSTRUCT_FINT,
  * EBC and corrector.  MAIN.F90:304-658. */

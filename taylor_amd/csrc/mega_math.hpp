@@ -80,7 +80,7 @@ MEGA_HD void inv3_ref(const T *a, T *r)
 #undef R_
 }
 
-// DETERMINANT (DETERMINANT.F90:250-252)
+// DETERMINANT (DETERMINANT.F90:28-30)
 template <class T>
 MEGA_HD T det3_abs(const T *a)
 {

@@ -73,6 +73,7 @@ def load_library():
         "mega_interp": [vp, vp, vp],
         "mega_fint": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, dp],
         "mega_constitute": [i32, vp, vp, vp, vp, vp, i32, i32, i32],
+        "mega_preprocess": [i32, vp, i32, vp, vp, vp, vp, vp, vp],
         "mega_step": [vp, C.c_double, i32],
         "mega_download": [vp, i32, vp],
         "mega_upload": [vp, i32, vp],
@@ -299,6 +300,25 @@ def constitute(lmat, stress, strain, state, props, mat_type: int = 2, finite_str
     st = lib.mega_constitute(n, _p(_f64(lmat)), _p(stress), _p(strain), _p(state), _p(_f64(props)),
                              int(mat_type), int(finite_strain), int(precision))
     return _chk(st, "mega_constitute", allow=(1,))
+
+
+def preprocess(coo, elcon, hex=None):
+    """GET_ADIAL + GET_VOLUMES + GET_SMOOTHING_LENGTHS on the GPU
+    (PRE_MODEL.F90:248-262): coo (n, 3), elcon (numel, 8) 1-based node ids,
+    hex (numel,) 1 = hexahedron / 0 = tetrahedron or None (all hexahedra).
+    Returns (win (n, 3), vol (n,), sm_len (n, 6), dist_max (3,))."""
+    lib = load_library()
+    coo = _f64(coo)
+    el = _i32(elcon)
+    n, numel = coo.shape[0], el.shape[0]
+    assert coo.shape == (n, 3) and el.shape == (numel, 8)
+    hx = None if hex is None else _i32(hex)
+    if hx is not None:
+        assert hx.shape == (numel,)
+    win, vol, sm, dmax = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 6)), np.zeros(3)
+    st = lib.mega_preprocess(n, _p(coo), numel, _p(el), _p(hx), _p(win), _p(vol), _p(sm), _p(dmax))
+    _chk(st, "mega_preprocess")
+    return win, vol, sm, dmax
 
 
 class Mlp:

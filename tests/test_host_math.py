@@ -1,7 +1,10 @@
-"""Host checks of device math that replaces a reference expression by an
-exact equivalent: the division-free box test of the neighbor search
-(mega::quotient_below_one, BIN_SEARCH.F90:268-289) against the division
-itself, compiled for the host from the same header the kernels use."""
+"""Host checks of device math, compiled for the host from the same headers
+the kernels use:
+- the division-free box test of the neighbor search (mega::quotient_below_one,
+  BIN_SEARCH.F90:268-289) against the division it replaces;
+- the double-double power of the geometry preprocessing (dd::pow_dd, for
+  VOL**THRD in PRE_MODEL_SMOOTHING_LENGTHS.F90:61): correctly rounded
+  (against sqrt, x*x, 1/x) and within one ulp of the C library's pow."""
 import os
 import shutil
 import subprocess
@@ -18,12 +21,22 @@ def _hipcc():
     return None
 
 
-@pytest.mark.skipif(_hipcc() is None, reason="hipcc not available")
-def test_quotient_below_one_equals_division(tmp_path):
-    exe = tmp_path / "kat_quotient"
+def _run_kat(tmp_path, name):
+    exe = tmp_path / name
     subprocess.run([_hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "taylor_amd", "csrc"),
-                    os.path.join(ROOT, "tests", "kat_quotient.cpp"), "-o", str(exe)],
+                    os.path.join(ROOT, "tests", name + ".cpp"), "-o", str(exe)],
                    check=True, capture_output=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "mismatches 0" in out.stdout
+    return out.stdout
+
+
+@pytest.mark.skipif(_hipcc() is None, reason="hipcc not available")
+def test_quotient_below_one_equals_division(tmp_path):
+    assert "mismatches 0" in _run_kat(tmp_path, "kat_quotient")
+
+
+@pytest.mark.skipif(_hipcc() is None, reason="hipcc not available")
+def test_pow_dd_correctly_rounded(tmp_path):
+    out = _run_kat(tmp_path, "kat_pow")
+    assert "exact_mismatches 0" in out and "libm_beyond_one_ulp 0" in out
