@@ -106,6 +106,19 @@ module mega_iface
        real(c_double), intent(out) :: e_int, e_kin, e_ext
      end function mega_energies
 
+     ! GET_ADIAL + GET_VOLUMES + GET_SMOOTHING_LENGTHS on the device (PRE_MODEL.F90:248-262);
+     ! sm_len may be c_null_ptr
+     integer(c_int) function mega_preprocess(nump, coo, numel, elcon, hex, win, vol, sm_len, dist_max) &
+          bind(C, name="mega_preprocess")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int32_t), value :: nump, numel
+       real(c_double), intent(in) :: coo(*)
+       integer(c_int32_t), intent(in) :: elcon(*), hex(*)
+       real(c_double), intent(out) :: win(*), vol(*)
+       type(c_ptr), value :: sm_len
+       real(c_double), intent(out) :: dist_max(3)
+     end function mega_preprocess
+
      type(c_ptr) function mega_status_string(status) bind(C, name="mega_status_string")
        import :: c_int, c_ptr
        integer(c_int), value :: status

@@ -30,3 +30,81 @@ def test_fortran_host_equals_python_host(tmp_path, precision):
     x, eps = g.download("x"), g.download("eps")
     g.close()
     assert np.array_equal(x, x_f) and np.array_equal(eps, eps_f)
+
+
+def write_deck(tmp_path, prob, mesh):
+    kfile = str(tmp_path / "bar.k")
+    deck.write_k(kfile, mesh)
+    deck.write_control(str(tmp_path / "Control.dat"), kfile, prob, list(mesh.sets.keys()), time_output=1.0)
+    return str(tmp_path / "Control.dat")
+
+
+def reference_step_count(dt, time_end):
+    """MAIN.F90:644-656: TIME = TIME + DLT after each step, exit once TIME > TIME_END."""
+    t, k = 0.0, 0
+    while True:
+        k += 1
+        t = t + dt
+        if t > time_end:
+            return k
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVE), reason="Fortran host not built")
+@pytest.mark.parametrize("lagrangian,precision", [(1, mega.MEGA_FP64), (1, mega.MEGA_MIXED), (0, mega.MEGA_FP64)])
+def test_fortran_host_reads_the_reference_input(tmp_path, lagrangian, precision):
+    """The Fortran host reads the reference's own input -- Control.dat and the
+    LS-DYNA model file it names -- preprocesses the geometry on the device and
+    runs the reference's number of steps; the Python host builds the same model
+    from the generator's arrays.  Final positions and plastic strain are equal
+    bit for bit."""
+    h, length = 0.7e-3, 6e-3
+    prob = deck.taylor_problem(h, length=length, seed=0, lagrangian=lagrangian, pdsearch_steps=5)
+    prob.time_end = 40.5 * prob.dt
+    mesh = deck.taylor_bar_mesh(h, length=length, seed=0)
+    ctl = write_deck(tmp_path, prob, mesh)
+    nsteps = reference_step_count(prob.dt, prob.time_end)
+    result = str(tmp_path / "result.bin")
+    out = subprocess.run([DRIVE, "--deck", ctl, result, str(precision), "0", "16"], capture_output=True,
+                         text=True, timeout=120)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert f"steps {nsteps}" in out.stdout
+    r = np.fromfile(result, dtype=np.float64)
+    x_f, eps_f = r[:3 * prob.n].reshape(prob.n, 3), r[3 * prob.n:]
+    g = mega.Mega(prob, precision=precision)
+    done = 0
+    while done < nsteps:
+        k = min(16, nsteps - done)
+        g.step(k)
+        done += k
+    x, eps = g.download("x"), g.download("eps")
+    g.close()
+    assert np.array_equal(x, x_f) and np.array_equal(eps, eps_f)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVE), reason="Fortran host not built")
+def test_fortran_host_reference_input_reaches_the_reference_final_state(tmp_path):
+    """The deck of the CPU-runnable bar (spacing 0.5 mm, 80 us), written as the
+    fixture's generator wrote it for the reference program
+    (tests/golden/make_golden.py --final), read and run to the end by the
+    Fortran host on the GPU in FP64: final length and mushroom radius against
+    the reference program's own final state."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "taylor_h05_final.npz"))
+    mesh = deck.taylor_bar_mesh(0.5e-3, seed=0)
+    prob = deck.taylor_problem(0.5e-3, seed=0)
+    nsteps = int(z["nsteps"][0])
+    prob.time_end = prob.dt * (nsteps - 0.5)
+    ctl = write_deck(tmp_path, prob, mesh)
+    result = str(tmp_path / "result.bin")
+    out = subprocess.run([DRIVE, "--deck", ctl, result, str(mega.MEGA_FP64), "0", "500"], capture_output=True,
+                         text=True, timeout=300)
+    print(out.stdout[-400:])
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert f"steps {nsteps}" in out.stdout
+    x = np.fromfile(result, dtype=np.float64)[:3 * prob.n].reshape(prob.n, 3)
+    wall = prob.sets["WALL"] - 1
+    L, L0 = x[:, 2].max(), z["x"][:, 2].max()
+    R = np.sqrt(x[wall, 0] ** 2 + x[wall, 1] ** 2).max()
+    R0 = np.sqrt(z["x"][wall, 0] ** 2 + z["x"][wall, 1] ** 2).max()
+    print(f"length rel {abs(L - L0) / L0:.2e}, radius rel {abs(R - R0) / R0:.2e}")
+    assert abs(L - L0) <= 1e-8 * L0 and abs(R - R0) <= 1e-8 * R0
