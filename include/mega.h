@@ -116,7 +116,8 @@ int mega_fint(mega_ctx *ctx, int32_t linit, const double *dinc, const double *di
 /* Per-point kinematics + constitution without a context (FINT.F90:758-801):
  * for n points, lmat[9n] (velocity-gradient increment, column-major),
  * stress[6n], strain[6n], state[20n] in/out, one property row prop[30] of
- * material type mat_type (1 elastic, 2 J2).  Status MEGA_ERR_RETURN_MAP if
+ * material type mat_type (1 elastic, 2 J2, 3 Drucker-Prager with damage, 6 J2
+ * with damage; CONSTITUTION.F90:43-75).  Status MEGA_ERR_RETURN_MAP if
  * any return map failed to converge. */
 int mega_constitute(int32_t n, const double *lmat, double *stress, double *strain, double *state,
                     const double *prop, int32_t mat_type, int32_t finite_strain, int32_t precision);
@@ -156,7 +157,9 @@ enum {
     MEGA_F_STRAIN_EQ = 11, /* equivalent total strain n (GSTRAIN_EQ)        */
     MEGA_F_WAVE_VEL = 12,  /* wave speed n (GMAX_WVEL)                     */
     MEGA_F_VPHY = 13,    /* physical velocity 3n = sum_j phi_ij v_j (current v) */
-    MEGA_F_NFIELDS = 14
+    MEGA_F_DAMAGE_STATE = 14, /* damage models' state 4n: EPS_SHEAR, EPS_VOL, D, DAM_SV
+                                 (STATE(1),(2),(4),(5); CONSTITUTE_DRUCKPRAG.F90:63-67) */
+    MEGA_F_NFIELDS = 15
 };
 int mega_download(mega_ctx *ctx, int32_t field, double *host);
 int mega_upload(mega_ctx *ctx, int32_t field, const double *host);

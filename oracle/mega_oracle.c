@@ -241,6 +241,130 @@ int or_von_mises(double *stress, const double *pred, double *state, const double
     return iters;
 }
 
+/* The effective trial stress of the damage models: the rotated stress of the
+ * previous step over (1 - D), plus C : (objective strain increment), with C
+ * from props 28-30 (FORM_CMAT, FINT_FUNCTIONS.F90:252-280). */
+static void effective_trial(const double *stress, double dam, const double *inc, const double *props, double *eff)
+{
+    const double lam = props[27], mu = props[28], l2m = props[29];
+    for (int k = 0; k < 6; ++k) eff[k] = stress[k] / (1.0 - dam);
+    eff[0] = eff[0] + (l2m * inc[0] + lam * inc[1] + lam * inc[2]);
+    eff[1] = eff[1] + (lam * inc[0] + l2m * inc[1] + lam * inc[2]);
+    eff[2] = eff[2] + (lam * inc[0] + lam * inc[1] + l2m * inc[2]);
+    eff[3] = eff[3] + mu * inc[3];
+    eff[4] = eff[4] + mu * inc[4];
+    eff[5] = eff[5] + mu * inc[5];
+}
+
+/* Isotropic damage from the norm of the total strain, irreversible through
+ * DAM_SV (CONSTITUTE_VONMISES_DAM.F90:86-120, CONSTITUTE_DRUCKPRAG.F90:88-118):
+ * the strain norm is TENSOR_NORM (shear terms doubled). */
+static void damage_update(const double *total, double *dam, double *dam_sv, const double *props)
+{
+    const double k_i = props[8], k_c = props[9], dam_max = props[10];
+    const double t = total[0] * total[0] + total[1] * total[1] + total[2] * total[2] +
+                     2.0 * (total[3] * total[3] + total[4] * total[4] + total[5] * total[5]);
+    const double eta = sqrt(t);
+    if (eta < *dam_sv + k_i) {
+        /* no new damage */
+    } else if (eta > k_i && eta < k_c) {
+        *dam = k_c * (eta - k_i) / (eta * (k_c - k_i));
+        *dam_sv = eta - k_i;
+    }
+    if (*dam > dam_max) *dam = dam_max;
+}
+
+/* VON_MISES_DAM: J2 radial return with power-law hardening in effective
+ * stress space, isotropic damage (CONSTITUTE_VONMISES_DAM.F90:14-196).
+ * stress: in, the rotated stress of the previous step; out, the new stress.
+ * state[2] = eps, state[3] = D, state[4] = DAM_SV.  Returns Newton
+ * iterations (0 if elastic) or -1 where the reference stops. */
+int or_von_mises_dam(double *stress, const double *total, const double *inc, double *state, const double *props)
+{
+    const double poiss = props[0], young = props[1], sig0 = props[3];
+    const double mu = young / (2.0 * (1.0 + poiss));
+    const double B = props[4], CE = props[5];
+    double eps = state[2], dam = state[3], dam_sv = state[4];
+    double eff[6], dev[6];
+    effective_trial(stress, dam, inc, props, eff);
+    double kk = eff[0] + eff[1] + eff[2];
+    dev[0] = eff[0] - kk / 3.0; dev[1] = eff[1] - kk / 3.0; dev[2] = eff[2] - kk / 3.0;
+    dev[3] = eff[3]; dev[4] = eff[4]; dev[5] = eff[5];
+    damage_update(total, &dam, &dam_sv, props);
+    const double sph = -kk / 3.0;
+    double tn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                     2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+    double sig23 = sqrt(2.0 / 3.0) * sig0;
+    double xk = sig23 * pow(1.0 + B * eps, CE);
+    double yld = tn - xk;
+    int iters = 0;
+    if (yld > 0.0) {
+        double dg = 0.0, epst = eps;
+        const double tol = sig23 * (double)1.0e-05f;
+        for (int i = 1; i <= 20; ++i) {
+            xk = sig23 * pow(1.0 + B * eps, CE);
+            yld = tn - xk - (2.0 * mu * dg);
+            if (fabs(yld) > tol) {
+                double xkp = B * CE * sig0 * pow(1.0 + B * eps, CE - 1.0);
+                double dyld = -2.0 * mu * (1.0 + xkp / 3.0 / mu);
+                dg = dg - yld / dyld;
+                iters = i;
+                if (i == 20) return -1;
+            } else {
+                eps = epst + sqrt(2.0 / 3.0) * dg;
+                break;
+            }
+            eps = epst + sqrt(2.0 / 3.0) * dg;
+        }
+        for (int k = 0; k < 6; ++k) dev[k] = dev[k] - dg * 2.0 * mu * dev[k] / tn;
+    }
+    state[2] = eps; state[3] = dam; state[4] = dam_sv;
+    for (int k = 0; k < 3; ++k) stress[k] = -sph + dev[k];
+    for (int k = 3; k < 6; ++k) stress[k] = -0.0 + dev[k];
+    for (int k = 0; k < 6; ++k) stress[k] = stress[k] * (1.0 - dam);
+    return iters;
+}
+
+/* DRUCK_PRAG: Drucker-Prager shear return with isotropic damage
+ * (CONSTITUTE_DRUCKPRAG.F90:14-241).  props 25-27 = Q_PHI, K_PHI, Q_PSI as the
+ * reader prepares them; the tension cut-off branch is never entered there.
+ * state[0] = EPS_SHEAR, [1] = EPS_VOL, [2] = EPS, [3] = D, [4] = DAM_SV.
+ * Returns 1 if the shear surface was crossed, else 0. */
+int or_druck_prag(double *stress, const double *total, const double *inc, double *state, const double *props)
+{
+    const double poiss = props[0], young = props[1];
+    const double q_phi = props[24], k_phi = props[25], q_psi = props[26];
+    double eps_shear = state[0], eps = state[2], dam = state[3], dam_sv = state[4];
+    double eff[6], dev[6];
+    effective_trial(stress, dam, inc, props, eff);
+    damage_update(total, &dam, &dam_sv, props);
+    double kk = eff[0] + eff[1] + eff[2];
+    dev[0] = eff[0] - kk / 3.0; dev[1] = eff[1] - kk / 3.0; dev[2] = eff[2] - kk / 3.0;
+    dev[3] = eff[3]; dev[4] = eff[4]; dev[5] = eff[5];
+    const double tau_pre = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                                2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+    const double sph_pre = (eff[0] + eff[1] + eff[2]) / 3.0;
+    const double f = tau_pre + q_phi * sph_pre - k_phi;
+    int sheared = 0;
+    if (f < 0.0) {
+        for (int k = 0; k < 6; ++k) stress[k] = eff[k];
+    } else {
+        sheared = 1;
+        const double tau = k_phi - q_phi * sph_pre;
+        const double bulk = young / (3.0 * (1.0 - 2.0 * poiss));
+        const double shear = young / (2.0 * (1.0 + poiss));
+        const double gamma = f / (2.0 * shear + bulk * q_phi * q_psi);
+        const double sph = sph_pre - bulk * q_psi * gamma;
+        for (int k = 0; k < 6; ++k) stress[k] = dev[k] * tau / tau_pre;
+        for (int k = 0; k < 3; ++k) stress[k] = stress[k] + sph;
+        eps_shear = eps_shear + gamma * sqrt(2.0 / 3.0 + 2.0 / 9.0 * (q_psi * q_psi));
+        eps = eps + eps_shear;
+    }
+    for (int k = 0; k < 6; ++k) stress[k] = stress[k] * (1.0 - dam);
+    state[0] = eps_shear; state[2] = eps; state[3] = dam; state[4] = dam_sv;
+    return sheared;
+}
+
 static double tnorm_strain(const double *v)
 {
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
@@ -306,6 +430,10 @@ int or_constitute_point(const double *lmat, double *stress, double *strain, doub
         for (int k = 0; k < 6; ++k) stress[k] = pred[k];
     } else if (mat_type == 2) {
         it = or_von_mises(stress, pred, state, props);
+    } else if (mat_type == 6) {   /* CONSTITUTION.F90:70-75: stress holds the rotated old stress */
+        it = or_von_mises_dam(stress, strain, de, state, props);
+    } else if (mat_type == 3) {
+        or_druck_prag(stress, strain, de, state, props);
     } else {
         it = -2;
     }

@@ -44,6 +44,10 @@ double or_determinant(const double *a);
 void or_hughes_winget(const double *lmat, double *rot, double *strain, double *d);
 void or_rotate_tensor(const double *rot, double *v);
 int  or_von_mises(double *stress, const double *stress_pred, double *state, const double *props);
+int  or_von_mises_dam(double *stress, const double *total_strain, const double *inc_strain, double *state,
+                      const double *props);
+int  or_druck_prag(double *stress, const double *total_strain, const double *inc_strain, double *state,
+                   const double *props);
 void or_estimate_moduli(const double *stress_inc, const double *strain_inc, double *shear, double *bulk);
 /* Per-point update as CONSTRUCT_FINT does it for one node (FINT.F90:758-801):
  * HW(L) -> rotate stress/strain -> strain += dstrain -> trial -> return map.

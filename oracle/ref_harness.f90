@@ -18,7 +18,7 @@ module ref_harness
   implicit none
 
   external :: MLS_KERNEL0, M44INV, INV3, INVERSE, DETERMINANT
-  external :: HUGHES_WINGET, ROTATE_TENSOR, VON_MISES, RK1
+  external :: HUGHES_WINGET, ROTATE_TENSOR, VON_MISES, RK1, VON_MISES_DAM, DRUCK_PRAG
   external :: GET_BINS, SOFT_SEARCH, CONSTRUCT_FINT, CONSTRUCT_FINT_PD
   external :: UDFM_SHAPE_TENSOR, DFM_SHAPE_TENSOR, ESTIMATE_MODULI
 
@@ -115,6 +115,22 @@ contains
     real(c_double), intent(inout) :: stress(6), state(20)
     real(c_double), intent(in) :: strain(6), stress_pred(6), props(30)
     call VON_MISES(stress, strain, stress_pred, state, props)
+  end subroutine
+
+  ! stress: in, the rotated stress of the previous step (what CONSTITUTION's
+  ! LSTRESS holds at FINT.F90:797); out, the new stress
+  subroutine ref_von_mises_dam(stress, total_strain, inc_strain, stress_pred, state, props) &
+       bind(C, name="ref_von_mises_dam")
+    real(c_double), intent(inout) :: stress(6), state(20)
+    real(c_double), intent(in) :: total_strain(6), inc_strain(6), stress_pred(6), props(30)
+    call VON_MISES_DAM(stress, total_strain, inc_strain, stress_pred, state, props)
+  end subroutine
+
+  subroutine ref_druck_prag(stress, total_strain, inc_strain, stress_pred, state, props) &
+       bind(C, name="ref_druck_prag")
+    real(c_double), intent(inout) :: stress(6), state(20)
+    real(c_double), intent(in) :: total_strain(6), inc_strain(6), stress_pred(6), props(30)
+    call DRUCK_PRAG(stress, total_strain, inc_strain, stress_pred, state, props)
   end subroutine
 
   subroutine ref_estimate_moduli(stress_inc, strain_inc, shear, bulk) bind(C, name="ref_estimate_moduli")

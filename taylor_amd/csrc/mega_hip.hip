@@ -111,6 +111,7 @@ struct Params {
     float4 *d4, *t4;                                 // FP32 gather copies of d and dtot (mixed)
     // state and scratch in R
     void *stress, *strain, *eps, *red, *P;
+    void *st4;   // R[4n]: the damage models' EPS_SHEAR, EPS_VOL, D, DAM_SV (STATE(1),(2),(4),(5))
     const int32_t *ebc, *ebc_node, *mat;
     const Material *mats;
     double *wpart;                                   // per-block cumulative work (k_node)
@@ -864,6 +865,19 @@ __global__ __launch_bounds__(kBlock) void k_node(Params p)
         } else if (mt.type == 1) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) sr[k] = pred[k];
+        } else if (mt.type == 6 || mt.type == 3) {
+            // CONSTITUTION.F90:58-75: the damage models take the rotated stress of the
+            // previous step, the total strain and the objective increment
+            R *st4 = as<R>(p.st4) + 4 * (int64_t)i;
+            R es = st4[0], dam = st4[2], dsv = st4[3], tot[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) tot[k] = (R)e6[k];
+            if (mt.type == 6) {
+                if (mega::von_mises_dam(sr, tot, der, ep, dam, dsv, mt) < 0 && valid) atomicAdd(&p.flags[0], 1);
+            } else {
+                mega::druck_prag(sr, tot, der, es, ep, dam, dsv, mt);
+            }
+            if (valid) { st4[0] = es; st4[2] = dam; st4[3] = dsv; }
         } else if (valid) {
             atomicAdd(&p.flags[1], 1);
         }
@@ -1533,6 +1547,15 @@ __global__ void k_constitute(int n, const double *lmat, double *stress, double *
     R ep = (R)state[20 * i + 2];
     if (mt.type == 2) {
         if (mega::von_mises(sr, pred, ep, mt) < 0) atomicAdd(fail, 1);
+    } else if (mt.type == 6 || mt.type == 3) {
+        R es = (R)state[20 * i], dam = (R)state[20 * i + 3], dsv = (R)state[20 * i + 4], tot[6];
+        for (int k = 0; k < 6; ++k) tot[k] = (R)e6[k];
+        if (mt.type == 6) {
+            if (mega::von_mises_dam(sr, tot, der, ep, dam, dsv, mt) < 0) atomicAdd(fail, 1);
+        } else {
+            mega::druck_prag(sr, tot, der, es, ep, dam, dsv, mt);
+        }
+        state[20 * i] = (double)es; state[20 * i + 3] = (double)dam; state[20 * i + 4] = (double)dsv;
     } else {
         for (int k = 0; k < 6; ++k) sr[k] = pred[k];
     }
@@ -1548,6 +1571,8 @@ Material make_material(const double *prop, int type)
     m.lam = prop[27]; m.mu = prop[28]; m.l2m = prop[29];
     m.bulk = m.E / (3.0 * (1.0 - 2.0 * m.nu));
     m.shear = m.E / (2.0 * (1.0 + m.nu));
+    m.k_i = prop[8]; m.k_c = prop[9]; m.dam_max = prop[10];
+    m.q_phi = prop[24]; m.k_phi = prop[25]; m.q_psi = prop[26];
     m.type = type;
     return m;
 }
@@ -1572,7 +1597,7 @@ struct mega_ctx {
     int32_t *ebc = nullptr, *ebc_node = nullptr, *mat = nullptr;
     Material *mats = nullptr;
     // R arrays
-    void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr;
+    void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr, *st4 = nullptr;
     void *shp = nullptr, *dshp = nullptr, *tpair = nullptr, *tshp = nullptr;
     // lists
     int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr, *tq = nullptr;
@@ -1654,7 +1679,7 @@ Params params(mega_ctx *c)
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
     p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
     p.d4 = c->d4; p.t4 = c->t4; p.Fst = c->Fst;
-    p.stress = c->stress; p.strain = c->strain; p.eps = c->eps; p.red = c->red; p.P = c->P;
+    p.stress = c->stress; p.strain = c->strain; p.eps = c->eps; p.red = c->red; p.P = c->P; p.st4 = c->st4;
     p.ebc = c->ebc; p.ebc_node = c->ebc_node; p.mat = c->mat; p.mats = c->mats;
     p.wpart = c->wpart; p.epart = c->epart; p.flags = c->flags;
     p.ql = c->c.ql; p.ql_coef = c->c.ql_coef; p.ql_len = c->c.ql_len;
@@ -2182,6 +2207,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
     e |= dalloc(&c->mats, c->host_mats.size());
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
+    e |= dalloc(&c->st4, 4 * (size_t)n, r);
     e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, (c->mixed ? 12 : 10) * (size_t)n, r);
     e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
@@ -2213,7 +2239,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     ok = ok && (vinit ? up(c->v, to_internal(c, vinit, 3).data(), n3 * 8) : zero(c->v, n3 * 8));
     ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
-         zero(c->eps, n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
+         zero(c->eps, n * r) && zero(c->st4, 4 * (size_t)n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
          zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n) &&
          zero(c->sstat, 32);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
@@ -2473,7 +2499,7 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
                     int32_t mat_type, int32_t finite_strain, int32_t precision)
 {
     if (n <= 0 || !lmat || !stress || !strain || !state || !prop) return MEGA_ERR_ARG;
-    if (mat_type != 1 && mat_type != 2) return MEGA_ERR_MATERIAL;
+    if (mat_type != 1 && mat_type != 2 && mat_type != 3 && mat_type != 6) return MEGA_ERR_MATERIAL;
     if (precision != MEGA_FP64 && precision != MEGA_MIXED) return MEGA_ERR_ARG;
     const Material mt = make_material(prop, mat_type);
     double *dl = nullptr, *ds = nullptr, *de = nullptr, *dst = nullptr;
@@ -2567,6 +2593,7 @@ static int field_width(int f)
     switch (f) {
     case MEGA_F_STRESS: case MEGA_F_STRAIN: return 6;
     case MEGA_F_EPS: case MEGA_F_STRAIN_EQ: case MEGA_F_WAVE_VEL: return 1;
+    case MEGA_F_DAMAGE_STATE: return 4;
     default: return 3;
     }
 }
@@ -2597,6 +2624,7 @@ int mega_download(mega_ctx *c, int32_t field, double *host)
     case MEGA_F_STRESS: r = state_out(c, c->stress, 6 * (int64_t)n, host); break;
     case MEGA_F_STRAIN: r = state_out(c, c->strain, 6 * (int64_t)n, host); break;
     case MEGA_F_EPS: r = state_out(c, c->eps, n, host); break;
+    case MEGA_F_DAMAGE_STATE: r = state_out(c, c->st4, 4 * (int64_t)n, host); break;
     case MEGA_F_STRAIN_EQ:
         if (c->mixed) k_strain_eq<float><<<grid_for(n), kBlock, 0, c->stream>>>(n, (const float *)c->strain, c->scratch);
         else k_strain_eq<double><<<grid_for(n), kBlock, 0, c->stream>>>(n, (const double *)c->strain, c->scratch);
@@ -2638,6 +2666,7 @@ int mega_upload(mega_ctx *c, int32_t field, const double *host_caller)
     case MEGA_F_STRESS: r = state_in(c, c->stress, 6 * (int64_t)n, host); break;
     case MEGA_F_STRAIN: r = state_in(c, c->strain, 6 * (int64_t)n, host); break;
     case MEGA_F_EPS: r = state_in(c, c->eps, n, host); break;
+    case MEGA_F_DAMAGE_STATE: r = state_in(c, c->st4, 4 * (int64_t)n, host); break;
     default: {
         int w = 0;
         double *dst = field_ptr(c, field, &w);

@@ -23,7 +23,9 @@ struct Material {       // one property row, with derived constants
     double nu, E, rho, sig0, B, CE, damping;
     double lam, mu, l2m;   // props 28-30 (READ.F90:496-507)
     double bulk, shear;    // BULK_MOD, SHEAR_MOD (FINT_FUNCTIONS.F90:197-229)
-    int type;              // 1 elastic, 2 J2
+    double k_i, k_c, dam_max;       // damage props 9-11
+    double q_phi, k_phi, q_psi;     // Drucker-Prager props 25-27 (READ.F90:420-483)
+    int type;              // 1 elastic, 2 J2, 3 Drucker-Prager + damage, 6 J2 + damage
     int pad;
 };
 
@@ -278,6 +280,131 @@ MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
     stress[4] = dev[4];
     stress[5] = dev[5];
     return iters;
+}
+
+// The damage models (CONSTITUTION.F90:58-75) work in effective stress space:
+// eff = (rotated stress of the previous step) / (1 - D) + C : (objective
+// strain increment), C from props 28-30 (FORM_CMAT).
+template <class T>
+MEGA_HD void effective_trial(const T *stress, T dam, const T *inc, const Material &m, T *eff)
+{
+    const T lam = T(m.lam), mu = T(m.mu), l2m = T(m.l2m);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) eff[k] = stress[k] / (T(1) - dam);
+    eff[0] = eff[0] + (l2m * inc[0] + lam * inc[1] + lam * inc[2]);
+    eff[1] = eff[1] + (lam * inc[0] + l2m * inc[1] + lam * inc[2]);
+    eff[2] = eff[2] + (lam * inc[0] + lam * inc[1] + l2m * inc[2]);
+    eff[3] = eff[3] + mu * inc[3];
+    eff[4] = eff[4] + mu * inc[4];
+    eff[5] = eff[5] + mu * inc[5];
+}
+
+// Isotropic damage from the norm of the total strain (TENSOR_NORM, shear
+// terms doubled), irreversible through DAM_SV, capped at DAM_MAX
+// (CONSTITUTE_VONMISES_DAM.F90:86-120, CONSTITUTE_DRUCKPRAG.F90:88-118).
+template <class T>
+MEGA_HD void damage_update(const T *total, T &dam, T &dam_sv, const Material &m)
+{
+    const T k_i = T(m.k_i), k_c = T(m.k_c);
+    const T eta = sqrt(total[0] * total[0] + total[1] * total[1] + total[2] * total[2] +
+                       T(2) * (total[3] * total[3] + total[4] * total[4] + total[5] * total[5]));
+    if (eta < dam_sv + k_i) {
+        // no new damage
+    } else if (eta > k_i && eta < k_c) {
+        dam = k_c * (eta - k_i) / (eta * (k_c - k_i));
+        dam_sv = eta - k_i;
+    }
+    if (dam > T(m.dam_max)) dam = T(m.dam_max);
+}
+
+// VON_MISES_DAM (CONSTITUTE_VONMISES_DAM.F90:14-196): the radial return of
+// VON_MISES in effective stress space, then sigma = (1 - D) sigma_eff.
+// stress: in, the rotated stress of the previous step; out, the new stress.
+// Returns Newton iterations, or -1 where the reference stops.
+template <class T>
+MEGA_HD int von_mises_dam(T *stress, const T *total, const T *inc, T &eps, T &dam, T &dam_sv, const Material &m)
+{
+    const T mu = T(m.shear), sig0 = T(m.sig0), B = T(m.B), CE = T(m.CE);
+    T eff[6];
+    effective_trial(stress, dam, inc, m, eff);
+    const T kk = eff[0] + eff[1] + eff[2];
+    T dev[6] = {eff[0] - kk / T(3), eff[1] - kk / T(3), eff[2] - kk / T(3), eff[3], eff[4], eff[5]};
+    damage_update(total, dam, dam_sv, m);
+    const T tn = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                      T(2) * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+    const T r23 = T(sqrt(2.0 / 3.0));
+    const T sig23 = r23 * sig0;
+    T xk = sig23 * hard_pow(T(1) + B * eps, CE);
+    T yld = tn - xk;
+    int iters = 0;
+    if (yld > T(0)) {
+        T dg = T(0);
+        const T epst = eps;
+        const T tol = sig23 * T(kNewtonTol);
+        for (int it = 1; it <= 20; ++it) {
+            xk = sig23 * hard_pow(T(1) + B * eps, CE);
+            yld = tn - xk - (T(2) * mu * dg);
+            if (fabs(yld) > tol) {
+                const T xkp = B * CE * sig0 * hard_pow(T(1) + B * eps, CE - T(1));
+                const T dy = T(-2) * mu * (T(1) + xkp / T(3) / mu);
+                dg = dg - yld / dy;
+                iters = it;
+                if (it == 20) return -1;
+            } else {
+                eps = epst + r23 * dg;
+                break;
+            }
+            eps = epst + r23 * dg;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dev[k] = dev[k] - dg * T(2) * mu * dev[k] / tn;
+    }
+    const T one_d = T(1) - dam;
+    const T p3 = -(-kk / T(3));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) stress[k] = (p3 + dev[k]) * one_d;
+#pragma unroll
+    for (int k = 3; k < 6; ++k) stress[k] = (T(-0.0) + dev[k]) * one_d;
+    return iters;
+}
+
+// DRUCK_PRAG (CONSTITUTE_DRUCKPRAG.F90:14-241): Drucker-Prager shear return in
+// effective stress space with isotropic damage; props 25-27 as the reader
+// prepares them.  The reference never enters its tension cut-off branch.
+// Returns 1 where the shear surface was crossed.
+template <class T>
+MEGA_HD int druck_prag(T *stress, const T *total, const T *inc, T &eps_shear, T &eps, T &dam, T &dam_sv,
+                       const Material &m)
+{
+    const T q_phi = T(m.q_phi), k_phi = T(m.k_phi), q_psi = T(m.q_psi);
+    T eff[6];
+    effective_trial(stress, dam, inc, m, eff);
+    damage_update(total, dam, dam_sv, m);
+    const T kk = eff[0] + eff[1] + eff[2];
+    const T dev[6] = {eff[0] - kk / T(3), eff[1] - kk / T(3), eff[2] - kk / T(3), eff[3], eff[4], eff[5]};
+    const T tau_pre = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                           T(2) * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]));
+    const T sph_pre = (eff[0] + eff[1] + eff[2]) / T(3);
+    const T f = tau_pre + q_phi * sph_pre - k_phi;
+    const T one_d = T(1) - dam;
+    if (f < T(0)) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stress[k] = eff[k] * one_d;
+        return 0;
+    }
+    const T tau = k_phi - q_phi * sph_pre;
+    const T bulk = T(m.bulk), shear = T(m.shear);
+    const T gamma = f / (T(2) * shear + bulk * q_phi * q_psi);
+    const T sph = sph_pre - bulk * q_psi * gamma;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) stress[k] = dev[k] * tau / tau_pre;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) stress[k] = stress[k] + sph;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) stress[k] = stress[k] * one_d;
+    eps_shear = eps_shear + gamma * T(sqrt(2.0 / 3.0 + 2.0 / 9.0 * (m.q_psi * m.q_psi)));
+    eps = eps + eps_shear;
+    return 1;
 }
 
 // trial yield value ||dev pred|| - sqrt(2/3) sigma_0 (1 + B eps)^CE, the test

@@ -63,10 +63,61 @@ class Material:
         return Material(2, fill_lame(p))
 
     @staticmethod
+    def j2_damage(E: float, nu: float, rho: float, sig0: float, B: float, CE: float, k_i: float, k_c: float,
+                  dam_max: float, damping: float = 0.0) -> "Material":
+        """Type 6, VON_MISES_DAM (CONSTITUTE_VONMISES_DAM.F90): J2 as type 2 plus
+        isotropic damage with props 9-11 = K_I, K_C, DAM_MAX."""
+        p = np.zeros(30)
+        p[0], p[1], p[2], p[3], p[4], p[5] = nu, E, rho, sig0, B, CE
+        p[8], p[9], p[10] = k_i, k_c, dam_max
+        p[20] = damping
+        return Material(6, fill_lame(p))
+
+    @staticmethod
+    def drucker_prager(E: float, nu: float, rho: float, dp_type: int, phi_deg: float, c: float, psi_deg: float,
+                       t_cut: float, k_i: float, k_c: float, dam_max: float, damping: float = 0.0) -> "Material":
+        """Type 3, DRUCK_PRAG (CONSTITUTE_DRUCKPRAG.F90): props 4-7 = DP_TYPE, PHI,
+        C, PSI as the deck gives them, 8-11 = T_CUT, K_I, K_C, DAM_MAX; props
+        25-27 prepared as the reader does (drucker_prager_prep)."""
+        p = np.zeros(30)
+        p[0], p[1], p[2] = nu, E, rho
+        p[3], p[4], p[5], p[6] = float(dp_type), phi_deg, c, psi_deg
+        p[7], p[8], p[9], p[10] = t_cut, k_i, k_c, dam_max
+        p[20] = damping
+        return Material(3, fill_lame(drucker_prager_prep(p)))
+
+    @staticmethod
     def elastic(E: float, nu: float, rho: float) -> "Material":
         p = np.zeros(30)
         p[0], p[1], p[2] = nu, E, rho
         return Material(1, fill_lame(p))
+
+
+def drucker_prager_prep(p: np.ndarray) -> np.ndarray:
+    """Props 25-27 = Q_PHI, K_PHI, Q_PSI of the Drucker-Prager cone from
+    DP_TYPE, PHI (degrees), C = props 4-6 (READ.F90:420-483): 1 circumscribes,
+    2 middle-circumscribes, 3 inscribes the Mohr-Coulomb surface, 4 takes
+    Q_PHI/3, K_PHI, Q_PSI as given; Q_PSI is then reset to 0 (no dilatation)."""
+    p = np.array(p, dtype=np.float64)
+    dp_type, phi, c = int(p[3]), p[4], p[5]
+    phi_rad = phi * 3.141592653589793 / 180.0
+    srt32 = math.sqrt(3.0 / 2.0)
+    s = math.sin(phi_rad)
+    q_phi, k_phi = p[24], p[25]
+    if dp_type == 1:
+        q_phi = 2.0 * s / (srt32 * (3.0 - s)) * 3.0
+        k_phi = 6.0 * c * math.cos(phi_rad) / (srt32 * (3.0 - s))
+    elif dp_type == 2:
+        q_phi = 2.0 * s / (srt32 * (3.0 + s)) * 3.0
+        k_phi = 6.0 * c * math.cos(phi_rad) / (srt32 * (3.0 + s))
+    elif dp_type == 3:
+        q_phi = s / math.sqrt(9.0 + 3.0 * s * s) / math.sqrt(0.5) * 3.0
+        k_phi = 3.0 * c * math.cos(phi_rad) / math.sqrt(9.0 + 3.0 * s * s) / math.sqrt(0.5)
+    elif dp_type == 4:
+        q_phi = p[4] * 3.0
+        k_phi = p[5]
+    p[24], p[25], p[26] = q_phi, k_phi, 0.0
+    return p
 
 
 def fill_lame(p: np.ndarray) -> np.ndarray:

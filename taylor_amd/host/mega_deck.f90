@@ -240,6 +240,37 @@ contains
     existing = trial
   end subroutine assign_dbl
 
+  ! props 25-27 = Q_PHI, K_PHI, Q_PSI of the Drucker-Prager cone from DP_TYPE,
+  ! PHI (degrees), C = props 4-6 (READ.F90:420-483); Q_PSI is reset to 0
+  subroutine drucker_prager_prep(props)
+    real(c_double), intent(inout) :: props(30)
+    integer :: dp_type
+    real(c_double) :: phi, c, phi_rad, srt32, q_phi, k_phi
+    dp_type = int(props(4))
+    phi = props(5)
+    c = props(6)
+    phi_rad = phi * 3.141592653589793d0 / 180.0d0
+    srt32 = sqrt(3.0d0 / 2.d0)
+    q_phi = props(25)
+    k_phi = props(26)
+    if (dp_type == 1) then
+       q_phi = 2.0d0 * sin(phi_rad) / (srt32 * (3.0d0 - sin(phi_rad))) * 3.d0
+       k_phi = 6.0d0 * c * cos(phi_rad) / (srt32 * (3.0d0 - sin(phi_rad)))
+    else if (dp_type == 2) then
+       q_phi = 2.0d0 * sin(phi_rad) / (srt32 * (3.0d0 + sin(phi_rad))) * 3.d0
+       k_phi = 6.0d0 * c * cos(phi_rad) / (srt32 * (3.0d0 + sin(phi_rad)))
+    else if (dp_type == 3) then
+       q_phi = sin(phi_rad) / sqrt(9.0d0 + 3.0d0 * sin(phi_rad) * sin(phi_rad)) / sqrt(0.5d0) * 3.d0
+       k_phi = 3.0d0 * c * cos(phi_rad) / sqrt(9.0d0 + 3.0d0 * sin(phi_rad) * sin(phi_rad)) / sqrt(0.5d0)
+    else if (dp_type == 4) then
+       q_phi = props(5) * 3.d0
+       k_phi = props(6)
+    end if
+    props(25) = q_phi
+    props(26) = k_phi
+    props(27) = 0.0d0
+  end subroutine drucker_prager_prep
+
   ! ------------------------------------------------------------------ the whole deck
   subroutine deck_load(control_path, precision, m)
     character(*), intent(in) :: control_path
@@ -409,7 +440,7 @@ contains
              read(lines(i + 6 + j), *, iostat=ios) props(5 * j + 1:5 * j + 5)
              if (ios /= 0) call deck_fail('error reading a value row of *MATERIALS')
           end do
-          if (mtype == 3) call deck_fail('material type 3 (Drucker-Prager) is not on this path')
+          if (mtype == 3) call drucker_prager_prep(props)
           e = props(2); nu = props(1)              ! READ.F90:489-505
           lam = e * nu / ((1.0d0 + nu) * (1.0d0 - 2.0d0 * nu))
           mu = e / (2.0d0 * (1.0d0 + nu))

@@ -9,6 +9,7 @@ compressed .npz files.  The fixtures are data only.
     python tests/golden/make_golden.py            # dense, geometry, search/RK/FINT, runs
     python tests/golden/make_golden.py --pd       # peridynamics vectors and run
     python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
+    python tests/golden/make_golden.py --damage   # material types 6 and 3
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+from damage_util import damage_material  # noqa: E402
 from oracle_bind import REF_MEGA, load_ref, ref_set_control  # noqa: E402
 from taylor_amd import deck  # noqa: E402
 from vtk_read import read_unformatted_vtk  # noqa: E402
@@ -232,13 +234,14 @@ def pd_vectors(ref, rng):
     return out
 
 
-def mega_run(prob, mesh, nsteps, frames, workdir):
+def mega_run(prob, mesh, nsteps, frames, workdir, extra=()):
     """Run the reference program on the deck; return the chosen frames."""
     kfile = os.path.join(workdir, "bar.k")
     deck.write_k(kfile, mesh)
     prob.time_end = prob.dt * (nsteps - 0.5)
     deck.write_control(os.path.join(workdir, "Control.dat"), kfile, prob, ["BAR", "WALL"], time_output=-1.0,
-                       out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps")
+                       out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps" +
+                       "".join("\n" + v for v in extra))
     with open(os.path.join(workdir, "mega.out"), "w") as fo:
         subprocess.run([REF_MEGA, "-j", "1"], cwd=workdir, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
                        check=True, timeout=3600)
@@ -251,7 +254,60 @@ def mega_run(prob, mesh, nsteps, frames, workdir):
         out[f"step{k}_fint"] = np.stack([fr["fint_x"], fr["fint_y"], fr["fint_z"]], 1)
         out[f"step{k}_stress"] = np.stack([fr[f"stress_{c}"] for c in ("xx", "yy", "zz", "yz", "xz", "xy")], 1)
         out[f"step{k}_eps"] = fr["eps"]
+        for v in extra:
+            out[f"step{k}_{v}"] = fr[v]
     return out
+
+
+def damage_vectors(ref, rng):
+    """VON_MISES_DAM and DRUCK_PRAG of the compiled reference on random
+    states spanning their branches (elastic / plastic or sheared, damage
+    below threshold / growing / capped, CE = 1 and a power law, Q_PSI 0 and
+    not)."""
+    out = {}
+    n = 400
+    for tag, fn in (("dam6", ref.ref_von_mises_dam), ("dp3", ref.ref_druck_prag)):
+        P, S0, T, I, ST0, S1, ST1 = [], [], [], [], [], [], []
+        for t in range(n):
+            if tag == "dam6":
+                p = deck.Material.j2_damage(E=117e9, nu=0.35, rho=8930.0, sig0=400e6, B=rng.uniform(0, 3),
+                                            CE=(1.0, 0.3)[t % 2], k_i=rng.uniform(0.01, 0.1),
+                                            k_c=rng.uniform(0.2, 0.6), dam_max=rng.uniform(0.5, 0.99)).props
+                scale = 4e8
+            else:
+                p = deck.Material.drucker_prager(E=30e9, nu=0.2, rho=2400.0, dp_type=1 + t % 3,
+                                                 phi_deg=rng.uniform(10, 45), c=rng.uniform(1e6, 2e7),
+                                                 psi_deg=0.0, t_cut=3e6, k_i=rng.uniform(0.001, 0.01),
+                                                 k_c=rng.uniform(0.02, 0.1), dam_max=rng.uniform(0.5, 0.99)).props
+                if t % 4 == 3:
+                    p[26] = rng.uniform(0.05, 0.3)   # a dilatant cone, beyond what the reader prepares
+                scale = 3e7
+            s = rng.normal(size=6) * scale * rng.uniform(0.1, 2)
+            tot = rng.normal(size=6) * rng.uniform(0, 0.4 if tag == "dam6" else 0.08)
+            inc = rng.normal(size=6) * 1e-3 * rng.uniform(0, 3)
+            st = np.zeros(20)
+            st[0], st[1], st[2] = rng.uniform(0, 0.1), rng.uniform(0, 0.1), rng.uniform(0, 0.5)
+            st[3] = rng.uniform(0, 0.5) * (t % 2)
+            st[4] = rng.uniform(0, 0.2) * (t % 3 == 0)
+            s1, st1 = s.copy(), st.copy()
+            fn(s1, np.ascontiguousarray(tot), np.ascontiguousarray(inc), np.zeros(6), st1, np.ascontiguousarray(p))
+            P.append(p); S0.append(s); T.append(tot); I.append(inc); ST0.append(st); S1.append(s1); ST1.append(st1)
+        for k, v in (("props", P), ("stress0", S0), ("total", T), ("inc", I), ("state0", ST0), ("stress", S1),
+                     ("state", ST1)):
+            out[f"{tag}_{k}"] = np.array(v)
+    return out
+
+
+def damage_main():
+    ref = load_ref()
+    rng = np.random.default_rng(20260304)
+    np.savez_compressed(os.path.join(HERE, "damage.npz"), **damage_vectors(ref, rng))
+    mesh = deck.taylor_bar_mesh(H, length=LEN, seed=0)
+    for mt, name in ((6, "run_dam6.npz"), (3, "run_dp3.npz")):
+        prob = small(material=damage_material(mt))
+        with tempfile.TemporaryDirectory() as wd:
+            np.savez_compressed(os.path.join(HERE, name),
+                                **mega_run(prob, mesh, 40, (1, 10, 40), wd, extra=("dam", "eps_shear")))
 
 
 def final_state_vectors(workdir):
@@ -303,6 +359,9 @@ def main():
         return
     if "--pd" in sys.argv:
         pd_main()
+        return
+    if "--damage" in sys.argv:
+        damage_main()
         return
     ref = load_ref()
     rng = np.random.default_rng(20260101)

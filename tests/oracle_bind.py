@@ -71,6 +71,9 @@ def load_oracle():
     lib.or_rotate_tensor.argtypes = [dp, dp]
     lib.or_von_mises.argtypes = [dp, dp, dp, dp]
     lib.or_von_mises.restype = C.c_int
+    for nm in ("or_von_mises_dam", "or_druck_prag"):
+        getattr(lib, nm).argtypes = [dp, dp, dp, dp, dp]
+        getattr(lib, nm).restype = C.c_int
     lib.or_estimate_moduli.argtypes = [dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or_constitute_point.argtypes = [dp, dp, dp, dp, dp, C.c_int, C.c_int, dp]
     lib.or_constitute_point.restype = C.c_int
@@ -109,6 +112,8 @@ def load_ref():
     lib.ref_hughes_winget.argtypes = [dp, dp, dp, dp]
     lib.ref_rotate_tensor.argtypes = [dp, dp]
     lib.ref_von_mises.argtypes = [dp, dp, dp, dp, dp]
+    for nm in ("ref_von_mises_dam", "ref_druck_prag"):
+        getattr(lib, nm).argtypes = [dp, dp, dp, dp, dp, dp]
     lib.ref_estimate_moduli.argtypes = [dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.ref_rk1.argtypes = [dp, C.c_int, dp, dp, ip, C.c_int, ip, C.c_int, dp, dp]
     lib.ref_bin_geometry.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, dp, ip]

@@ -39,7 +39,7 @@ def run(ctl, tmp_path):
     (lambda t: t + "*BOUNDARY CONDITION\n#SET\nWALL\n#FIXITY\n1 0 0\n", "multiple values of fixity"),
     (lambda t: t.replace("#SET\nWALL\n#FIXITY", "#SET\nNOSUCHSET\n#FIXITY"), "set name not found"),
     (lambda t: t.replace("#IAUTO_TS\n  0", "#IAUTO_TS\n  1"), "automatic time step"),
-    (lambda t: t.replace("#TYPE\n2\n", "#TYPE\n3\n"), "Drucker-Prager"),
+    (lambda t: t.replace("#TYPE\n2\n", "#TYPE\nJ2\n"), "error reading MAT_TYPE"),
 ])
 def test_reader_refuses_malformed_decks(tmp_path, edit, message):
     ctl = write_deck(tmp_path, edit)

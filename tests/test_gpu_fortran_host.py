@@ -50,15 +50,20 @@ def reference_step_count(dt, time_end):
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVE), reason="Fortran host not built")
-@pytest.mark.parametrize("lagrangian,precision", [(1, mega.MEGA_FP64), (1, mega.MEGA_MIXED), (0, mega.MEGA_FP64)])
-def test_fortran_host_reads_the_reference_input(tmp_path, lagrangian, precision):
+@pytest.mark.parametrize("lagrangian,precision,mat_type", [(1, mega.MEGA_FP64, 2), (1, mega.MEGA_MIXED, 2),
+                                                           (0, mega.MEGA_FP64, 2), (1, mega.MEGA_FP64, 6),
+                                                           (1, mega.MEGA_FP64, 3)])
+def test_fortran_host_reads_the_reference_input(tmp_path, lagrangian, precision, mat_type):
     """The Fortran host reads the reference's own input -- Control.dat and the
     LS-DYNA model file it names -- preprocesses the geometry on the device and
     runs the reference's number of steps; the Python host builds the same model
     from the generator's arrays.  Final positions and plastic strain are equal
-    bit for bit."""
+    bit for bit (J2, and the damage types 6 and 3, whose Drucker-Prager cone
+    both hosts prepare from the deck's friction angle and cohesion)."""
+    from damage_util import damage_material
     h, length = 0.7e-3, 6e-3
-    prob = deck.taylor_problem(h, length=length, seed=0, lagrangian=lagrangian, pdsearch_steps=5)
+    mat = None if mat_type == 2 else damage_material(mat_type)
+    prob = deck.taylor_problem(h, length=length, seed=0, lagrangian=lagrangian, pdsearch_steps=5, material=mat)
     prob.time_end = 40.5 * prob.dt
     mesh = deck.taylor_bar_mesh(h, length=length, seed=0)
     ctl = write_deck(tmp_path, prob, mesh)

@@ -185,3 +185,36 @@ def test_whole_run_pd_bit_exact(oracle):
         assert np.array_equal(A["fint"], g[f"step{k}_fint"])
         assert np.array_equal(A["stress"], g[f"step{k}_stress"])
         assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
+
+
+@pytest.mark.parametrize("tag,fn", [("dam6", "or_von_mises_dam"), ("dp3", "or_druck_prag")])
+def test_damage_models_bit_exact(oracle, tag, fn):
+    """VON_MISES_DAM (type 6) and DRUCK_PRAG (type 3) of the compiled reference
+    on 400 random states each, spanning their branches."""
+    g = load("damage.npz")
+    f = getattr(oracle, fn)
+    for k in range(len(g[f"{tag}_props"])):
+        s = g[f"{tag}_stress0"][k].copy()
+        st = g[f"{tag}_state0"][k].copy()
+        f(s, g[f"{tag}_total"][k].copy(), g[f"{tag}_inc"][k].copy(), st, g[f"{tag}_props"][k].copy())
+        assert np.array_equal(s, g[f"{tag}_stress"][k]) and np.array_equal(st, g[f"{tag}_state"][k])
+
+
+@pytest.mark.parametrize("mat_type,name", [(6, "run_dam6.npz"), (3, "run_dp3.npz")])
+def test_whole_run_damage_bit_exact(oracle, mat_type, name):
+    """40 explicit steps of the reference program with material type 6 / 3:
+    frames 1, 10, 40 including the damage and the shear plastic strain."""
+    from damage_util import damage_material
+    g = load(name)
+    run = OracleRun(oracle, small(material=damage_material(mat_type)))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        assert np.array_equal(A["x"], g[f"step{k}_x"])
+        assert np.array_equal(A["fint"], g[f"step{k}_fint"])
+        assert np.array_equal(A["stress"], g[f"step{k}_stress"])
+        assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
+        assert np.array_equal(A["state"][:, 3], g[f"step{k}_dam"])
+        assert np.array_equal(A["state"][:, 0], g[f"step{k}_eps_shear"])

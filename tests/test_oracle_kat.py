@@ -319,3 +319,111 @@ def test_pd_free_flight_is_rigid(oracle):
     A = OracleRun(oracle, prob).step(5).arr
     assert np.abs(A["v"][:, 2] + 1.0e4).max() < 1e-7
     assert np.abs(A["stress"]).max() < 1e-6          # MPa, against a 400 MPa yield stress
+
+
+def _j2_dam_props(**kw):
+    p = deck.Material.j2_damage(E=117e9, nu=0.35, rho=8930.0, sig0=400e6, B=0.25, CE=1.0,
+                                k_i=kw.get("k_i", 0.01), k_c=kw.get("k_c", 0.3), dam_max=kw.get("dam_max", 0.9)).props
+    return np.ascontiguousarray(p)
+
+
+def test_damage_model_with_frozen_damage_is_scaled_j2(oracle):
+    """With the strain norm below the threshold the damage stays D, and
+    VON_MISES_DAM is (1 - D) x the J2 return of s/(1 - D) + C:de -- the same
+    operations as VON_MISES on that trial stress (plastic and elastic cases)."""
+    p = _j2_dam_props()
+    rng = np.random.default_rng(3)
+    lam, mu, l2m = p[27], p[28], p[29]
+    for t in range(200):
+        D = rng.uniform(0, 0.6)
+        s = rng.normal(size=6) * 3e8
+        de = rng.normal(size=6) * 2e-3
+        tot = rng.normal(size=6) * 1e-3           # norm far below K_I + DAM_SV
+        st = np.zeros(20); st[2] = rng.uniform(0, 0.2); st[3] = D; st[4] = 0.5
+        s_dam, st_dam = s.copy(), st.copy()
+        oracle.or_von_mises_dam(s_dam, tot, de, st_dam, p)
+        eff = s / (1.0 - D)
+        pred = eff.copy()
+        pred[0] = eff[0] + (l2m * de[0] + lam * de[1] + lam * de[2])
+        pred[1] = eff[1] + (lam * de[0] + l2m * de[1] + lam * de[2])
+        pred[2] = eff[2] + (lam * de[0] + lam * de[1] + l2m * de[2])
+        pred[3:] = eff[3:] + mu * de[3:]
+        s_j2, st_j2 = np.zeros(6), st.copy()
+        oracle.or_von_mises(s_j2, pred, st_j2, p)
+        assert st_dam[3] == D and st_dam[2] == st_j2[2]
+        assert np.array_equal(s_dam, s_j2 * (1.0 - D))
+
+
+def test_damage_law(oracle):
+    """D(eta) = K_C (eta - K_I) / (eta (K_C - K_I)) between the thresholds:
+    0 at K_I, 1 at K_C (continuity), increasing, capped at DAM_MAX, and
+    irreversible (a smaller strain afterwards keeps it)."""
+    k_i, k_c = 0.01, 0.3
+    p = _j2_dam_props(k_i=k_i, k_c=k_c, dam_max=1.0)
+
+    def damage(eta, st):
+        tot = np.array([eta, 0, 0, 0, 0, 0.0])
+        s = np.zeros(6)
+        oracle.or_von_mises_dam(s, tot, np.zeros(6), st, p)
+        return st[3]
+
+    etas = np.linspace(k_i * 1.0001, k_c * 0.9999, 50)
+    Ds = [damage(e, np.zeros(20)) for e in etas]
+    assert Ds[0] < 1e-3 and abs(Ds[-1] - 1.0) < 1e-3 and np.all(np.diff(Ds) > 0)
+    assert damage(0.5 * k_i, np.zeros(20)) == 0.0
+    st = np.zeros(20)
+    d1 = damage(0.2, st)
+    assert damage(0.05, st) == d1                       # no healing
+    p_cap = _j2_dam_props(k_i=k_i, k_c=k_c, dam_max=0.4)
+    st = np.zeros(20)
+    s = np.zeros(6)
+    oracle.or_von_mises_dam(s, np.array([0.2, 0, 0, 0, 0, 0.0]), np.zeros(6), st, p_cap)
+    assert st[3] == 0.4
+
+
+def test_drucker_prager_return_lands_on_the_cone(oracle):
+    """Without damage and dilatation (Q_PSI = 0): an elastic trial is returned
+    as C:de + s; a trial outside the cone, below its apex (K_PHI - Q_PHI p >
+    0), lands on it, ||dev s|| + Q_PHI p = K_PHI, with the pressure and the
+    deviatoric direction kept and the shear plastic strain grown by
+    sqrt(2/3) f / (2G).  Beyond the apex the reference's shear return (its
+    tension cut-off branch is never entered) reverses the deviator."""
+    mat = deck.Material.drucker_prager(E=30e9, nu=0.2, rho=2400.0, dp_type=1, phi_deg=30.0, c=10e6,
+                                       psi_deg=0.0, t_cut=3e6, k_i=10.0, k_c=20.0, dam_max=0.9)
+    p = np.ascontiguousarray(mat.props)
+    q_phi, k_phi = p[24], p[25]
+    G = p[1] / (2 * (1 + p[0]))
+    rng = np.random.default_rng(4)
+    n_in = n_out = n_apex = 0
+    for t in range(300):
+        s = rng.normal(size=6) * 1e7 * rng.uniform(0.1, 3)
+        de = rng.normal(size=6) * 1e-4
+        st = np.zeros(20)
+        out = s.copy()
+        sheared = oracle.or_druck_prag(out, np.zeros(6), de, st, p)
+        lam, mu, l2m = p[27], p[28], p[29]
+        eff = s.copy()
+        eff[0] += l2m * de[0] + lam * de[1] + lam * de[2]
+        eff[1] += lam * de[0] + l2m * de[1] + lam * de[2]
+        eff[2] += lam * de[0] + lam * de[1] + l2m * de[2]
+        eff[3:] += mu * de[3:]
+        pe = eff[:3].mean()
+        dev_e = eff.copy(); dev_e[:3] -= pe
+        tn = lambda v: np.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2 + 2 * (v[3] ** 2 + v[4] ** 2 + v[5] ** 2))
+        f = tn(dev_e) + q_phi * pe - k_phi
+        if not sheared:
+            n_in += 1
+            assert f < 0 and np.allclose(out, eff, rtol=1e-14, atol=1e-6)
+            continue
+        po = out[:3].mean()
+        dev_o = out.copy(); dev_o[:3] -= po
+        assert abs(po - pe) <= 1e-9 * abs(pe) + 1e-3
+        assert abs(st[0] - np.sqrt(2.0 / 3.0) * f / (2 * G)) <= 1e-12 * abs(st[0]) + 1e-18
+        if k_phi - q_phi * pe > 0:
+            n_out += 1
+            assert abs(tn(dev_o) + q_phi * po - k_phi) <= 1e-9 * k_phi
+            assert np.allclose(dev_o * tn(dev_e), dev_e * tn(dev_o), rtol=1e-9, atol=1e-3 * k_phi)
+        else:
+            n_apex += 1
+            assert float(np.dot(dev_o, dev_e)) < 0.0
+    assert n_in > 20 and n_out > 20 and n_apex > 0
