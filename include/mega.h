@@ -159,7 +159,10 @@ enum {
     MEGA_F_VPHY = 13,    /* physical velocity 3n = sum_j phi_ij v_j (current v) */
     MEGA_F_DAMAGE_STATE = 14, /* damage models' state 4n: EPS_SHEAR, EPS_VOL, D, DAM_SV
                                  (STATE(1),(2),(4),(5); CONSTITUTE_DRUCKPRAG.F90:63-67) */
-    MEGA_F_NFIELDS = 15
+    MEGA_F_APHY = 15,    /* LOCAL_ACL_PHY 3n: sum_j phi_ij a_j of the previous acceleration, as
+                            DO_INTERP forms it in the last step of mega_step (HANDELER.F90:114-133) */
+    MEGA_F_VPHY_STEP = 16, /* LOCAL_VEL_PHY 3n: sum_j phi_ij v_j of the predicted velocity, same place */
+    MEGA_F_NFIELDS = 17
 };
 int mega_download(mega_ctx *ctx, int32_t field, double *host);
 int mega_upload(mega_ctx *ctx, int32_t field, const double *host);

@@ -51,8 +51,8 @@ def build_hip(verbose: bool = False) -> str:
 
 def build_host(verbose: bool = False) -> str | None:
     """The Fortran host (taylor_amd/host/mega_drive): the bind(C) interface
-    module, the reader of the reference's input files and the time-loop
-    driver, linked against libmega_hip.so."""
+    module, the reader of the reference's input files, the writer of its
+    VTK frames and the time-loop driver, linked against libmega_hip.so."""
     fc = shutil.which("flang") or ("/opt/rocm/lib/llvm/bin/flang" if os.path.exists("/opt/rocm/lib/llvm/bin/flang") else None)
     if not fc:
         return None
@@ -64,9 +64,11 @@ def build_host(verbose: bool = False) -> str | None:
                  os.path.join(td, "mega_iface.o")],
                 [fc, "-O2", "-c", "-module-dir", td, "-I", td, os.path.join(host, "mega_deck.f90"), "-o",
                  os.path.join(td, "mega_deck.o")],
+                [fc, "-O2", "-c", "-module-dir", td, "-I", td, os.path.join(host, "mega_vtk.f90"), "-o",
+                 os.path.join(td, "mega_vtk.o")],
                 [fc, "-O2", "-module-dir", td, "-I", td, os.path.join(host, "mega_drive.f90"),
-                 os.path.join(td, "mega_iface.o"), os.path.join(td, "mega_deck.o"), "-L", HERE, "-lmega_hip",
-                 "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]]
+                 os.path.join(td, "mega_iface.o"), os.path.join(td, "mega_deck.o"), os.path.join(td, "mega_vtk.o"),
+                 "-L", HERE, "-lmega_hip", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]]
         for cmd in cmds:
             if verbose:
                 print(" ".join(cmd), flush=True)

@@ -1598,6 +1598,7 @@ struct mega_ctx {
     Material *mats = nullptr;
     // R arrays
     void *stress = nullptr, *strain = nullptr, *eps = nullptr, *red = nullptr, *P = nullptr, *st4 = nullptr;
+    double *vphy_step = nullptr, *aphy_step = nullptr;   // DO_INTERP of v, a at the last step of a call
     void *shp = nullptr, *dshp = nullptr, *tpair = nullptr, *tshp = nullptr;
     // lists
     int32_t *row_ptr = nullptr, *col = nullptr, *trow = nullptr, *tsrc = nullptr, *tpos = nullptr, *tq = nullptr;
@@ -2208,6 +2209,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->mats, c->host_mats.size());
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
     e |= dalloc(&c->st4, 4 * (size_t)n, r);
+    e |= dalloc(&c->vphy_step, n3); e |= dalloc(&c->aphy_step, n3);
     e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, (c->mixed ? 12 : 10) * (size_t)n, r);
     e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
@@ -2239,7 +2241,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     ok = ok && (vinit ? up(c->v, to_internal(c, vinit, 3).data(), n3 * 8) : zero(c->v, n3 * 8));
     ok = ok && zero(c->a, n3 * 8) && zero(c->d, n3 * 8) && zero(c->dtot, n3 * 8) && zero(c->uphy, n3 * 8) &&
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
-         zero(c->eps, n * r) && zero(c->st4, 4 * (size_t)n * r) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
+         zero(c->eps, n * r) && zero(c->st4, 4 * (size_t)n * r) && zero(c->vphy_step, n3 * 8) &&
+         zero(c->aphy_step, n3 * 8) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
          zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n) &&
          zero(c->sstat, 32);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
@@ -2564,6 +2567,22 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
         // the predictor is fused into the previous k_force within one call
         if (!c->predicted) r = do_predict(c);
         if (r) return r;
+        if (last) {   // LOCAL_VEL_PHY, LOCAL_ACL_PHY as DO_INTERP forms them (HANDELER.F90:114-137)
+            if (c->pd) {
+                HIPCHK(hipMemcpyAsync(c->vphy_step, c->v, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipMemcpyAsync(c->aphy_step, c->a, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                Params p = params(c);
+                const unsigned g = grid_for((int64_t)n * kGroup);
+                if (c->mixed) {
+                    k_interp<float, false><<<g, kBlock, 0, c->stream>>>(p, c->v, c->vphy_step);
+                    k_interp<float, false><<<g, kBlock, 0, c->stream>>>(p, c->a, c->aphy_step);
+                } else {
+                    k_interp<double, false><<<g, kBlock, 0, c->stream>>>(p, c->v, c->vphy_step);
+                    k_interp<double, false><<<g, kBlock, 0, c->stream>>>(p, c->a, c->aphy_step);
+                }
+            }
+        }
         if (c->pd) {
             r = do_pd_fint(c, 0, last ? 1 : 2);   // DO_INTERP is the identity (HANDELER.F90:134-137)
         } else if (lag) {
@@ -2611,6 +2630,8 @@ static double *field_ptr(mega_ctx *c, int f, int *width)
     case MEGA_F_FINT: return c->fint;
     case MEGA_F_FEXT: return c->fext;
     case MEGA_F_WAVE_VEL: *width = 1; return c->wave_vel;
+    case MEGA_F_APHY: return c->aphy_step;
+    case MEGA_F_VPHY_STEP: return c->vphy_step;
     default: return nullptr;
     }
 }
@@ -2670,7 +2691,7 @@ int mega_upload(mega_ctx *c, int32_t field, const double *host_caller)
     default: {
         int w = 0;
         double *dst = field_ptr(c, field, &w);
-        if (!dst || field == MEGA_F_WAVE_VEL) return MEGA_ERR_ARG;
+        if (!dst || field == MEGA_F_WAVE_VEL || field == MEGA_F_APHY || field == MEGA_F_VPHY_STEP) return MEGA_ERR_ARG;
         HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * w * n, hipMemcpyHostToDevice, c->stream));
         if (field == MEGA_F_D || field == MEGA_F_DTOT) r = sync_d4(c);
         if (field == MEGA_F_DTOT) c->F_valid = 0;

@@ -34,17 +34,30 @@ module mega_deck
   use mega_iface
   implicit none
   private
-  public :: deck_model, deck_load, deck_fail
+  public :: deck_model, deck_load, deck_fail, NOUTV, out_names
+
+  integer, parameter :: LL = 512, LLD = 4096
+
+  ! *OUTPUT VARIABLES names (PRE_CONTROL.F90:139-183), in this order
+  integer, parameter :: NOUTV = 27
+  character(len=24), parameter :: out_names(NOUTV) = [character(len=24) :: &
+       'node_id', 'displacement', 'velocity', 'acceleration', 'fint', 'fixity', 'bodyid', 'material_type', &
+       'initial_coordinate', 'init_velocity', 'normalized_window', 'nodal_volume', 'nodal_mass', &
+       'Poissons_ratio', 'Youngs_modulus', 'density', 'physical_window', 'characteristic_distance', &
+       'max_wave_vel', 'pre_disp_force', 'stress', 'strain', 'eps', 'eps_shear', 'eps_vol', 'dam', 'IJKspace']
 
   type :: deck_model
      integer(c_int32_t) :: n = 0, numel = 0
      real(c_double) :: dt = 0.0d0, time_end = 0.0d0, time_output = 0.0d0
      type(mega_control) :: ctrl
-     real(c_double), allocatable :: x0(:), win(:), vol(:), mass(:), prop(:), vinit(:)
-     integer(c_int32_t), allocatable :: ebc(:), ebc_node(:), mat_type(:)
+     real(c_double), allocatable :: x0(:), win(:), vol(:), mass(:), prop(:), vinit(:), norm_win(:)
+     real(c_double), allocatable :: vinit_card(:)     ! MODEL_VINIT as the cards give it (fixed dofs too)
+     integer(c_int32_t), allocatable :: ebc(:), ebc_node(:), mat_type(:), body_id(:), elcon(:)
+     logical :: unf = .false., fem_output = .false.   ! 'unformatted' frames; *FEM OUTPUT cells
+     logical :: outv(NOUTV) = .false.                 ! requested output variables
+     character(len=LLD) :: dir = ''                   ! directory of Control.dat (frames go there)
   end type deck_model
 
-  integer, parameter :: LL = 512
 
 contains
 
@@ -284,7 +297,7 @@ contains
     real(c_double) :: dltval, qcoef, qlen, pdstime, grav(3), vel(3), nwin, props(30), bforce(3)
     real(c_double) :: e, nu, lam, mu, dmax(3), qmin
     integer :: fixity(3), body
-    logical :: have_time, have_ql, have_qlen, have_rk, have_form, have_int, have_win, have_mat
+    logical :: have_time, have_ql, have_qlen, have_rk, have_form, have_int, have_win, have_mat, have_outv, out_basic
     real(c_double), allocatable :: coo(:), win0(:), vol(:), norm_win(:), prop(:), vinit(:)
     integer(c_int32_t), allocatable :: elcon(:), part(:), set_id(:), set_ptr(:), set_node(:), hex(:)
     integer(c_int32_t), allocatable :: ebc(:), mat(:), body_id(:)
@@ -293,11 +306,12 @@ contains
     slash = index(control_path, '/', back=.true.)
     dir = ''
     if (slash > 0) dir = control_path(1:slash)
+    m%dir = dir
 
     ! control cards (PRE_CONTROL.F90:132-588) and their defaults (:589-843)
     kfile = ''
     have_time = .false.; have_ql = .false.; have_qlen = .false.; have_rk = .false.
-    have_form = .false.; have_int = .false.
+    have_form = .false.; have_int = .false.; have_outv = .false.; out_basic = .false.
     iql = 1; qcoef = 0.001d0; qlen = 0.0d0; rk = [1, 3, 0]; iperi = 0; isup = 0; pdstime = 0.0d0
     ifin = 1; ilag = 0; itype = 2; grav = 0.0d0; m%time_output = -1.0d0; iauto = 0; dltval = 0.0d0
     do i = 1, nl
@@ -348,8 +362,32 @@ contains
        case ('*OUTPUT PARAMETERS')
           read(val(lines, nl, i, 1), *, iostat=ios) m%time_output
           if (ios /= 0) call deck_fail('error reading the card *OUTPUT PARAMETERS')
+       case ('*FEM OUTPUT')
+          read(val(lines, nl, i, 1), *, iostat=ios) k
+          if (ios /= 0 .or. k < 0 .or. k > 1) call deck_fail('error reading the card *FEM OUTPUT')
+          m%fem_output = k == 1
+       case ('*OUTPUT VARIABLES')   ! every later line names a variable (PRE_CONTROL.F90:139-214)
+          have_outv = .true.
+          do j = i + 1, nl
+             select case (trim(lines(j)))
+             case ('unformatted')
+                m%unf = .true.
+             case ('all')
+                m%outv = .true.
+             case ('basic')
+                out_basic = .true.
+             case default
+                do k = 1, NOUTV
+                   if (trim(lines(j)) == trim(out_names(k))) m%outv(k) = .true.
+                end do
+             end select
+          end do
        end select
     end do
+    if (out_basic) then           ! node_id, displacement, velocity, bodyid, eps (PRE_CONTROL.F90:208-214)
+       m%outv([1, 2, 3, 7, 23]) = .true.
+    end if
+    if (.not. have_outv) m%outv([1, 2, 7, 8, 12, 16]) = .true.   ! defaults (PRE_CONTROL.F90:624-636)
     if (.not. have_time) call deck_fail('card *TIME PARAMETERS not found')
     if (iauto /= 0) call deck_fail('IAUTO_TS = 1 (automatic time step) is not on this path')
     m%dt = dltval
@@ -453,6 +491,7 @@ contains
        end select
     end do
     where (ebc == -1) ebc = 0                      ! unassigned dofs are free (READ.F90:532-556)
+    where (body_id == -1) body_id = 1              ! READ.F90:563-569
     if (.not. have_win) then
        norm_win = 1.8d0                            ! READ.F90:603-609
     else if (any(norm_win < 0.00001d0)) then
@@ -494,6 +533,11 @@ contains
     m%ebc = ebc
     m%mat_type = mat
     m%vinit = vinit
+    m%vinit_card = vinit
+    allocate(m%norm_win(n), m%body_id(n), m%elcon(8 * max(numel, 1)))
+    m%norm_win = norm_win
+    m%body_id = body_id
+    if (numel > 0) m%elcon = elcon(1:8 * numel)
     where (ebc == 1) m%vinit = 0.0d0
 
     m%ctrl%rk_degree = rk(1); m%ctrl%rk_cont = rk(2); m%ctrl%itype_int = itype

@@ -9,7 +9,13 @@
 !
 !   mega_drive --deck <Control.dat> <result.bin> [precision [nsteps [chunk]]]
 !       precision 0 = FP64 (default), 1 = mixed; nsteps 0 or absent = the
-!       reference's count (TIME += DLT until TIME > TIME_END, MAIN.F90:644-656)
+!       reference's count (TIME += DLT until TIME > TIME_END, MAIN.F90:644-656).
+!       With 'unformatted' among the output variables, frames MEGAOUT_*.vtk
+!       are written next to Control.dat on the reference's schedule: frame 0
+!       before the loop, then after each step where the time since the last
+!       frame exceeds TIME_OUTPUT (MAIN.F90:252-272,557-645).  At the end it
+!       prints the length (z extent), the radius of the nodes fixed in z and
+!       the volume-weighted L2 norm of the plastic strain.
 !   mega_drive <model.bin> <result.bin>
 !
 ! model.bin (stream, native endianness): int32 n, nsteps, chunk; real64 dt;
@@ -19,6 +25,7 @@ program mega_drive
   use iso_c_binding
   use mega_iface
   use mega_deck
+  use mega_vtk
   implicit none
   type(mega_control) :: ctrl
   type(deck_model) :: dm
@@ -28,7 +35,11 @@ program mega_drive
   real(c_double), allocatable :: x0(:), win(:), vol(:), mass(:), prop(:), vinit(:), x(:), eps(:)
   integer(c_int32_t), allocatable :: ebc(:), ebc_node(:), mat_type(:)
   character(len=4096) :: fin, fout, arg
-  integer :: done, ios
+  integer :: done, ios, j, frame
+  logical :: frames
+  logical, allocatable :: outstep(:)
+  real(c_double) :: tc, zlo, zhi, rad, num, den
+  type(frame_fields) :: ff
 
   call get_command_argument(1, arg)
   if (trim(arg) == '--deck') then
@@ -54,9 +65,8 @@ program mega_drive
         end do
      end if
      write(*, '(A,I0,A,I0,A,ES14.6,A,I0)') 'deck: ', n, ' nodes, ', dm%numel, ' elements, dt ', dt, ', steps ', nsteps
-     call move_alloc(dm%x0, x0); call move_alloc(dm%win, win); call move_alloc(dm%vol, vol)
-     call move_alloc(dm%mass, mass); call move_alloc(dm%ebc, ebc); call move_alloc(dm%ebc_node, ebc_node)
-     call move_alloc(dm%mat_type, mat_type); call move_alloc(dm%prop, prop); call move_alloc(dm%vinit, vinit)
+     x0 = dm%x0; win = dm%win; vol = dm%vol; mass = dm%mass; ebc = dm%ebc; ebc_node = dm%ebc_node
+     mat_type = dm%mat_type; prop = dm%prop; vinit = dm%vinit
   else
      fin = arg
      call get_command_argument(2, fout)
@@ -69,11 +79,41 @@ program mega_drive
   end if
 
   call mega_check(mega_create(ctrl, n, x0, win, vol, mass, ebc, ebc_node, mat_type, prop, vinit, ctx), 'mega_create')
+  frames = .false.
+  if (allocated(dm%norm_win)) frames = dm%unf
+  if (frames) then
+     call vtk_notice_skipped(dm)
+     allocate(outstep(nsteps))
+     tc = 0.0d0
+     do j = 1, nsteps          ! TIME_COUNTER: tested after the step, then advanced
+        outstep(j) = tc > dm%time_output
+        if (outstep(j)) tc = 0.0d0
+        tc = tc + dt
+     end do
+     call get_frame(.true.)
+     call vtk_frame(0, dm, ff)
+     frame = 1
+  end if
   done = 0
   do while (done < nsteps)
      k = min(chunk, nsteps - done)
+     if (frames) then          ! stop at the next frame
+        do j = done + 1, done + k
+           if (outstep(j)) then
+              k = j - done
+              exit
+           end if
+        end do
+     end if
      call mega_check(mega_step(ctx, dt, k), 'mega_step')
      done = done + k
+     if (frames) then
+        if (outstep(done)) then
+           call get_frame(.false.)
+           call vtk_frame(frame, dm, ff)
+           frame = frame + 1
+        end if
+     end if
      call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
      write(*, '(A,I8,A,3ES14.6)') 'step ', done, '  energies (int, kin, ext)', e_int, e_kin, e_ext
   end do
@@ -84,4 +124,50 @@ program mega_drive
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(11) x, eps
   close(11)
+  ! metrics: z extent, radius of the nodes fixed in z (the impact face), eqps L2
+  zlo = x(3); zhi = x(3); rad = 0.0d0; num = 0.0d0; den = 0.0d0
+  do j = 1, n
+     zlo = min(zlo, x(3 * j)); zhi = max(zhi, x(3 * j))
+     if (ebc(3 * j) == 1) rad = max(rad, sqrt(x(3 * j - 2) ** 2 + x(3 * j - 1) ** 2))
+     num = num + vol(j) * eps(j) ** 2
+     den = den + vol(j)
+  end do
+  write(*, '(A,ES24.16,A,ES24.16,A,ES24.16)') 'metrics: length ', zhi - zlo, ' radius ', rad, ' eqps_l2 ', &
+       sqrt(num / den)
+
+contains
+
+  ! the fields of one frame that the requested variables need; before the
+  ! first step the reference writes the generalized displacement, velocity
+  ! and acceleration (MAIN.F90:256-261)
+  subroutine get_frame(initial)
+    logical, intent(in) :: initial
+    if (.not. allocated(ff%x)) then
+       allocate(ff%x(3*n), ff%dsp(3*n), ff%vel(3*n), ff%acl(3*n), ff%fint(3*n), ff%stress(6*n), ff%strain(6*n), &
+                ff%eps(n), ff%strain_eq(n), ff%wave(n), ff%dstate(4*n))
+       ff%have_wave = ctrl%precision == MEGA_FP64
+       if (dm%outv(19) .and. .not. ff%have_wave) &
+            write(*, '(A)') 'output variable max_wave_vel is not computed in mixed precision; left out'
+    end if
+    call mega_check(mega_download(ctx, MEGA_F_X, ff%x), 'mega_download')
+    if (initial) then
+       if (dm%outv(2)) call mega_check(mega_download(ctx, MEGA_F_DTOT, ff%dsp), 'mega_download')
+       if (dm%outv(3)) call mega_check(mega_download(ctx, MEGA_F_V, ff%vel), 'mega_download')
+       if (dm%outv(4)) call mega_check(mega_download(ctx, MEGA_F_A, ff%acl), 'mega_download')
+    else
+       if (dm%outv(2)) call mega_check(mega_download(ctx, MEGA_F_UPHY, ff%dsp), 'mega_download')
+       if (dm%outv(3)) call mega_check(mega_download(ctx, MEGA_F_VPHY_STEP, ff%vel), 'mega_download')
+       if (dm%outv(4)) call mega_check(mega_download(ctx, MEGA_F_APHY, ff%acl), 'mega_download')
+    end if
+    if (dm%outv(5)) call mega_check(mega_download(ctx, MEGA_F_FINT, ff%fint), 'mega_download')
+    if (dm%outv(21)) call mega_check(mega_download(ctx, MEGA_F_STRESS, ff%stress), 'mega_download')
+    if (dm%outv(22)) then
+       call mega_check(mega_download(ctx, MEGA_F_STRAIN, ff%strain), 'mega_download')
+       call mega_check(mega_download(ctx, MEGA_F_STRAIN_EQ, ff%strain_eq), 'mega_download')
+    end if
+    if (dm%outv(23)) call mega_check(mega_download(ctx, MEGA_F_EPS, ff%eps), 'mega_download')
+    if (dm%outv(19) .and. ff%have_wave) call mega_check(mega_download(ctx, MEGA_F_WAVE_VEL, ff%wave), 'mega_download')
+    if (any(dm%outv(24:26))) call mega_check(mega_download(ctx, MEGA_F_DAMAGE_STATE, ff%dstate), 'mega_download')
+  end subroutine get_frame
+
 end program mega_drive

@@ -12,7 +12,8 @@ module mega_iface
   integer(c_int32_t), parameter :: MEGA_FP64 = 0, MEGA_MIXED = 1
   integer(c_int32_t), parameter :: MEGA_F_X = 0, MEGA_F_V = 1, MEGA_F_A = 2, MEGA_F_D = 3, MEGA_F_DTOT = 4, &
        MEGA_F_UPHY = 5, MEGA_F_FINT = 6, MEGA_F_FEXT = 7, MEGA_F_STRESS = 8, MEGA_F_STRAIN = 9, &
-       MEGA_F_EPS = 10, MEGA_F_STRAIN_EQ = 11, MEGA_F_WAVE_VEL = 12, MEGA_F_VPHY = 13, MEGA_F_DAMAGE_STATE = 14
+       MEGA_F_EPS = 10, MEGA_F_STRAIN_EQ = 11, MEGA_F_WAVE_VEL = 12, MEGA_F_VPHY = 13, MEGA_F_DAMAGE_STATE = 14, &
+       MEGA_F_APHY = 15, MEGA_F_VPHY_STEP = 16
 
   ! the CONTROL-module values the step reads (CONTROL.F90:27-95)
   type, bind(C) :: mega_control

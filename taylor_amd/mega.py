@@ -32,7 +32,7 @@ LIB_PATH = os.path.join(HERE, "libmega_hip.so")
 MEGA_FP64, MEGA_MIXED = 0, 1
 MLP_PARAMS, MLP_NORM = 9671, 59
 FIELDS = {"x": 0, "v": 1, "a": 2, "d": 3, "dtot": 4, "uphy": 5, "fint": 6, "fext": 7, "stress": 8,
-          "strain": 9, "eps": 10, "strain_eq": 11, "wave_vel": 12, "vphy": 13, "damage_state": 14}
+          "strain": 9, "eps": 10, "strain_eq": 11, "wave_vel": 12, "vphy": 13, "damage_state": 14, "aphy": 15, "vphy_step": 16}
 WIDTH = {"stress": 6, "strain": 6, "eps": 1, "strain_eq": 1, "wave_vel": 1, "damage_state": 4}
 
 

@@ -10,6 +10,7 @@ compressed .npz files.  The fixtures are data only.
     python tests/golden/make_golden.py --pd       # peridynamics vectors and run
     python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
     python tests/golden/make_golden.py --damage   # material types 6 and 3
+    python tests/golden/make_golden.py --vtk      # the reference's VTK frames of a small deck
 """
 from __future__ import annotations
 
@@ -298,6 +299,39 @@ def damage_vectors(ref, rng):
     return out
 
 
+# the deck of the VTK frame fixture: a small type-6 bar, frames every 5 steps
+VTK_VARS = ("unformatted", "node_id", "displacement", "velocity", "acceleration", "fint", "bodyid", "material_type",
+            "initial_coordinate", "init_velocity", "normalized_window", "nodal_volume", "nodal_mass",
+            "Poissons_ratio", "Youngs_modulus", "density", "physical_window", "max_wave_vel", "stress", "strain",
+            "eps", "eps_shear", "eps_vol", "dam")
+
+
+def vtk_deck(workdir):
+    """Write the frame fixture's deck into workdir; returns the problem."""
+    h, radius, length = 1.0e-3, 2.0e-3, 3.0e-3
+    mesh = deck.taylor_bar_mesh(h, radius=radius, length=length, seed=0)
+    prob = deck.taylor_problem(h, radius=radius, length=length, seed=0, material=damage_material(6))
+    kfile = os.path.join(workdir, "bar.k")
+    deck.write_k(kfile, mesh)
+    prob.time_end = prob.dt * 11.5
+    deck.write_control(os.path.join(workdir, "Control.dat"), kfile, prob, ["BAR", "WALL"],
+                       time_output=prob.dt * 3.5, out_vars="\n".join(VTK_VARS))
+    return prob
+
+
+def vtk_main():
+    with tempfile.TemporaryDirectory() as wd:
+        vtk_deck(wd)
+        with open(os.path.join(wd, "mega.out"), "w") as fo:
+            subprocess.run([REF_MEGA, "-j", "1"], cwd=wd, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
+                           check=True, timeout=600)
+        frames = sorted(glob.glob(os.path.join(wd, "MEGAOUT_*.vtk")))
+        np.savez_compressed(os.path.join(HERE, "vtk_frames.npz"),
+                            names=np.array([os.path.basename(f) for f in frames]),
+                            texts=np.array([open(f).read() for f in frames]))
+        print([os.path.basename(f) for f in frames])
+
+
 def damage_main():
     ref = load_ref()
     rng = np.random.default_rng(20260304)
@@ -362,6 +396,9 @@ def main():
         return
     if "--damage" in sys.argv:
         damage_main()
+        return
+    if "--vtk" in sys.argv:
+        vtk_main()
         return
     ref = load_ref()
     rng = np.random.default_rng(20260101)

@@ -113,3 +113,66 @@ def test_fortran_host_reference_input_reaches_the_reference_final_state(tmp_path
     R0 = np.sqrt(z["x"][wall, 0] ** 2 + z["x"][wall, 1] ** 2).max()
     print(f"length rel {abs(L - L0) / L0:.2e}, radius rel {abs(R - R0) / R0:.2e}")
     assert abs(L - L0) <= 1e-8 * L0 and abs(R - R0) <= 1e-8 * R0
+
+
+def _numbers(line):
+    try:
+        return [float(t) for t in line.split()]
+    except ValueError:
+        return None
+
+
+def compare_frames(ours, ref, rtol):
+    """Line by line: text records equal; numeric records within rtol of the
+    largest magnitude of their block (integers exactly).  Returns the largest
+    relative difference seen."""
+    lo, lr = ours.splitlines(), ref.splitlines()
+    assert len(lo) == len(lr), (len(lo), len(lr))
+    scale, block, blk_of = {}, "POINTS", []
+    for line in lr:
+        if line.startswith("SCALARS ") or line.startswith("POINTS ") or line.startswith("CELLS") or \
+                line.startswith("CELL_TYPES"):
+            block = line.split()[1] if line.startswith("SCALARS ") else line.split()[0]
+        blk_of.append(block)
+        v = _numbers(line)
+        if v:
+            scale[block] = max(scale.get(block, 0.0), max(abs(x) for x in v))
+    worst = 0.0
+    for a, b, blk in zip(lo, lr, blk_of):
+        va, vb = _numbers(a), _numbers(b)
+        if va is None or vb is None or not vb:
+            assert a.rstrip() == b.rstrip(), (a, b)
+            continue
+        assert len(va) == len(vb), (a, b)
+        if all(("." not in t and "E" not in t) for t in b.split()):
+            assert va == vb, (blk, a, b)
+            continue
+        s = max(scale.get(blk, 0.0), 1e-300)
+        d = max(abs(x - y) for x, y in zip(va, vb)) / s
+        assert d <= rtol, (blk, a, b, d)
+        worst = max(worst, d)
+    return worst
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVE), reason="Fortran host not built")
+def test_fortran_host_writes_the_reference_frames(tmp_path):
+    """With 'unformatted' among the output variables the Fortran host writes
+    the reference's VTK frames on the reference's schedule: against the
+    reference program's own frames of the same deck
+    (tests/golden/vtk_frames.npz), every text record equal and every value
+    within 1e-9 of its field's largest magnitude (FP64)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import vtk_deck
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "vtk_frames.npz"))
+    vtk_deck(str(tmp_path))
+    out = subprocess.run([DRIVE, "--deck", str(tmp_path / "Control.dat"), str(tmp_path / "r.bin")],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout[-600:])
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "metrics: length" in out.stdout
+    for name, text in zip(z["names"], z["texts"]):
+        ours = open(tmp_path / str(name)).read()
+        worst = compare_frames(ours, str(text), 1e-9)
+        print(name, "largest relative difference", worst)
+    assert not os.path.exists(tmp_path / "MEGAOUT_0003.vtk")
