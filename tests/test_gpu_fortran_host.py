@@ -165,12 +165,22 @@ def test_fortran_host_writes_the_reference_frames(tmp_path):
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
     from make_golden import vtk_deck
     z = np.load(os.path.join(os.path.dirname(__file__), "golden", "vtk_frames.npz"))
-    vtk_deck(str(tmp_path))
+    prob = vtk_deck(str(tmp_path))
     out = subprocess.run([DRIVE, "--deck", str(tmp_path / "Control.dat"), str(tmp_path / "r.bin")],
                          capture_output=True, text=True, timeout=120)
     print(out.stdout[-600:])
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "metrics: length" in out.stdout
+    # the metrics line against the same quantities taken from the result file
+    n = prob.vol.shape[0]
+    res = np.fromfile(tmp_path / "r.bin", dtype=np.float64)
+    x, eps = res[:3 * n].reshape(n, 3), res[3 * n:]
+    fixed = prob.ebc[:, 2] == 1
+    want = (x[:, 2].max() - x[:, 2].min(), np.hypot(x[fixed, 0], x[fixed, 1]).max(),
+            np.sqrt((prob.vol * eps ** 2).sum() / prob.vol.sum()))
+    line = [l for l in out.stdout.splitlines() if l.startswith("metrics: length")][0].split()
+    got = (float(line[2]), float(line[4]), float(line[6]))
+    print("metrics", got, want)
+    assert np.allclose(got, want, rtol=1e-13, atol=0) and want[2] > 0
     for name, text in zip(z["names"], z["texts"]):
         ours = open(tmp_path / str(name)).read()
         worst = compare_frames(ours, str(text), 1e-9)
