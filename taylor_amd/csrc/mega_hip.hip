@@ -13,7 +13,7 @@
 //              rotation, elastic trial, J2 radial return, state write-back,
 //              work, and P = V J sigma F^-T
 //                                            FINT.F90:758-870; CONSTITUTION.F90:18-89
-//   k_force    kGroup lanes per node over the transposed neighbor list:
+//   k_force    kFGroup lanes per node over the transposed neighbor list:
 //              f_j = sum_i P_i grad phi_ij (= B^T sigma V J scattered by the
 //              reference, FINT.F90:997-1116, gathered here without atomics,
 //              deterministic), then EBC + corrector (MAIN.F90:432-488), energies
@@ -36,11 +36,16 @@
 // run, else grouped by the source's bin) carry {grad phi, source} records
 // (TPair) in transposed order.
 // Pair kernels map blocks to XCDs in contiguous node ranges so that each
-// XCD's L2 holds the neighbor data its blocks gather.
+// XCD's L2 holds the neighbor data its blocks gather.  For lists that stay
+// for the run (Lagrangian), each block of consecutive (Morton-ordered) nodes
+// has its set of distinct sources (k_blk_set, about a quarter of its list
+// entries): k_force stages their P rows and the mixed k_gather their
+// increments in LDS once, and the list entries name sources by set position.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -68,19 +73,41 @@ namespace {
 #define MEGA_RK_CACHE 4
 #endif
 static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one neighbor per lane");
+#ifndef MEGA_SET_CAP
+#define MEGA_SET_CAP 512
+#endif
 #ifndef MEGA_UNROLL_MIX
-#define MEGA_UNROLL_MIX 6
+#define MEGA_UNROLL_MIX 4
 #endif
 #ifndef MEGA_UNROLL_F64
 #define MEGA_UNROLL_F64 4
 #endif
+#ifndef MEGA_FGROUP
+#define MEGA_FGROUP 8
+#endif
+#ifndef MEGA_FUNROLL_MIX
+#define MEGA_FUNROLL_MIX 5
+#endif
+#ifndef MEGA_FUNROLL_F64
+#define MEGA_FUNROLL_F64 5
+#endif
 constexpr int kBlock = 256;
 constexpr int kGroup = MEGA_KGROUP;   // lanes cooperating on one node's neighbor list
+constexpr int kFGroup = MEGA_FGROUP;  // the same for the force gather over the transposed list
 constexpr int kRkGroup = MEGA_RK_GROUP;   // the same for the shape-function kernel
 constexpr int kRkCache = MEGA_RK_CACHE;   // kernel evaluations per lane kept between its two passes
 constexpr int kXcd = 8;               // XCDs on MI355X
 // neighbor-list entries each lane keeps in flight in the gathers
 constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
+constexpr int kFUnrollMix = MEGA_FUNROLL_MIX, kFUnrollF64 = MEGA_FUNROLL_F64;
+constexpr int kNodesPerBlock = kBlock / kGroup;    // nodes of one block in the gathers
+constexpr int kNodesPerFBlock = kBlock / kFGroup;  // nodes of one block in the force gather
+static_assert(kFGroup <= 16 && kGroup <= 16, "groups inside one DPP row");
+// Source sets: a block stages the values its rows gather (P rows, increments)
+// once in LDS when it has at most kSetCap distinct sources.
+constexpr int kSetCap = MEGA_SET_CAP;
+constexpr int kSetSort = 4096;   // list entries of one block the set builder sorts in LDS
+static_assert(kSetCap <= 65535, "set positions are 16-bit");
 
 // ---------------------------------------------------------------------------
 // kernel arguments
@@ -102,6 +129,10 @@ struct Params {
     int64_t cap;                                     // component stride of the pair arrays
     // lists
     const int32_t *row_ptr, *col, *trow, *tsrc, *tpos;
+    // per-block source sets of lists that stay for the run (null: the gathers
+    // read global memory): set of block b at [ptr[b], ptr[b+1]) of set
+    const int32_t *gset_ptr, *gset, *fset_ptr, *fset;
+    const uint16_t *gloc;                            // forward slot -> position in its block's set
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
     // node arrays, FP64 masters
@@ -535,10 +566,24 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr bool DIR = FM == F_DIRECT;
     constexpr int U = MIX ? kUnrollMix : kUnrollF64;
+    extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
+    const int node = blk * kNodesPerBlock + threadIdx.x / kGroup;
     const int lane = threadIdx.x & (kGroup - 1);
+    // mixed, lists that stay: the block's distinct neighbors' d (and dtot) in LDS
+    int m = 0;
+    float4 *Ld = reinterpret_cast<float4 *>(mega_dyn);
+    if (MIX && p.gset_ptr) {
+        const int s0 = p.gset_ptr[blk];
+        m = p.gset_ptr[blk + 1] - s0;
+        for (int k = threadIdx.x; k < m; k += kBlock) {
+            const int j = p.gset[s0 + k];
+            Ld[k] = p.d4[j];
+            if (DIR) Ld[m + k] = p.t4[j];
+        }
+        if (m > 0) __syncthreads();
+    }
     if (node >= p.n) return;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
     const R *__restrict__ shp = as<R>(p.shp);
@@ -555,12 +600,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
         di = p.d4[node];
         if (DIR) ti = p.t4[node];
     }
+    // LDS: neighbors by their position in the block's set
+    auto sweep = [&](auto lds_tag) {
+    constexpr bool LDS = decltype(lds_tag)::value;
     for (int base = p0 + lane; base < p1; base += kGroup * U) {
         int j[U];
 #pragma unroll
         for (int e = 0; e < U; ++e) {
             const int q = base + kGroup * e;
-            j[e] = q < p1 ? p.col[q] : node;
+            if (LDS) j[e] = q < p1 ? (int)p.gloc[q] : 0;
+            else j[e] = q < p1 ? p.col[q] : node;
         }
         R g0[U], g1[U], g2[U], f[U], dj[U][3], tj[U][3];
 #pragma unroll
@@ -572,10 +621,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
             g2[e] = ok ? gz[q] : R(0);
             f[e] = (UPDATE_X && ok) ? shp[q] : R(0);
             if (MIX) {
-                const float4 dv = p.d4[j[e]];
+                const float4 dv = LDS ? Ld[j[e]] : p.d4[j[e]];
                 dj[e][0] = dv.x - di.x; dj[e][1] = dv.y - di.y; dj[e][2] = dv.z - di.z;
                 if (DIR) {
-                    const float4 tv = p.t4[j[e]];
+                    const float4 tv = LDS ? Ld[m + j[e]] : p.t4[j[e]];
                     tj[e][0] = tv.x - ti.x; tj[e][1] = tv.y - ti.y; tj[e][2] = tv.z - ti.z;
                 }
             } else {
@@ -597,6 +646,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
             if (MIX) { S[0] += g0[e]; S[1] += g1[e]; S[2] += g2[e]; }
         }
     }
+    };
+    if (m > 0) sweep(std::true_type{});
+    else sweep(std::false_type{});
 #pragma unroll
     for (int k = 0; k < 9; ++k) L[k] = group_sum<kGroup>(L[k]);
     if (DIR) {
@@ -974,34 +1026,67 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
 {
     __shared__ double s_red[kBlock / 64];
     constexpr bool MIX = std::is_same<R, float>::value;
-    constexpr int U = MIX ? kUnrollMix : kUnrollF64;
+    constexpr int U = MIX ? kFUnrollMix : kFUnrollF64;
+    constexpr int PV = MIX ? 3 : 5;   // 16 B pieces of a P row (pstride values)
+    using V = typename std::conditional<MIX, float4, double2>::type;
+    extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
-    const int lane = threadIdx.x & (kGroup - 1);
+    const int node = blk * kNodesPerFBlock + threadIdx.x / kFGroup;
+    const int lane = threadIdx.x & (kFGroup - 1);
     const bool live = node < p.n;
+    // lists that stay: the P rows of the block's distinct sources in LDS, and
+    // the src field of each record is the source's position in that set
+    int m = 0;
+    V *Lp = reinterpret_cast<V *>(mega_dyn);
+    if (p.fset_ptr) {
+        const int s0 = p.fset_ptr[blk];
+        m = p.fset_ptr[blk + 1] - s0;
+        const V *Pg = reinterpret_cast<const V *>(p.P);
+        for (int k = threadIdx.x; k < PV * m; k += kBlock) {
+            const int r = k / PV;
+            Lp[k] = Pg[PV * (int64_t)p.fset[s0 + r] + (k - PV * r)];
+        }
+        if (m > 0) __syncthreads();
+    }
     R f0 = R(0), f1 = R(0), f2 = R(0), fe = R(0);
-    if (live) {
+    auto sweep = [&](auto lds_tag) {
+        constexpr bool LDS = decltype(lds_tag)::value;
         const int t0 = p.trow[node], t1 = p.trow[node + 1];
         const TPair<R> *__restrict__ tp = as<const TPair<R>>(p.tpair);
         const R *__restrict__ Pb = as<R>(p.P);
         const R *__restrict__ ts = as<R>(p.tshp);
-        for (int base = t0 + lane; base < t1; base += kGroup * U) {
+        for (int base = t0 + lane; base < t1; base += kFGroup * U) {
             R g0[U], g1[U], g2[U], P[U][10], sh[U];
             int src[U];
 #pragma unroll
             for (int e = 0; e < U; ++e) {
-                const int t = base + kGroup * e;
+                const int t = base + kFGroup * e;
                 const bool ok = t < t1;
                 const TPair<R> v = tp[ok ? t : t0];
                 g0[e] = ok ? v.gx : R(0);
                 g1[e] = ok ? v.gy : R(0);
                 g2[e] = ok ? v.gz : R(0);
-                src[e] = ok ? v.src : node;
+                src[e] = ok ? v.src : (LDS ? 0 : node);
                 sh[e] = (ok && p.has_gravity) ? ts[t] : R(0);
             }
 #pragma unroll
-            for (int e = 0; e < U; ++e) load_P<R>(Pb, src[e], P[e]);
+            for (int e = 0; e < U; ++e) {
+                if constexpr (LDS) {
+                    const V *q = Lp + PV * src[e];
+                    R *o = P[e];
+                    if constexpr (MIX) {
+                        const V a = q[0], b = q[1], c = q[2];
+                        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z;
+                        o[7] = b.w; o[8] = c.x; o[9] = c.y;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) { const V v2 = q[k]; o[2 * k] = v2.x; o[2 * k + 1] = v2.y; }
+                    }
+                } else {
+                    load_P<R>(Pb, src[e], P[e]);
+                }
+            }
 #pragma unroll
             for (int e = 0; e < U; ++e) {
                 f0 += P[e][0] * g0[e] + P[e][3] * g1[e] + P[e][6] * g2[e];
@@ -1010,9 +1095,13 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
                 fe += sh[e] * P[e][9];
             }
         }
+    };
+    if (live) {
+        if (m > 0) sweep(std::true_type{});
+        else sweep(std::false_type{});
     }
-    f0 = group_sum<kGroup>(f0); f1 = group_sum<kGroup>(f1); f2 = group_sum<kGroup>(f2);
-    if (p.has_gravity) fe = group_sum<kGroup>(fe);
+    f0 = group_sum<kFGroup>(f0); f1 = group_sum<kFGroup>(f1); f2 = group_sum<kFGroup>(f2);
+    if (p.has_gravity) fe = group_sum<kFGroup>(fe);
     double ke = 0.0, ie = 0.0, ee = 0.0;
     if (live && lane < 3) {
         const double f = (double)(lane == 0 ? f0 : (lane == 1 ? f1 : f2));
@@ -1506,6 +1595,88 @@ __global__ void k_tbin_start(int n, const int32_t *trow, int32_t *tbin)
 }
 
 // non-empty bins, in bin order
+// Per-block source sets of lists that stay for the run.  The pair kernels give
+// block b the npb nodes from b * npb; the distinct
+// sources of their rows (in Morton node order a small fraction of the
+// entries) are staged once per launch in LDS.  One workgroup per block sorts
+// the block's entries (bitonic, in LDS) and keeps the distinct values.
+// COUNT: the set size, or 0 when the block has more than cap distinct sources
+// or more than kSetSort entries (such blocks gather from global memory), and
+// the largest size in *maxm.  FILL: the set in ascending order at set[setp[b]]
+// and each entry's position in it: loc[q] for forward lists, the src field of
+// the transposed record otherwise.
+template <bool FILL, class R>
+__global__ __launch_bounds__(kBlock) void k_blk_set(int n, int npb, const int32_t *rowp, const int32_t *vals, int cap,
+                                                    int32_t *cnt, int32_t *maxm, const int32_t *setp, int32_t *set,
+                                                    uint16_t *loc, TPair<R> *tp)
+{
+    __shared__ int32_t key[kSetSort];
+    __shared__ int32_t uq[kSetSort];
+    __shared__ int32_t part[kBlock];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int r0 = b * npb, r1 = min(r0 + npb, n);
+    const int q0 = rowp[r0], q1 = rowp[r1], len = q1 - q0;
+    if (len > kSetSort) {
+        if (!FILL && tid == 0) cnt[b] = 0;
+        return;
+    }
+    int np2 = 2 * kBlock;
+    while (np2 < len) np2 <<= 1;
+    for (int k = tid; k < np2; k += kBlock) key[k] = k < len ? vals[q0 + k] : INT_MAX;
+    __syncthreads();
+    for (int size = 2; size <= np2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = tid; k < np2 / 2; k += kBlock) {
+                const int i = 2 * stride * (k / stride) + (k & (stride - 1)), j = i + stride;
+                const int a = key[i], c = key[j];
+                if ((a > c) == ((i & size) == 0)) { key[i] = c; key[j] = a; }
+            }
+            __syncthreads();
+        }
+    // thread tid owns the sorted chunk [tid C, tid C + C)
+    const int C = np2 / kBlock;
+    int own = 0;
+    for (int k = tid * C; k < tid * C + C; ++k) own += (k < len && (k == 0 || key[k] != key[k - 1])) ? 1 : 0;
+    part[tid] = own;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {   // inclusive scan of the chunk counts
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    const int m = part[kBlock - 1];
+    const bool fits = m <= cap;
+    if (!FILL) {
+        if (tid == 0) {
+            cnt[b] = fits ? m : 0;
+            if (fits) atomicMax(maxm, m);
+        }
+        return;
+    }
+    if (!fits) return;
+    const int s0 = setp[b];
+    int w = part[tid] - own;
+    for (int k = tid * C; k < tid * C + C; ++k)
+        if (k < len && (k == 0 || key[k] != key[k - 1])) {
+            uq[w] = key[k];
+            set[s0 + w] = key[k];
+            ++w;
+        }
+    __syncthreads();
+    for (int q = q0 + tid; q < q1; q += kBlock) {
+        const int v = vals[q];
+        int lo = 0, hi = m - 1;   // v is in uq[0, m)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (uq[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        if (loc) loc[q] = (uint16_t)lo;
+        else tp[q].src = lo;
+    }
+}
+
 __global__ void k_cells_ne(int64_t ncell, const int32_t *cell_start, const int32_t *pos, int32_t *cells)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1606,6 +1777,11 @@ struct mega_ctx {
     uint64_t *smask = nullptr; // [kMaskCap n] search ballot masks, COUNT -> FILL
     int32_t *counts = nullptr;
     int64_t nnz = 0, nnz_cap = 0;
+    // per-block source sets of lists that stay (k_blk_set); gather sets in mixed precision only
+    int32_t *gset_ptr = nullptr, *gset = nullptr, *fset_ptr = nullptr, *fset = nullptr, *set_max = nullptr;
+    uint16_t *gloc = nullptr;
+    int64_t gset_cap = 0, fset_cap = 0, gloc_cap = 0;
+    int sets_on = 0, gset_max = 0, fset_max = 0;
     // bins
     BinGeom geom{};
     int64_t ncell = 0;
@@ -1675,6 +1851,10 @@ Params params(mega_ctx *c)
     p.n = c->n;
     p.cap = c->nnz_cap;
     p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
+    if (c->sets_on) {
+        p.fset_ptr = c->fset_ptr; p.fset = c->fset;
+        if (c->mixed) { p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc; }
+    }
     p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
@@ -1719,9 +1899,9 @@ struct Scope {
 unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
 
 // grid for the XCD-mapped pair kernels (kBlock/kGroup nodes per block)
-unsigned xcd_grid(int n, int *nblocks)
+unsigned xcd_grid(int n, int *nblocks, int npb = kNodesPerBlock)
 {
-    *nblocks = (n + kBlock / kGroup - 1) / (kBlock / kGroup);
+    *nblocks = (n + npb - 1) / npb;
     const int per = (*nblocks + kXcd - 1) / kXcd;
     return (unsigned)(per * kXcd);
 }
@@ -1843,6 +2023,7 @@ int do_search(mega_ctx *c, const double *x)
     HIPCHK(hipGetLastError());
     c->has_lists = 1;
     c->has_shapes = 0;
+    c->sets_on = 0;
     return MEGA_OK;
 }
 
@@ -1860,6 +2041,73 @@ int search_step(mega_ctx *c, const double *x, int first)
     return do_search(c, x);
 }
 
+// one list's sets: rowp/vals the CSR, loc (forward) or the records (transposed)
+template <class R>
+int build_set(mega_ctx *c, int npb, const int32_t *rowp, const int32_t *vals, int32_t *setp, int32_t **set,
+              int64_t *set_cap, uint16_t *loc, TPair<R> *tp, int *maxm)
+{
+    const int nb = (c->n + npb - 1) / npb;
+    HIPCHK(hipMemsetAsync(c->set_max, 0, sizeof(int32_t), c->stream));
+    HIPCHK(hipMemsetAsync(c->counts + nb, 0, sizeof(int32_t), c->stream));
+    k_blk_set<false, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, kSetCap, c->counts, c->set_max,
+                                                      nullptr, nullptr, nullptr, nullptr);
+    int r = cub_scan(c, c->counts, setp, nb + 1);
+    if (r) return r;
+    int32_t tot = 0, mx = 0;
+    HIPCHK(hipMemcpyAsync(&tot, setp + nb, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&mx, c->set_max, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tot > *set_cap) {
+        (void)hipFree(*set);
+        *set = nullptr;
+        *set_cap = 0;
+        if (dalloc(set, tot)) return MEGA_ERR_NOMEM;
+        *set_cap = tot;
+    }
+    k_blk_set<true, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, kSetCap, nullptr, nullptr, setp, *set,
+                                                     loc, tp);
+    HIPCHK(hipGetLastError());
+    *maxm = mx;
+    return MEGA_OK;
+}
+
+// source sets of lists that stay for the run (the Lagrangian RK lists): the
+// transposed sets for the force, in mixed precision the forward sets for the
+// gather; after the records are written, as FILL rewrites their src field
+int build_sets(mega_ctx *c)
+{
+    c->sets_on = 0;
+    if (!c->tq || c->pd || c->nnz == 0) return MEGA_OK;
+    const int nbf = (c->n + kNodesPerFBlock - 1) / kNodesPerFBlock, nbg = (c->n + kNodesPerBlock - 1) / kNodesPerBlock;
+    int e = 0;
+    if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nbf + 1);
+    if (!c->set_max) e |= dalloc(&c->set_max, 1);
+    if (c->mixed && !c->gset_ptr) e |= dalloc(&c->gset_ptr, nbg + 1);
+    if (c->mixed && c->gloc_cap < c->nnz) {
+        (void)hipFree(c->gloc);
+        c->gloc = nullptr;
+        c->gloc_cap = 0;
+        e |= dalloc(&c->gloc, c->nnz_cap);
+        if (!e) c->gloc_cap = c->nnz_cap;
+    }
+    if (e) return MEGA_ERR_NOMEM;
+    Scope s(c, "k_blk_set");
+    int r;
+    if (c->mixed) {
+        r = build_set<float>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
+                             static_cast<TPair<float> *>(c->tpair), &c->fset_max);
+        if (!r)
+            r = build_set<float>(c, kNodesPerBlock, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
+                                 nullptr, &c->gset_max);
+    } else {
+        r = build_set<double>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
+                              static_cast<TPair<double> *>(c->tpair), &c->fset_max);
+    }
+    if (r) return r;
+    c->sets_on = 1;
+    return MEGA_OK;
+}
+
 int do_shapes(mega_ctx *c, const double *xs)
 {
     if (!c->has_lists) return MEGA_ERR_ARG;
@@ -1870,6 +2118,8 @@ int do_shapes(mega_ctx *c, const double *xs)
     if (c->mixed) k_rk<float><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
     else k_rk<double><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
     HIPCHK(hipGetLastError());
+    const int r = build_sets(c);
+    if (r) return r;
     c->has_shapes = 1;
     return MEGA_OK;
 }
@@ -1882,8 +2132,9 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
     const unsigned g = xcd_grid(c->n, &nb);
     {
         Scope s(c, "k_gather");
-        if (update_x) k_gather<R, FM, true><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else k_gather<R, FM, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        const size_t lds = p.gset_ptr ? (size_t)c->gset_max * (FM == F_DIRECT ? 32 : 16) : 0;
+        if (update_x) k_gather<R, FM, true><<<g, kBlock, lds, c->stream>>>(p, nb);
+        else k_gather<R, FM, false><<<g, kBlock, lds, c->stream>>>(p, nb);
     }
     {
         Scope s(c, "k_node");
@@ -1896,9 +2147,12 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
     }
     if (force_mode >= 0) {
         Scope s(c, "k_force");
-        if (force_mode == 0) k_force<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else if (force_mode == 1) k_force<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else k_force<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+        int nbf = 0;
+        const unsigned gf = xcd_grid(c->n, &nbf, kNodesPerFBlock);
+        const size_t lds = p.fset_ptr ? (size_t)c->fset_max * 16 * (std::is_same<R, float>::value ? 3 : 5) : 0;
+        if (force_mode == 0) k_force<R, false, false><<<gf, kBlock, lds, c->stream>>>(p, nbf);
+        else if (force_mode == 1) k_force<R, true, false><<<gf, kBlock, lds, c->stream>>>(p, nbf);
+        else k_force<R, true, true><<<gf, kBlock, lds, c->stream>>>(p, nbf);
     }
     HIPCHK(hipGetLastError());
     if (FM == F_DIRECT) c->F_valid = 1;
@@ -2220,7 +2474,12 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->smask, kMaskCap * (int64_t)n);
     e |= dalloc(&c->cell_flag, c->ncell + 1);
     c->nb_node = (int)grid_for(n);
-    xcd_grid(n, &c->nb_force);
+    {   // energy partials: one per block of the force gather (RK) or of the peridynamic force
+        int nbf = 0, nbg = 0;
+        xcd_grid(n, &nbf, kNodesPerFBlock);
+        xcd_grid(n, &nbg);
+        c->nb_force = std::max(nbf, nbg);
+    }
     e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
     e |= dalloc(&c->flags, 4);
     e |= dalloc(&c->sstat, 4);
@@ -2263,7 +2522,8 @@ int mega_destroy(mega_ctx *c)
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
-                    c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask};
+                    c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
+                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
