@@ -133,6 +133,7 @@ struct Params {
     // read global memory): set of block b at [ptr[b], ptr[b+1]) of set
     const int32_t *gset_ptr, *gset, *fset_ptr, *fset;
     const uint16_t *gloc;                            // forward slot -> position in its block's set
+    const float4 *fpair;                             // mixed, with gloc: {grad phi, phi} per forward slot
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
     // node arrays, FP64 masters
@@ -616,10 +617,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
         for (int e = 0; e < U; ++e) {
             const int q = base + kGroup * e;
             const bool ok = q < p1;
-            g0[e] = ok ? gx[q] : R(0);
-            g1[e] = ok ? gy[q] : R(0);
-            g2[e] = ok ? gz[q] : R(0);
-            f[e] = (UPDATE_X && ok) ? shp[q] : R(0);
+            if (LDS) {   // mixed, lists that stay: the packed record
+                const float4 r = ok ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                g0[e] = (R)r.x; g1[e] = (R)r.y; g2[e] = (R)r.z;
+                f[e] = UPDATE_X ? (R)r.w : R(0);
+            } else {
+                g0[e] = ok ? gx[q] : R(0);
+                g1[e] = ok ? gy[q] : R(0);
+                g2[e] = ok ? gz[q] : R(0);
+                f[e] = (UPDATE_X && ok) ? shp[q] : R(0);
+            }
             if (MIX) {
                 const float4 dv = LDS ? Ld[j[e]] : p.d4[j[e]];
                 dj[e][0] = dv.x - di.x; dj[e][1] = dv.y - di.y; dj[e][2] = dv.z - di.z;
@@ -702,18 +709,18 @@ __device__ __forceinline__ int lanes_below(uint64_t m)
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// per wave: normalised inputs of its candidate lanes, then the outputs
 struct SurrLds {
-    mlp::bf16x8 tiles[mlp::kTilesFwd * 64];
-    float bias[mlp::kBiasPack];
-    float norm[mlp::kNorm];
-    float stage[kBlock / 64][64][16];   // per wave: normalised inputs, then outputs
+    float stage[kBlock / 64][64][16];
 };
 
 // cand: this lane's J2 trial state is plastic and belongs to the trained
 // material.  Inside the training envelope the network gives the plastic
 // correction; the answer is kept when the residual of the return-map
 // equations (mega::return_residual) is within tol and eps does not decrease,
-// otherwise the caller runs the exact return map.
+// otherwise the caller runs the exact return map.  Each wave works alone:
+// only a wave with a lane inside the envelope runs the network, reading the
+// packed weights (22 KB, cache-resident) from global memory.
 __device__ bool surr_return(const Params &p, bool cand, const float *der, const float *sr, const float *pred,
                             float &ep, float *s_out, const Material &mt)
 {
@@ -734,21 +741,15 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
         atomicAdd(&p.sstat[1], (unsigned long long)(__popcll(mc) - __popcll(mu)));
         atomicAdd(&p.sstat[2], (unsigned long long)__popcll(mu));
     }
-    if (!__syncthreads_or(inenv)) return false;
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(p.surr_tiles);
-        uint4 *dst = reinterpret_cast<uint4 *>(L.tiles);
-        for (int k = threadIdx.x; k < mlp::kTilesFwd * 64; k += kBlock) dst[k] = src[k];
-        for (int k = threadIdx.x; k < mlp::kBiasPack; k += kBlock) L.bias[k] = p.surr_bias[k];
-        for (int k = threadIdx.x; k < mlp::kNorm; k += kBlock) L.norm[k] = p.surr_norm[k];
-    }
-    __syncthreads();
+    if (!mu) return false;
+    const mlp::bf16x8 *tiles = reinterpret_cast<const mlp::bf16x8 *>(p.surr_tiles);
+    const float *norm = p.surr_norm;
     const int nc = __popcll(mu);
     const int slot = lanes_below(mu);
     float(*st)[16] = L.stage[wv];
     if (inenv) {
 #pragma unroll
-        for (int k = 0; k < mlp::kIn; ++k) st[slot][k] = (x[k] - L.norm[mlp::kInMean + k]) * L.norm[mlp::kInScale + k];
+        for (int k = 0; k < mlp::kIn; ++k) st[slot][k] = (x[k] - norm[mlp::kInMean + k]) * norm[mlp::kInScale + k];
         st[slot][13] = st[slot][14] = st[slot][15] = 0.0f;
     }
     wave_sync();
@@ -762,14 +763,14 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
             x0 = mlp::pack8(mlp::f32x4{a.x, a.y, a.z, a.w}, mlp::f32x4{b.x, b.y, b.z, b.w});
         }
         mlp::f32x4 z1[4], z2[4], z3[4], y;
-        mlp::forward16(L.tiles, L.bias, x0, lane, z1, z2, z3, y);
+        mlp::forward16(tiles, p.surr_bias, x0, lane, z1, z2, z3, y);
         wave_sync();
         if (g < 2 && s < nc) *reinterpret_cast<float4 *>(&st[s][4 * g]) = float4{y[0], y[1], y[2], y[3]};
     }
     wave_sync();
     bool ok = false;
     if (inenv) {
-        const float *os = L.norm + mlp::kOutScale;
+        const float *os = norm + mlp::kOutScale;
         float s[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) s[k] = pred[k] - os[k] * st[slot][k];
@@ -824,7 +825,7 @@ __device__ void harvest_put(const Params &p, bool h, const R *der, const R *s_in
 // store nothing, so that every lane reaches the wave-wide sections.
 
 template <class R, int FM, bool SURR>
-__global__ __launch_bounds__(kBlock) void k_node(Params p)
+__device__ __forceinline__ void node_update(const Params &p)
 {
     __shared__ double s_red[kBlock / 64];
     constexpr bool MIX = std::is_same<R, float>::value;
@@ -968,6 +969,19 @@ __global__ __launch_bounds__(kBlock) void k_node(Params p)
     if (threadIdx.x == 0) p.wpart[blockIdx.x] += ws;
 }
 
+template <class R, int FM>
+__global__ __launch_bounds__(kBlock) void k_node(Params p)
+{
+    node_update<R, FM, false>(p);
+}
+
+// with the learned return map (mixed precision)
+template <int FM>
+__global__ __launch_bounds__(kBlock) void k_node_surr(Params p)
+{
+    node_update<float, FM, true>(p);
+}
+
 // ---------------------------------------------------------------------------
 // force gather over the transposed lists, then (CORRECT) EBC, damping and the
 // corrector, energies, and (PREDICT) the next predictor, per degree of freedom.
@@ -1103,7 +1117,26 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
     f0 = group_sum<kFGroup>(f0); f1 = group_sum<kFGroup>(f1); f2 = group_sum<kFGroup>(f2);
     if (p.has_gravity) fe = group_sum<kFGroup>(fe);
     double ke = 0.0, ie = 0.0, ee = 0.0;
-    if (live && lane < 3) {
+    if constexpr (MIX) {
+        // the block's degrees of freedom on consecutive lanes: the corrector's
+        // node arrays are then read and written in one contiguous run per block
+        static_assert(3 * kNodesPerFBlock <= kBlock, "one lane per degree of freedom of the block");
+        __shared__ double s_f[3 * kNodesPerFBlock], s_fe[kNodesPerFBlock];
+        const int loc = threadIdx.x / kFGroup;
+        if (lane == 0) {
+            s_f[3 * loc] = (double)f0; s_f[3 * loc + 1] = (double)f1; s_f[3 * loc + 2] = (double)f2;
+            s_fe[loc] = (double)fe;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3 * kNodesPerFBlock) {
+            const int dn = threadIdx.x / 3, k = threadIdx.x - 3 * dn;
+            const int nd = blk * kNodesPerFBlock + dn;
+            if (nd < p.n) {
+                const double fx = p.has_gravity ? s_fe[dn] * p.grav[k] : 0.0;
+                dof_update<CORRECT, PREDICT>(p, nd, 3 * (int64_t)nd + k, s_f[threadIdx.x], fx, ke, ie, ee);
+            }
+        }
+    } else if (live && lane < 3) {
         const double f = (double)(lane == 0 ? f0 : (lane == 1 ? f1 : f2));
         const double fx = p.has_gravity ? (double)fe * p.grav[lane] : 0.0;
         dof_update<CORRECT, PREDICT>(p, node, 3 * (int64_t)node + lane, f, fx, ke, ie, ee);
@@ -1677,6 +1710,14 @@ __global__ __launch_bounds__(kBlock) void k_blk_set(int n, int npb, const int32_
     }
 }
 
+// {grad phi, phi} of each forward slot in one 16 B record (mixed, lists that stay)
+__global__ void k_fpair(int64_t nnz, int64_t cap, const float *shp, const float *dshp, float4 *out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    out[q] = make_float4(dshp[q], dshp[cap + q], dshp[2 * cap + q], shp[q]);
+}
+
 __global__ void k_cells_ne(int64_t ncell, const int32_t *cell_start, const int32_t *pos, int32_t *cells)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1781,6 +1822,8 @@ struct mega_ctx {
     int32_t *gset_ptr = nullptr, *gset = nullptr, *fset_ptr = nullptr, *fset = nullptr, *set_max = nullptr;
     uint16_t *gloc = nullptr;
     int64_t gset_cap = 0, fset_cap = 0, gloc_cap = 0;
+    float4 *fpair = nullptr;   // mixed: {grad phi, phi} per forward slot, beside gloc
+    int64_t fpair_cap = 0;
     int sets_on = 0, gset_max = 0, fset_max = 0;
     // bins
     BinGeom geom{};
@@ -1853,7 +1896,7 @@ Params params(mega_ctx *c)
     p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
     if (c->sets_on) {
         p.fset_ptr = c->fset_ptr; p.fset = c->fset;
-        if (c->mixed) { p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc; }
+        if (c->mixed) { p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc; p.fpair = c->fpair; }
     }
     p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
@@ -2090,6 +2133,13 @@ int build_sets(mega_ctx *c)
         e |= dalloc(&c->gloc, c->nnz_cap);
         if (!e) c->gloc_cap = c->nnz_cap;
     }
+    if (c->mixed && c->fpair_cap < c->nnz) {
+        (void)hipFree(c->fpair);
+        c->fpair = nullptr;
+        c->fpair_cap = 0;
+        e |= dalloc(&c->fpair, c->nnz_cap);
+        if (!e) c->fpair_cap = c->nnz_cap;
+    }
     if (e) return MEGA_ERR_NOMEM;
     Scope s(c, "k_blk_set");
     int r;
@@ -2099,6 +2149,11 @@ int build_sets(mega_ctx *c)
         if (!r)
             r = build_set<float>(c, kNodesPerBlock, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
                                  nullptr, &c->gset_max);
+        if (!r) {
+            k_fpair<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->nnz_cap, (const float *)c->shp,
+                                                               (const float *)c->dshp, c->fpair);
+            HIPCHK(hipGetLastError());
+        }
     } else {
         r = build_set<double>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
                               static_cast<TPair<double> *>(c->tpair), &c->fset_max);
@@ -2139,10 +2194,10 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
     {
         Scope s(c, "k_node");
         if constexpr (std::is_same<R, float>::value) {
-            if (c->surr_on) k_node<R, FM, true><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
-            else k_node<R, FM, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+            if (c->surr_on) k_node_surr<FM><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+            else k_node<R, FM><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
         } else {
-            k_node<R, FM, false><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
+            k_node<R, FM><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
         }
     }
     if (force_mode >= 0) {
@@ -2523,7 +2578,7 @@ int mega_destroy(mega_ctx *c)
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
                     c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
-                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc};
+                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
