@@ -154,7 +154,8 @@ struct Params {
     const float *surr_bias, *surr_norm;
     int surr_mat;                                    // material slot it was trained for
     float surr_tol;                                  // yield residual / sigma_y accepted
-    unsigned long long *sstat;                       // [plastic, outside envelope, evaluated, rejected]
+    unsigned long long *sstat;                       // per k_node block: [plastic, outside envelope,
+                                                     // evaluated, rejected], summed on the host
     // tuple harvest (k_node): 20 floats per plastic point
     float *hv;
     unsigned long long *hv_count;
@@ -721,8 +722,9 @@ struct SurrLds {
 // otherwise the caller runs the exact return map.  Each wave works alone:
 // only a wave with a lane inside the envelope runs the network, reading the
 // packed weights (22 KB, cache-resident) from global memory.
+// cnt: this wave's slot of the block's counters (lane 0 writes all four)
 __device__ bool surr_return(const Params &p, bool cand, const float *der, const float *sr, const float *pred,
-                            float &ep, float *s_out, const Material &mt)
+                            float &ep, float *s_out, const Material &mt, unsigned *cnt)
 {
     __shared__ SurrLds L;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -736,10 +738,11 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
         for (int k = 0; k < mlp::kIn; ++k)
             inenv = inenv && x[k] >= p.surr_norm[mlp::kEnvLo + k] && x[k] <= p.surr_norm[mlp::kEnvHi + k];
     const uint64_t mc = __ballot(cand), mu = __ballot(inenv);
-    if (lane == 0 && mc) {
-        atomicAdd(&p.sstat[0], (unsigned long long)__popcll(mc));
-        atomicAdd(&p.sstat[1], (unsigned long long)(__popcll(mc) - __popcll(mu)));
-        atomicAdd(&p.sstat[2], (unsigned long long)__popcll(mu));
+    if (lane == 0) {
+        cnt[0] = (unsigned)__popcll(mc);
+        cnt[1] = (unsigned)(__popcll(mc) - __popcll(mu));
+        cnt[2] = (unsigned)__popcll(mu);
+        cnt[3] = 0u;
     }
     if (!mu) return false;
     const mlp::bf16x8 *tiles = reinterpret_cast<const mlp::bf16x8 *>(p.surr_tiles);
@@ -783,7 +786,7 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
         }
     }
     const uint64_t mf = __ballot(inenv && !ok);
-    if (lane == 0 && mf) atomicAdd(&p.sstat[3], (unsigned long long)__popcll(mf));
+    if (lane == 0) cnt[3] = (unsigned)__popcll(mf);
     return ok;
 }
 
@@ -828,6 +831,7 @@ template <class R, int FM, bool SURR>
 __device__ __forceinline__ void node_update(const Params &p)
 {
     __shared__ double s_red[kBlock / 64];
+    __shared__ unsigned s_cnt[kBlock / 64][4];   // learned return map: counters per wave
     constexpr bool MIX = std::is_same<R, float>::value;
     static_assert(!SURR || MIX, "the learned return map runs in mixed precision");
     constexpr int W = MIX ? 21 : 18;
@@ -907,7 +911,7 @@ __device__ __forceinline__ void node_update(const Params &p)
     if constexpr (SURR) {
         const bool cand = valid && mt.type == 2 && mi == p.surr_mat && mega::trial_excess(pred, ep, mt) > 0.0f;
         float s_new[6];
-        done = surr_return(p, cand, der, sr, pred, ep, s_new, mt);
+        done = surr_return(p, cand, der, sr, pred, ep, s_new, mt, s_cnt[threadIdx.x >> 6]);
         if (done)
 #pragma unroll
             for (int k = 0; k < 6; ++k) sr[k] = s_new[k];
@@ -967,6 +971,16 @@ __device__ __forceinline__ void node_update(const Params &p)
     }
     const double ws = block_sum(work, s_red);
     if (threadIdx.x == 0) p.wpart[blockIdx.x] += ws;
+    if (SURR && threadIdx.x == 0) {   // the block's counters; no same-address atomics
+        unsigned long long *o = p.sstat + 4 * (int64_t)blockIdx.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unsigned long long t = 0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) t += s_cnt[w][k];
+            o[k] += t;
+        }
+    }
 }
 
 template <class R, int FM>
@@ -2278,6 +2292,22 @@ int do_pd_fint(mega_ctx *c, int linit, int force_mode)
     return c->mixed ? pd_kernels<float>(c, linit, force_mode) : pd_kernels<double>(c, linit, force_mode);
 }
 
+// the learned return map's counters summed over the k_node blocks
+int surrogate_totals(mega_ctx *c, int64_t *stats, int reset)
+{
+    const size_t m = 4 * (size_t)c->nb_node;
+    std::vector<unsigned long long> h(m);
+    HIPCHK(hipMemcpyAsync(h.data(), c->sstat, sizeof(unsigned long long) * m, hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(hipMemsetAsync(c->sstat, 0, sizeof(unsigned long long) * m, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) {
+        unsigned long long t = 0;
+        for (size_t b = 0; b < (size_t)c->nb_node; ++b) t += h[4 * b + k];
+        stats[k] = (int64_t)t;
+    }
+    return MEGA_OK;
+}
+
 double host_sum(mega_ctx *c, const double *dev, int64_t count, int64_t stride, int64_t offset, int *err)
 {
     std::vector<double> h((size_t)count * stride);
@@ -2537,7 +2567,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     }
     e |= dalloc(&c->wpart, c->nb_node); e |= dalloc(&c->epart, 3 * (size_t)c->nb_force);
     e |= dalloc(&c->flags, 4);
-    e |= dalloc(&c->sstat, 4);
+    e |= dalloc(&c->sstat, 4 * (size_t)c->nb_node);
     if (e) { mega_destroy(c); return MEGA_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { mega_destroy(c); return MEGA_ERR_HIP; }
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -2558,7 +2588,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
          zero(c->eps, n * r) && zero(c->st4, 4 * (size_t)n * r) && zero(c->vphy_step, n3 * 8) &&
          zero(c->aphy_step, n3 * 8) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
          zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n) &&
-         zero(c->sstat, 32);
+         zero(c->sstat, 32 * (size_t)c->nb_node);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
     if (ok && c->mixed) ok = zero(c->d4, (size_t)n * 16) && zero(c->t4, (size_t)n * 16);
     if (ok && c->wave_vel) ok = zero(c->wave_vel, (size_t)n * 8);
@@ -3044,10 +3074,10 @@ int mega_stats(mega_ctx *c, int64_t *nnz, int32_t *max_neighbors, int64_t *fallb
     if (nnz) *nnz = rp[n];
     if (max_neighbors) *max_neighbors = mx;
     if (fallbacks) {
-        unsigned long long h[4];
-        HIPCHK(hipMemcpyAsync(h, c->sstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        *fallbacks = (int64_t)h[3];
+        int64_t st[4];
+        const int r = surrogate_totals(c, st, 0);
+        if (r) return r;
+        *fallbacks = st[3];
     }
     return MEGA_OK;
 }
@@ -3124,12 +3154,7 @@ int mega_set_surrogate(mega_ctx *c, const float *params, const float *norm, cons
 int mega_surrogate_stats(mega_ctx *c, int64_t *stats, int32_t reset)
 {
     if (!c || !stats) return MEGA_ERR_ARG;
-    unsigned long long h[4];
-    HIPCHK(hipMemcpyAsync(h, c->sstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(hipMemsetAsync(c->sstat, 0, sizeof(h), c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 4; ++k) stats[k] = (int64_t)h[k];
-    return MEGA_OK;
+    return surrogate_totals(c, stats, reset);
 }
 
 int mega_harvest(mega_ctx *c, int64_t capacity, int32_t every)
