@@ -139,7 +139,7 @@ struct Params {
     // node arrays, FP64 masters
     const double *X0, *win, *vol, *mass;
     double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
-    double *Fst;                                     // deformation gradient per node (9, column-major)
+    double *Fst;                                     // deformation gradient per node (9, column-major, + 1 pad)
     float4 *d4, *t4;                                 // FP32 gather copies of d and dtot (mixed)
     // state and scratch in R
     void *stress, *strain, *eps, *red, *P;
@@ -173,6 +173,66 @@ struct Params {
 
 template <class R> __device__ __forceinline__ R *as(void *p) { return static_cast<R *>(p); }
 template <class R> constexpr int pstride() { return std::is_same<R, float>::value ? 12 : 10; }
+
+// Per-node sums of the RK gather for the node update, in whole 16 B pieces:
+// mixed [L0 (9) | S (3)], FP64 [L0 (9) | 0]; with F_DIRECT followed by
+// [F (9) | 0 (3)] (mixed) or [F (9) | 0] (FP64).
+template <class R, bool DIR> constexpr int red_stride()
+{
+    return std::is_same<R, float>::value ? (DIR ? 24 : 12) : (DIR ? 20 : 10);
+}
+
+// N values (a multiple of 16 B) moved as 16 B vector loads and stores
+template <class R, int N>
+__device__ __forceinline__ void store_vec(R *dst, const R *v)
+{
+    static_assert(N * sizeof(R) % 16 == 0, "whole 16 B pieces");
+    if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k)
+            reinterpret_cast<float4 *>(dst)[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) reinterpret_cast<double2 *>(dst)[k] = make_double2(v[2 * k], v[2 * k + 1]);
+    }
+}
+// six values (Voigt stress or strain) in three pieces of two
+template <class R>
+__device__ __forceinline__ void load6(const R *src, R *v)
+{
+    using V2 = typename std::conditional<std::is_same<R, float>::value, float2, double2>::type;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const V2 a = reinterpret_cast<const V2 *>(src)[k];
+        v[2 * k] = a.x; v[2 * k + 1] = a.y;
+    }
+}
+template <class R>
+__device__ __forceinline__ void store6(R *dst, const R *v)
+{
+    using V2 = typename std::conditional<std::is_same<R, float>::value, float2, double2>::type;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) reinterpret_cast<V2 *>(dst)[k] = V2{v[2 * k], v[2 * k + 1]};
+}
+
+template <class R, int N>
+__device__ __forceinline__ void load_vec(const R *src, R *v)
+{
+    static_assert(N * sizeof(R) % 16 == 0, "whole 16 B pieces");
+    if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k) {
+            const float4 a = reinterpret_cast<const float4 *>(src)[k];
+            v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) {
+            const double2 a = reinterpret_cast<const double2 *>(src)[k];
+            v[2 * k] = a.x; v[2 * k + 1] = a.y;
+        }
+    }
+}
 
 // deformation-gradient handling of the gather/node pair of kernels
 enum FMode { F_NONE = 0, F_DIRECT = 1, F_INCR = 2 };
@@ -671,17 +731,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 #pragma unroll
         for (int k = 0; k < 3; ++k) u[k] = group_sum<kGroup>(u[k]);
     }
-    constexpr int W = MIX ? 21 : 18;
-    constexpr int FO = MIX ? 12 : 9;
-    R *red = as<R>(p.red) + (int64_t)W * node;
+    constexpr int W = red_stride<R, DIR>();
+    constexpr int FO = MIX ? 12 : 10;
     if (lane == 0) {
+        R rec[W];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) red[k] = L[k];
-        if (MIX) { red[9] = S[0]; red[10] = S[1]; red[11] = S[2]; }
-        if (DIR) {
+        for (int k = 0; k < W; ++k) rec[k] = R(0);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) red[FO + k] = F[k];
+        for (int k = 0; k < 9; ++k) rec[k] = L[k];
+        if constexpr (MIX) { rec[9] = S[0]; rec[10] = S[1]; rec[11] = S[2]; }
+        if constexpr (DIR) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) rec[FO + k] = F[k];
         }
+        store_vec<R, W>(as<R>(p.red) + (int64_t)W * node, rec);
     }
     if (UPDATE_X && lane < 3) {
         const int qd = 3 * node + lane;
@@ -834,18 +897,20 @@ __device__ __forceinline__ void node_update(const Params &p)
     __shared__ unsigned s_cnt[kBlock / 64][4];   // learned return map: counters per wave
     constexpr bool MIX = std::is_same<R, float>::value;
     static_assert(!SURR || MIX, "the learned return map runs in mixed precision");
-    constexpr int W = MIX ? 21 : 18;
-    constexpr int FO = MIX ? 12 : 9;
+    constexpr bool DIR = FM == F_DIRECT;
+    constexpr int W = red_stride<R, DIR>();
+    constexpr int FO = MIX ? 12 : 10;
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i0 < p.n;
     const int i = valid ? i0 : p.n - 1;
     double work = 0.0;
-    const R *red = as<R>(p.red) + (int64_t)W * i;
+    R red[W];
+    load_vec<R, W>(as<R>(p.red) + (int64_t)W * i, red);
     double L0[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) L0[k] = (double)red[k];
     double S[3] = {0, 0, 0}, dI[3] = {0, 0, 0}, tI[3] = {0, 0, 0};
-    if (MIX) {   // add back S (x) own value: L0(k,l) += S_l d_ik
+    if constexpr (MIX) {   // add back S (x) own value: L0(k,l) += S_l d_ik
         const float4 dv = p.d4[i];
         S[0] = (double)red[9]; S[1] = (double)red[10]; S[2] = (double)red[11];
         dI[0] = dv.x; dI[1] = dv.y; dI[2] = dv.z;
@@ -860,7 +925,7 @@ __device__ __forceinline__ void node_update(const Params &p)
     double det = 1.0, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Lm[9];
     if (FM != F_NONE) {
         double Fm[9];
-        if (FM == F_DIRECT) {
+        if constexpr (FM == F_DIRECT) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) Fm[k] = (double)red[FO + k];
             if (MIX) {
@@ -873,12 +938,20 @@ __device__ __forceinline__ void node_update(const Params &p)
             }
             Fm[0] += 1.0; Fm[4] += 1.0; Fm[8] += 1.0;
         } else {
+            {
+                double Fo[10];
+                load_vec<double, 10>(p.Fst + 10 * (int64_t)i, Fo);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Fm[k] = p.Fst[9 * (int64_t)i + k] + L0[k];
+                for (int k = 0; k < 9; ++k) Fm[k] = Fo[k] + L0[k];
+            }
         }
-        if (valid)
+        if (valid) {
+            double Fo[10];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) p.Fst[9 * (int64_t)i + k] = Fm[k];
+            for (int k = 0; k < 9; ++k) Fo[k] = Fm[k];
+            Fo[9] = 0.0;
+            store_vec<double, 10>(p.Fst + 10 * (int64_t)i, Fo);
+        }
         det = mega::det3_abs(Fm);
         mega::inv3_ref(Fm, IF);
         mega::mm3(L0, IF, Lm);   // L = L0 F^-1 (FINT.F90:380-391 then 669-679)
@@ -888,8 +961,13 @@ __device__ __forceinline__ void node_update(const Params &p)
     }
     R *stress = as<R>(p.stress) + 6 * (int64_t)i, *strain = as<R>(p.strain) + 6 * (int64_t)i;
     double sig[6], e6[6], de[6], dd[6], old_s[6], old_e[6];
+    {
+        R s6[6], e6r[6];
+        load6<R>(stress, s6);
+        load6<R>(strain, e6r);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)stress[k]; e6[k] = old_e[k] = (double)strain[k]; }
+        for (int k = 0; k < 6; ++k) { sig[k] = old_s[k] = (double)s6[k]; e6[k] = old_e[k] = (double)e6r[k]; }
+    }
     if (p.finite_strain) {
         double Rot[9];
         mega::hughes_winget(Lm, Rot, de, dd);
@@ -944,10 +1022,13 @@ __device__ __forceinline__ void node_update(const Params &p)
     if (valid) {
         as<R>(p.eps)[i] = ep;
         double si[6], ei[6];
+        R e6r[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) e6r[k] = (R)e6[k];
+        store6<R>(stress, sr);
+        store6<R>(strain, e6r);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            stress[k] = sr[k];
-            strain[k] = (R)e6[k];
             sig[k] = (double)sr[k];
             si[k] = sig[k] - old_s[k];
             ei[k] = e6[k] - old_e[k];
@@ -961,13 +1042,16 @@ __device__ __forceinline__ void node_update(const Params &p)
         // P = V J sigma IF^T  (f_j = P grad phi_ij)
         const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
         const double w = vol * det;
-        R *P = as<R>(p.P) + pstride<R>() * (int64_t)i;
+        R P[pstride<R>()];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int r = 0; r < 3; ++r)
                 P[r + 3 * c] = (R)(w * (Sg[r] * IF[c] + Sg[r + 3] * IF[c + 3] + Sg[r + 6] * IF[c + 6]));
         P[9] = (R)(mt.rho * w);
+#pragma unroll
+        for (int k = 10; k < pstride<R>(); ++k) P[k] = R(0);
+        store_vec<R, pstride<R>()>(as<R>(p.P) + pstride<R>() * (int64_t)i, P);
     }
     const double ws = block_sum(work, s_red);
     if (threadIdx.x == 0) p.wpart[blockIdx.x] += ws;
@@ -2541,7 +2625,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (c->has_gravity) e |= dalloc(&c->fext_nmo, n3);
     if (c->mixed) { e |= dalloc(&c->d4, n); e |= dalloc(&c->t4, n); }
     else e |= dalloc(&c->wave_vel, n);
-    e |= dalloc(&c->Fst, 9 * (size_t)n);
+    e |= dalloc(&c->Fst, 10 * (size_t)n);
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
     e |= dalloc(&c->win, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
@@ -2549,7 +2633,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
     e |= dalloc(&c->st4, 4 * (size_t)n, r);
     e |= dalloc(&c->vphy_step, n3); e |= dalloc(&c->aphy_step, n3);
-    e |= dalloc(&c->red, 21 * (size_t)n, r); e |= dalloc(&c->P, (c->mixed ? 12 : 10) * (size_t)n, r);
+    e |= dalloc(&c->red, 24 * (size_t)n, r); e |= dalloc(&c->P, (c->mixed ? 12 : 10) * (size_t)n, r);
     e |= dalloc(&c->row_ptr, n + 1); e |= dalloc(&c->trow, n + 1); e |= dalloc(&c->counts, n + 1);
     e |= dalloc(&c->ijk, n3); e |= dalloc(&c->cell_start, c->ncell + 1); e |= dalloc(&c->cell_cursor, c->ncell + 1);
     e |= dalloc(&c->cell_nodes, n);
@@ -2587,7 +2671,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
          zero(c->fint, n3 * 8) && zero(c->fext, n3 * 8) && zero(c->stress, n6 * r) && zero(c->strain, n6 * r) &&
          zero(c->eps, n * r) && zero(c->st4, 4 * (size_t)n * r) && zero(c->vphy_step, n3 * 8) &&
          zero(c->aphy_step, n3 * 8) && zero(c->wpart, 8 * (size_t)c->nb_node) &&
-         zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 72 * (size_t)n) &&
+         zero(c->epart, 24 * (size_t)c->nb_force) && zero(c->flags, 16) && zero(c->Fst, 80 * (size_t)n) &&
          zero(c->sstat, 32 * (size_t)c->nb_node);
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
     if (ok && c->mixed) ok = zero(c->d4, (size_t)n * 16) && zero(c->t4, (size_t)n * 16);
