@@ -1102,35 +1102,76 @@ __device__ __forceinline__ void load_P(const R *Pb, int i, R *P)
 // one degree of freedom after its force is known: fint, EBC (fixity 1: v = a = 0,
 // reaction into fext), damping and the corrector (MAIN.F90:432-488), the
 // energy terms (MAIN.F90:524-553) and, fused, the next step's predictor
+// (the arithmetic of predict_dof, without reading back v, a and dtot).
+struct DofIn {
+    double v, mass, damp, fxo, d, dtot;
+    int ebc;
+};
+
+__device__ __forceinline__ DofIn dof_in(const Params &p, int node, int64_t q)
+{
+    DofIn r;
+    r.v = p.v[q];
+    r.ebc = p.ebc[q];
+    r.mass = p.mass[q];
+    r.damp = p.mats[p.mat[node]].damping;
+    r.fxo = p.has_gravity ? p.fext_nmo[q] : 0.0;
+    r.d = p.d[q];
+    r.dtot = p.dtot[q];
+    return r;
+}
+
 template <bool CORRECT, bool PREDICT>
-__device__ __forceinline__ void dof_update(const Params &p, int node, int64_t q, double f, double fx, double &ke,
-                                           double &ie, double &ee)
+__device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn &in, double f, double fx, double &ke,
+                                        double &ie, double &ee)
 {
     p.fint[q] = f;
     if (!CORRECT) {
         p.fext[q] = fx;
         return;
     }
-    const double fx_old = p.has_gravity ? p.fext_nmo[q] : 0.0;
-    double v = p.v[q];
-    if (p.ebc[q] == 1) {
+    double v = in.v, acc = 0.0;
+    if (in.ebc == 1) {
         v = 0.0;
-        p.a[q] = 0.0;
         fx = -f;
     } else {
-        const double mass = p.mass[q];
-        const double c = p.mats[p.mat[node]].damping * mass;
-        const double acc = (fx - f - c * v) / (mass + 0.5 * p.dt * c);
-        p.a[q] = acc;
+        const double c = in.damp * in.mass;
+        acc = (fx - f - c * v) / (in.mass + 0.5 * p.dt * c);
         v = v + 0.5 * p.dt * acc;
     }
-    p.v[q] = v;
+    p.a[q] = acc;
     p.fext[q] = fx;
     if (p.has_gravity) p.fext_nmo[q] = fx;
-    ke = 0.5 * p.mass[q] * v * v;
-    if (!p.finite_strain) ie = 0.5 * f * p.dtot[q];
-    ee = 0.5 * (fx + fx_old) * p.d[q];
-    if (PREDICT) predict_dof(p, q);
+    ke = 0.5 * in.mass * v * v;
+    if (!p.finite_strain) ie = 0.5 * f * in.dtot;
+    ee = 0.5 * (fx + in.fxo) * in.d;
+    if (PREDICT) {
+        const double dt = p.dt;
+        const double d = dt * v + dt * dt * 0.5 * acc;
+        const double t = in.dtot + d;
+        p.d[q] = d;
+        p.dtot[q] = t;
+        p.v[q] = v + dt * 0.5 * acc;
+        if (p.pd) {
+            const double u = p.uphy[q] + d;
+            p.uphy[q] = u;
+            p.x[q] = p.X0[q] + u;
+        }
+        if (p.d4) {
+            const int64_t i = q / 3, k = q - 3 * i;
+            reinterpret_cast<float *>(p.d4)[4 * i + k] = (float)d;
+            reinterpret_cast<float *>(p.t4)[4 * i + k] = (float)t;
+        }
+    } else {
+        p.v[q] = v;
+    }
+}
+
+template <bool CORRECT, bool PREDICT>
+__device__ __forceinline__ void dof_update(const Params &p, int node, int64_t q, double f, double fx, double &ke,
+                                           double &ie, double &ee)
+{
+    dof_out<CORRECT, PREDICT>(p, q, dof_in(p, node, q), f, fx, ke, ie, ee);
 }
 
 template <class R, bool CORRECT, bool PREDICT>
