@@ -156,6 +156,38 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
         assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("norm_win", [2.2, 4.4])
+def test_steps_with_wide_supports(mega, oracle, norm_win, precision):
+    """Lagrangian steps with wide supports against the oracle.  The force and
+    gather kernels stage each block's distinct sources in LDS only up to 512
+    sources and 4096 list entries per block; at about 90 neighbors per node
+    the sets are used, at about 360 the blocks exceed 4096 entries on average
+    and read global memory instead.  Bars as in the 12-step tests."""
+    prob = small_problem(lagrangian=1, norm_win=norm_win)
+    nsteps = 6
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(nsteps)
+    nnz, _, _ = g.stats()
+    kbar = nnz / prob.n
+    print("norm_win", norm_win, "mean neighbors", kbar, flush=True)
+    if norm_win > 3:
+        assert 16 * kbar > 4096   # both kernels' blocks beyond the set limit on average
+    else:
+        assert 32 * kbar < 4096   # within it
+    A = OracleRun(oracle, prob).step(nsteps).arr
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("uphy", "dphy_tot"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    print(precision, errs)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["uphy"] < 1e-10
+        assert errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+    else:
+        assert errs["x"] < 1e-7 and errs["uphy"] < 1e-4 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    g.close()
+
+
 def test_construct_fint_mixed_precision(mega, oracle):
     """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
     rotation) against the FP64 oracle on the same inputs: force within 1e-4
