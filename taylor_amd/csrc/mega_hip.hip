@@ -76,8 +76,11 @@ static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one 
 #ifndef MEGA_SET_CAP
 #define MEGA_SET_CAP 512
 #endif
+#ifndef MEGA_GGROUP_MIX
+#define MEGA_GGROUP_MIX 8
+#endif
 #ifndef MEGA_UNROLL_MIX
-#define MEGA_UNROLL_MIX 4
+#define MEGA_UNROLL_MIX 6
 #endif
 #ifndef MEGA_UNROLL_F64
 #define MEGA_UNROLL_F64 4
@@ -94,6 +97,7 @@ static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one 
 constexpr int kBlock = 256;
 constexpr int kGroup = MEGA_KGROUP;   // lanes cooperating on one node's neighbor list
 constexpr int kFGroup = MEGA_FGROUP;  // the same for the force gather over the transposed list
+constexpr int kGGroupMix = MEGA_GGROUP_MIX;   // the RK gather in mixed precision (FP64: kGroup)
 constexpr int kRkGroup = MEGA_RK_GROUP;   // the same for the shape-function kernel
 constexpr int kRkCache = MEGA_RK_CACHE;   // kernel evaluations per lane kept between its two passes
 constexpr int kXcd = 8;               // XCDs on MI355X
@@ -101,8 +105,11 @@ constexpr int kXcd = 8;               // XCDs on MI355X
 constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
 constexpr int kFUnrollMix = MEGA_FUNROLL_MIX, kFUnrollF64 = MEGA_FUNROLL_F64;
 constexpr int kNodesPerBlock = kBlock / kGroup;    // nodes of one block in the gathers
+// lanes per node and nodes per block of the RK gather (k_gather)
+template <class R> constexpr int gather_group() { return std::is_same<R, float>::value ? kGGroupMix : kGroup; }
+template <class R> constexpr int gather_npb() { return kBlock / gather_group<R>(); }
 constexpr int kNodesPerFBlock = kBlock / kFGroup;  // nodes of one block in the force gather
-static_assert(kFGroup <= 16 && kGroup <= 16, "groups inside one DPP row");
+static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupMix <= 16, "groups inside one DPP row");
 // Source sets: a block stages the values its rows gather (P rows, increments)
 // once in LDS when it has at most kSetCap distinct sources.
 constexpr int kSetCap = MEGA_SET_CAP;
@@ -628,11 +635,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr bool DIR = FM == F_DIRECT;
     constexpr int U = MIX ? kUnrollMix : kUnrollF64;
+    constexpr int GG = gather_group<R>();
     extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * kNodesPerBlock + threadIdx.x / kGroup;
-    const int lane = threadIdx.x & (kGroup - 1);
+    const int node = blk * gather_npb<R>() + threadIdx.x / GG;
+    const int lane = threadIdx.x & (GG - 1);
     // mixed, lists that stay: the block's distinct neighbors' d (and dtot) in LDS
     int m = 0;
     float4 *Ld = reinterpret_cast<float4 *>(mega_dyn);
@@ -665,18 +673,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
     // LDS: neighbors by their position in the block's set
     auto sweep = [&](auto lds_tag) {
     constexpr bool LDS = decltype(lds_tag)::value;
-    for (int base = p0 + lane; base < p1; base += kGroup * U) {
+    for (int base = p0 + lane; base < p1; base += GG * U) {
         int j[U];
 #pragma unroll
         for (int e = 0; e < U; ++e) {
-            const int q = base + kGroup * e;
+            const int q = base + GG * e;
             if (LDS) j[e] = q < p1 ? (int)p.gloc[q] : 0;
             else j[e] = q < p1 ? p.col[q] : node;
         }
         R g0[U], g1[U], g2[U], f[U], dj[U][3], tj[U][3];
 #pragma unroll
         for (int e = 0; e < U; ++e) {
-            const int q = base + kGroup * e;
+            const int q = base + GG * e;
             const bool ok = q < p1;
             if (LDS) {   // mixed, lists that stay: the packed record
                 const float4 r = ok ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -718,18 +726,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
     if (m > 0) sweep(std::true_type{});
     else sweep(std::false_type{});
 #pragma unroll
-    for (int k = 0; k < 9; ++k) L[k] = group_sum<kGroup>(L[k]);
+    for (int k = 0; k < 9; ++k) L[k] = group_sum<GG>(L[k]);
     if (DIR) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) F[k] = group_sum<kGroup>(F[k]);
+        for (int k = 0; k < 9; ++k) F[k] = group_sum<GG>(F[k]);
     }
     if (MIX) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) S[k] = group_sum<kGroup>(S[k]);
+        for (int k = 0; k < 3; ++k) S[k] = group_sum<GG>(S[k]);
     }
     if (UPDATE_X) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) u[k] = group_sum<kGroup>(u[k]);
+        for (int k = 0; k < 3; ++k) u[k] = group_sum<GG>(u[k]);
     }
     constexpr int W = red_stride<R, DIR>();
     constexpr int FO = MIX ? 12 : 10;
@@ -2260,7 +2268,8 @@ int build_sets(mega_ctx *c)
 {
     c->sets_on = 0;
     if (!c->tq || c->pd || c->nnz == 0) return MEGA_OK;
-    const int nbf = (c->n + kNodesPerFBlock - 1) / kNodesPerFBlock, nbg = (c->n + kNodesPerBlock - 1) / kNodesPerBlock;
+    const int nbf = (c->n + kNodesPerFBlock - 1) / kNodesPerFBlock;
+    const int nbg = (c->n + gather_npb<float>() - 1) / gather_npb<float>();
     int e = 0;
     if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nbf + 1);
     if (!c->set_max) e |= dalloc(&c->set_max, 1);
@@ -2286,7 +2295,7 @@ int build_sets(mega_ctx *c)
         r = build_set<float>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
                              static_cast<TPair<float> *>(c->tpair), &c->fset_max);
         if (!r)
-            r = build_set<float>(c, kNodesPerBlock, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
+            r = build_set<float>(c, gather_npb<float>(), c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
                                  nullptr, &c->gset_max);
         if (!r) {
             k_fpair<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->nnz_cap, (const float *)c->shp,
@@ -2323,7 +2332,7 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
 {
     Params p = params(c);
     int nb = 0;
-    const unsigned g = xcd_grid(c->n, &nb);
+    const unsigned g = xcd_grid(c->n, &nb, gather_npb<R>());
     {
         Scope s(c, "k_gather");
         const size_t lds = p.gset_ptr ? (size_t)c->gset_max * (FM == F_DIRECT ? 32 : 16) : 0;
