@@ -74,7 +74,7 @@ namespace {
 #endif
 static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one neighbor per lane");
 #ifndef MEGA_SET_CAP
-#define MEGA_SET_CAP 512
+#define MEGA_SET_CAP 448
 #endif
 #ifndef MEGA_GGROUP_MIX
 #define MEGA_GGROUP_MIX 8
@@ -111,7 +111,9 @@ template <class R> constexpr int gather_npb() { return kBlock / gather_group<R>(
 constexpr int kNodesPerFBlock = kBlock / kFGroup;  // nodes of one block in the force gather
 static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupMix <= 16, "groups inside one DPP row");
 // Source sets: a block stages the values its rows gather (P rows, increments)
-// once in LDS when it has at most kSetCap distinct sources.
+// once in LDS when it has at most kSetCap distinct sources.  448 rows of 48 B
+// plus the force kernel's own 1 KB let 7 of its blocks (its register limit)
+// share a CU's 160 KB of LDS; larger sets read global memory.
 constexpr int kSetCap = MEGA_SET_CAP;
 constexpr int kSetSort = 4096;   // list entries of one block the set builder sorts in LDS
 static_assert(kSetCap <= 65535, "set positions are 16-bit");
