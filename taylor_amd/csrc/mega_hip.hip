@@ -74,7 +74,7 @@ namespace {
 #endif
 static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one neighbor per lane");
 #ifndef MEGA_SET_CAP
-#define MEGA_SET_CAP 448
+#define MEGA_SET_CAP 512
 #endif
 #ifndef MEGA_GGROUP_MIX
 #define MEGA_GGROUP_MIX 8
@@ -111,9 +111,9 @@ template <class R> constexpr int gather_npb() { return kBlock / gather_group<R>(
 constexpr int kNodesPerFBlock = kBlock / kFGroup;  // nodes of one block in the force gather
 static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupMix <= 16, "groups inside one DPP row");
 // Source sets: a block stages the values its rows gather (P rows, increments)
-// once in LDS when it has at most kSetCap distinct sources.  448 rows of 48 B
-// plus the force kernel's own 1 KB let 7 of its blocks (its register limit)
-// share a CU's 160 KB of LDS; larger sets read global memory.
+// once in LDS when it has at most kSetCap distinct sources.  512 P rows of
+// 40 B (mixed) plus the force kernel's own 1 KB let 7 of its blocks (its
+// register limit) share a CU's 160 KB of LDS; larger sets read global memory.
 constexpr int kSetCap = MEGA_SET_CAP;
 constexpr int kSetSort = 4096;   // list entries of one block the set builder sorts in LDS
 static_assert(kSetCap <= 65535, "set positions are 16-bit");
@@ -1206,9 +1206,20 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
         const int s0 = p.fset_ptr[blk];
         m = p.fset_ptr[blk + 1] - s0;
         const V *Pg = reinterpret_cast<const V *>(p.P);
-        for (int k = threadIdx.x; k < PV * m; k += kBlock) {
-            const int r = k / PV;
-            Lp[k] = Pg[PV * (int64_t)p.fset[s0 + r] + (k - PV * r)];
+        if constexpr (MIX) {   // the 10 used values of a row, 40 B in LDS
+            float2 *L2 = reinterpret_cast<float2 *>(mega_dyn);
+            for (int r = threadIdx.x; r < m; r += kBlock) {
+                const float4 *q = reinterpret_cast<const float4 *>(Pg) + 3 * (int64_t)p.fset[s0 + r];
+                const float4 a = q[0], b = q[1], c = q[2];
+                L2[5 * r] = make_float2(a.x, a.y); L2[5 * r + 1] = make_float2(a.z, a.w);
+                L2[5 * r + 2] = make_float2(b.x, b.y); L2[5 * r + 3] = make_float2(b.z, b.w);
+                L2[5 * r + 4] = make_float2(c.x, c.y);
+            }
+        } else {
+            for (int k = threadIdx.x; k < PV * m; k += kBlock) {
+                const int r = k / PV;
+                Lp[k] = Pg[PV * (int64_t)p.fset[s0 + r] + (k - PV * r)];
+            }
         }
         if (m > 0) __syncthreads();
     }
@@ -1239,9 +1250,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Params p, int nblocks)
                     const V *q = Lp + PV * src[e];
                     R *o = P[e];
                     if constexpr (MIX) {
-                        const V a = q[0], b = q[1], c = q[2];
-                        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z;
-                        o[7] = b.w; o[8] = c.x; o[9] = c.y;
+                        const float2 *q2 = reinterpret_cast<const float2 *>(mega_dyn) + 5 * src[e];
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) { const float2 a = q2[k]; o[2 * k] = a.x; o[2 * k + 1] = a.y; }
                     } else {
 #pragma unroll
                         for (int k = 0; k < 5; ++k) { const V v2 = q[k]; o[2 * k] = v2.x; o[2 * k + 1] = v2.y; }
@@ -2354,7 +2365,7 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
         Scope s(c, "k_force");
         int nbf = 0;
         const unsigned gf = xcd_grid(c->n, &nbf, kNodesPerFBlock);
-        const size_t lds = p.fset_ptr ? (size_t)c->fset_max * 16 * (std::is_same<R, float>::value ? 3 : 5) : 0;
+        const size_t lds = p.fset_ptr ? (size_t)c->fset_max * (std::is_same<R, float>::value ? 40 : 80) : 0;
         if (force_mode == 0) k_force<R, false, false><<<gf, kBlock, lds, c->stream>>>(p, nbf);
         else if (force_mode == 1) k_force<R, true, false><<<gf, kBlock, lds, c->stream>>>(p, nbf);
         else k_force<R, true, true><<<gf, kBlock, lds, c->stream>>>(p, nbf);

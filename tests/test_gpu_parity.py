@@ -160,7 +160,7 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
 @pytest.mark.parametrize("norm_win", [2.2, 4.4])
 def test_steps_with_wide_supports(mega, oracle, norm_win, precision):
     """Lagrangian steps with wide supports against the oracle.  The force and
-    gather kernels stage each block's distinct sources in LDS only up to 448
+    gather kernels stage each block's distinct sources in LDS only up to 512
     sources and 4096 list entries per block; at about 90 neighbors per node
     the sets are used, at about 360 the blocks exceed 4096 entries on average
     and read global memory instead.  Bars as in the 12-step tests."""
