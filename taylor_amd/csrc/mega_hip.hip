@@ -783,8 +783,12 @@ __device__ __forceinline__ int lanes_below(uint64_t m)
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// per wave: normalised inputs of its candidate lanes, then the outputs
+// the packed weights (staged once per block that has a candidate) and, per
+// wave, the normalised inputs of its candidate lanes, then the outputs
 struct SurrLds {
+    mlp::bf16x8 tiles[mlp::kTilesFwd * 64];
+    float bias[mlp::kBiasPack];
+    float norm[mlp::kNorm];
     float stage[kBlock / 64][64][16];
 };
 
@@ -792,9 +796,9 @@ struct SurrLds {
 // material.  Inside the training envelope the network gives the plastic
 // correction; the answer is kept when the residual of the return-map
 // equations (mega::return_residual) is within tol and eps does not decrease,
-// otherwise the caller runs the exact return map.  Each wave works alone:
-// only a wave with a lane inside the envelope runs the network, reading the
-// packed weights (22 KB, cache-resident) from global memory.
+// otherwise the caller runs the exact return map.  A block with a lane inside
+// the envelope stages the packed weights (22 KB) in LDS; its waves then run
+// the network on their own candidates.
 // cnt: this wave's slot of the block's counters (lane 0 writes all four)
 __device__ bool surr_return(const Params &p, bool cand, const float *der, const float *sr, const float *pred,
                             float &ep, float *s_out, const Material &mt, unsigned *cnt)
@@ -817,9 +821,16 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
         cnt[2] = (unsigned)__popcll(mu);
         cnt[3] = 0u;
     }
-    if (!mu) return false;
-    const mlp::bf16x8 *tiles = reinterpret_cast<const mlp::bf16x8 *>(p.surr_tiles);
-    const float *norm = p.surr_norm;
+    if (!__syncthreads_or(inenv)) return false;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(p.surr_tiles);
+        uint4 *dst = reinterpret_cast<uint4 *>(L.tiles);
+        for (int k = threadIdx.x; k < mlp::kTilesFwd * 64; k += kBlock) dst[k] = src[k];
+        for (int k = threadIdx.x; k < mlp::kBiasPack; k += kBlock) L.bias[k] = p.surr_bias[k];
+        for (int k = threadIdx.x; k < mlp::kNorm; k += kBlock) L.norm[k] = p.surr_norm[k];
+    }
+    __syncthreads();
+    const float *norm = L.norm;
     const int nc = __popcll(mu);
     const int slot = lanes_below(mu);
     float(*st)[16] = L.stage[wv];
@@ -839,7 +850,7 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
             x0 = mlp::pack8(mlp::f32x4{a.x, a.y, a.z, a.w}, mlp::f32x4{b.x, b.y, b.z, b.w});
         }
         mlp::f32x4 z1[4], z2[4], z3[4], y;
-        mlp::forward16(tiles, p.surr_bias, x0, lane, z1, z2, z3, y);
+        mlp::forward16(L.tiles, L.bias, x0, lane, z1, z2, z3, y);
         wave_sync();
         if (g < 2 && s < nc) *reinterpret_cast<float4 *>(&st[s][4 * g]) = float4{y[0], y[1], y[2], y[3]};
     }
@@ -1083,9 +1094,10 @@ __global__ __launch_bounds__(kBlock) void k_node(Params p)
     node_update<R, FM, false>(p);
 }
 
-// with the learned return map (mixed precision)
+// with the learned return map (mixed precision): registers for three waves
+// per SIMD, which the network's tiles fit without spilling (166-168 VGPRs)
 template <int FM>
-__global__ __launch_bounds__(kBlock) void k_node_surr(Params p)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_node_surr(Params p)
 {
     node_update<float, FM, true>(p);
 }
