@@ -188,6 +188,33 @@ def test_steps_with_wide_supports(mega, oracle, norm_win, precision):
     g.close()
 
 
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64")])
+def test_steps_with_gravity(mega, oracle, lagrangian, precision):
+    """A body force (*GRAVITY PARAMETERS): the external force sum phi rho V J g
+    assembled with the internal force (FINT.F90, body-force term), the
+    previous step's external force in the energy balance, against the
+    oracle over 12 steps.  g is scaled so that the body force is a visible
+    fraction of the impact forces.  Bars as in the 12-step tests."""
+    prob = small_problem(lagrangian=lagrangian)
+    prob.gravity = np.array([0.0, 2.0e7, -5.0e7])
+    nsteps = 12
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(nsteps)
+    run = OracleRun(oracle, prob).step(nsteps)
+    A = run.arr
+    assert np.abs(A["fext"]).max() > 1e-3 * np.abs(A["fint"]).max()
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("fext", "fext"), ("stress", "stress"))}
+    print(lagrangian, precision, errs)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["fext"] < 1e-9
+        assert errs["stress"] < 1e-9
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["fext"] < 1e-5 and errs["stress"] < 1e-3
+    g.close()
+
+
 def test_construct_fint_mixed_precision(mega, oracle):
     """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
     rotation) against the FP64 oracle on the same inputs: force within 1e-4
