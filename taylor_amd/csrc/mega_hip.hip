@@ -76,11 +76,14 @@ static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one 
 #ifndef MEGA_SET_CAP
 #define MEGA_SET_CAP 512
 #endif
-#ifndef MEGA_GGROUP_MIX
-#define MEGA_GGROUP_MIX 8
+#ifndef MEGA_GGROUP_SETS
+#define MEGA_GGROUP_SETS 8
+#endif
+#ifndef MEGA_UNROLL_SETS
+#define MEGA_UNROLL_SETS 6
 #endif
 #ifndef MEGA_UNROLL_MIX
-#define MEGA_UNROLL_MIX 6
+#define MEGA_UNROLL_MIX 4
 #endif
 #ifndef MEGA_UNROLL_F64
 #define MEGA_UNROLL_F64 4
@@ -97,19 +100,20 @@ static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one 
 constexpr int kBlock = 256;
 constexpr int kGroup = MEGA_KGROUP;   // lanes cooperating on one node's neighbor list
 constexpr int kFGroup = MEGA_FGROUP;  // the same for the force gather over the transposed list
-constexpr int kGGroupMix = MEGA_GGROUP_MIX;   // the RK gather in mixed precision (FP64: kGroup)
+constexpr int kGGroupSets = MEGA_GGROUP_SETS;   // the mixed RK gather over block source sets
 constexpr int kRkGroup = MEGA_RK_GROUP;   // the same for the shape-function kernel
 constexpr int kRkCache = MEGA_RK_CACHE;   // kernel evaluations per lane kept between its two passes
 constexpr int kXcd = 8;               // XCDs on MI355X
 // neighbor-list entries each lane keeps in flight in the gathers
-constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64;
+constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64, kUnrollSets = MEGA_UNROLL_SETS;
 constexpr int kFUnrollMix = MEGA_FUNROLL_MIX, kFUnrollF64 = MEGA_FUNROLL_F64;
 constexpr int kNodesPerBlock = kBlock / kGroup;    // nodes of one block in the gathers
 // lanes per node and nodes per block of the RK gather (k_gather)
-template <class R> constexpr int gather_group() { return std::is_same<R, float>::value ? kGGroupMix : kGroup; }
-template <class R> constexpr int gather_npb() { return kBlock / gather_group<R>(); }
+// (the packed records and LDS-staged increments need fewer lanes per node)
+constexpr int gather_group(bool sets) { return sets ? kGGroupSets : kGroup; }
+constexpr int gather_npb(bool sets) { return kBlock / gather_group(sets); }
 constexpr int kNodesPerFBlock = kBlock / kFGroup;  // nodes of one block in the force gather
-static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupMix <= 16, "groups inside one DPP row");
+static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupSets <= 16, "groups inside one DPP row");
 // Source sets: a block stages the values its rows gather (P rows, increments)
 // once in LDS when it has at most kSetCap distinct sources.  512 P rows of
 // 40 B (mixed) plus the force kernel's own 1 KB let 7 of its blocks (its
@@ -631,22 +635,23 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
 // fused, u = sum phi d updates the physical displacement and coordinates.
 // Each lane issues the loads of up to U list entries before using any of them.
 
-template <class R, int FM, bool UPDATE_X>
+template <class R, int FM, bool UPDATE_X, bool SETS>
 __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 {
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr bool DIR = FM == F_DIRECT;
-    constexpr int U = MIX ? kUnrollMix : kUnrollF64;
-    constexpr int GG = gather_group<R>();
+    static_assert(!SETS || MIX, "source sets of the gather in mixed precision");
+    constexpr int U = SETS ? kUnrollSets : (MIX ? kUnrollMix : kUnrollF64);
+    constexpr int GG = gather_group(SETS);
     extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * gather_npb<R>() + threadIdx.x / GG;
+    const int node = blk * gather_npb(SETS) + threadIdx.x / GG;
     const int lane = threadIdx.x & (GG - 1);
     // mixed, lists that stay: the block's distinct neighbors' d (and dtot) in LDS
     int m = 0;
     float4 *Ld = reinterpret_cast<float4 *>(mega_dyn);
-    if (MIX && p.gset_ptr) {
+    if (SETS) {
         const int s0 = p.gset_ptr[blk];
         m = p.gset_ptr[blk + 1] - s0;
         for (int k = threadIdx.x; k < m; k += kBlock) {
@@ -2294,7 +2299,7 @@ int build_sets(mega_ctx *c)
     c->sets_on = 0;
     if (!c->tq || c->pd || c->nnz == 0) return MEGA_OK;
     const int nbf = (c->n + kNodesPerFBlock - 1) / kNodesPerFBlock;
-    const int nbg = (c->n + gather_npb<float>() - 1) / gather_npb<float>();
+    const int nbg = (c->n + gather_npb(true) - 1) / gather_npb(true);
     int e = 0;
     if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nbf + 1);
     if (!c->set_max) e |= dalloc(&c->set_max, 1);
@@ -2320,7 +2325,7 @@ int build_sets(mega_ctx *c)
         r = build_set<float>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
                              static_cast<TPair<float> *>(c->tpair), &c->fset_max);
         if (!r)
-            r = build_set<float>(c, gather_npb<float>(), c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
+            r = build_set<float>(c, gather_npb(true), c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc,
                                  nullptr, &c->gset_max);
         if (!r) {
             k_fpair<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->nnz_cap, (const float *)c->shp,
@@ -2357,12 +2362,23 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
 {
     Params p = params(c);
     int nb = 0;
-    const unsigned g = xcd_grid(c->n, &nb, gather_npb<R>());
     {
         Scope s(c, "k_gather");
-        const size_t lds = p.gset_ptr ? (size_t)c->gset_max * (FM == F_DIRECT ? 32 : 16) : 0;
-        if (update_x) k_gather<R, FM, true><<<g, kBlock, lds, c->stream>>>(p, nb);
-        else k_gather<R, FM, false><<<g, kBlock, lds, c->stream>>>(p, nb);
+        bool done = false;
+        if constexpr (std::is_same<R, float>::value) {
+            if (p.gset_ptr) {   // lists that stay: source sets, packed records
+                const unsigned g = xcd_grid(c->n, &nb, gather_npb(true));
+                const size_t lds = (size_t)c->gset_max * (FM == F_DIRECT ? 32 : 16);
+                if (update_x) k_gather<R, FM, true, true><<<g, kBlock, lds, c->stream>>>(p, nb);
+                else k_gather<R, FM, false, true><<<g, kBlock, lds, c->stream>>>(p, nb);
+                done = true;
+            }
+        }
+        if (!done) {
+            const unsigned g = xcd_grid(c->n, &nb, gather_npb(false));
+            if (update_x) k_gather<R, FM, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+            else k_gather<R, FM, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
+        }
     }
     {
         Scope s(c, "k_node");
