@@ -174,7 +174,9 @@ int mega_energies(mega_ctx *ctx, double *e_int, double *e_kin, double *e_ext);
  * return map used instead), 0 without the surrogate. */
 int mega_stats(mega_ctx *ctx, int64_t *nnz, int32_t *max_neighbors, int64_t *return_map_fallbacks);
 
-/* Enable (1) or disable (0) per-kernel HIP-event timing of mega_step. */
+/* Per-kernel HIP-event timing of mega_step: 0 off, 1 every kernel, 2 the
+ * force kernel only (k_force / k_pd_force; the events of every kernel add a
+ * few percent to a step). */
 int mega_profile(mega_ctx *ctx, int32_t enable);
 
 /* Average device time per launch (ms) of each kernel over the last

@@ -2101,6 +2101,7 @@ struct Scope {
     Scope(mega_ctx *ctx, const char *name) : c(ctx)
     {
         if (!c->profile) return;
+        if (c->profile == 2 && std::strcmp(name, "k_force") != 0 && std::strcmp(name, "k_pd_force") != 0) return;
         if (c->timer_used == c->timers.size()) {
             mega_ctx::Timer nt{name, nullptr, nullptr};
             if (hipEventCreate(&nt.a) != hipSuccess || hipEventCreate(&nt.b) != hipSuccess) return;
@@ -3260,7 +3261,8 @@ int mega_stats(mega_ctx *c, int64_t *nnz, int32_t *max_neighbors, int64_t *fallb
 int mega_profile(mega_ctx *c, int32_t enable)
 {
     if (!c) return MEGA_ERR_ARG;
-    c->profile = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return MEGA_ERR_ARG;
+    c->profile = enable;
     c->timer_used = 0;
     return MEGA_OK;
 }

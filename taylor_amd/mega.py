@@ -250,7 +250,9 @@ class Mega:
         _chk(self.lib.mega_stats(self.ctx, C.byref(nnz), C.byref(mx), C.byref(fb)), "mega_stats")
         return nnz.value, mx.value, fb.value
 
-    def profile(self, on: bool = True):
+    def profile(self, on: bool | int = True):
+        """Per-kernel HIP-event timing of step(): False/0 off, True/1 every
+        kernel, 2 the force kernel only."""
         _chk(self.lib.mega_profile(self.ctx, int(on)), "mega_profile")
 
     def set_surrogate(self, params, norm, props, tol: float = 1e-3):
