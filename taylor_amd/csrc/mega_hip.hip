@@ -4,14 +4,15 @@
 // One explicit step (MAIN.F90:304-488) runs as three kernels over HBM-resident
 // arrays, plus the search/shape kernels when the formulation needs them:
 //
-//   k_gather   kGroup lanes per node stream the node's neighbor list (CSR,
-//              coalesced) and gather the neighbors' increments:
+//   k_gather   16 lanes per node (8 over block source sets) stream the node's
+//              neighbor list (CSR, coalesced) and gather the neighbors' increments:
 //              F = I + sum grad phi (x) dtot, L0 = sum grad phi (x) d,
 //              u = sum phi d (DO_INTERP, fused when Lagrangian)
 //                                            FINT.F90:310-391,669-679; HANDELER.F90:114-133
 //   k_node     one lane per node: |det F|, INV3 push-forward, Hughes-Winget,
-//              rotation, elastic trial, J2 radial return, state write-back,
-//              work, and P = V J sigma F^-T
+//              rotation, elastic trial, J2 radial return (k_node_surr: the
+//              learned return map first), state write-back, work, and
+//              P = V J sigma F^-T
 //                                            FINT.F90:758-870; CONSTITUTION.F90:18-89
 //   k_force    kFGroup lanes per node over the transposed neighbor list:
 //              f_j = sum_i P_i grad phi_ij (= B^T sigma V J scattered by the
@@ -38,9 +39,10 @@
 // Pair kernels map blocks to XCDs in contiguous node ranges so that each
 // XCD's L2 holds the neighbor data its blocks gather.  For lists that stay
 // for the run (Lagrangian), each block of consecutive (Morton-ordered) nodes
-// has its set of distinct sources (k_blk_set, about a quarter of its list
+// has its set of distinct sources (k_blk_set, about a fifth of its list
 // entries): k_force stages their P rows and the mixed k_gather their
-// increments in LDS once, and the list entries name sources by set position.
+// increments in LDS once, and the list entries name sources by set position
+// (the mixed gather reads {grad phi, phi} of an entry as one 16 B record).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
