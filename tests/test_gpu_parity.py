@@ -215,6 +215,37 @@ def test_steps_with_gravity(mega, oracle, lagrangian, precision):
     g.close()
 
 
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64")])
+def test_steps_with_two_materials_and_damping(mega, oracle, lagrangian, precision):
+    """Two materials in one bar: the half at the wall has a lower yield stress
+    and mass-proportional damping (property 21; MAIN.F90:432-488), so every
+    node's return map and corrector read its own material, and both yield.
+    Against the oracle over 12 steps, bars as in the 12-step tests."""
+    prob = small_problem(lagrangian=lagrangian)
+    soft = deck.Material.j2(E=117e9, nu=0.35, rho=8930.0, sig0=250e6, B=100e6 / 250e6, CE=1.0, damping=3.0e5)
+    prob.materials = [prob.materials[0], soft]
+    zmid = 0.5 * (prob.coo[:, 2].min() + prob.coo[:, 2].max())
+    prob.mat_id = (prob.coo[:, 2] < zmid).astype(np.int32)
+    assert 0 < prob.mat_id.sum() < prob.n
+    nsteps = 12
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(nsteps)
+    A = OracleRun(oracle, prob).step(nsteps).arr
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    eps = g.download("eps")
+    print(lagrangian, precision, errs, "plastic nodes per material",
+          [int((eps[prob.mat_id == m] > 0).sum()) for m in (0, 1)])
+    assert (eps[prob.mat_id == 1] > 0).any()
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert rel_err(eps, A["state"][:, 2]) < 1e-8
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    g.close()
+
+
 def test_construct_fint_mixed_precision(mega, oracle):
     """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
     rotation) against the FP64 oracle on the same inputs: force within 1e-4
