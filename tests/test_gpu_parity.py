@@ -246,6 +246,42 @@ def test_steps_with_two_materials_and_damping(mega, oracle, lagrangian, precisio
     g.close()
 
 
+@pytest.mark.parametrize("finite,lagrangian,precision", [(0, 1, "fp64"), (0, 1, "mixed"), (0, 0, "fp64"),
+                                                         (1, 1, "fp64"), (1, 0, "fp64"), (1, 1, "mixed")])
+def test_steps_small_strain_and_energies(mega, oracle, finite, lagrangian, precision):
+    """IFINITE_STRAIN 0 (no deformation gradient; strain increments from the
+    shape-function gradients alone) and 1, against the oracle over 12 steps
+    with a body force, with the energy balance of MAIN.F90:524-553: kinetic,
+    external (trapezoid over the step), and internal (0.5 fint . dtot for
+    small strain, the accumulated stress work for finite strain).  Bars as in
+    the 12-step tests; energies within 1e-12 (FP64) and 1e-6 (mixed) of the
+    kinetic energy."""
+    prob = small_problem(lagrangian=lagrangian)
+    prob.finite_strain = finite
+    prob.gravity = np.array([0.0, 2.0e7, -5.0e7])
+    nsteps = 12
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(nsteps)
+    run = OracleRun(oracle, prob).step(nsteps)
+    A = run.arr
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    e_g = np.array(g.energies())
+    e_r = np.array([run.m.ie, run.m.ke, run.m.ee])
+    e_err = np.abs(e_g - e_r) / abs(run.m.ke)
+    print(finite, lagrangian, precision, errs, "energies", e_g, e_r, e_err)
+    assert abs(run.m.ie) > 1e-6 * abs(run.m.ke) and abs(run.m.ee) > 1e-9 * abs(run.m.ke)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert rel_err(g.download("eps"), A["state"][:, 2]) < 1e-8
+        assert e_err.max() < 1e-12
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        assert e_err.max() < 1e-6
+    g.close()
+
+
 def test_construct_fint_mixed_precision(mega, oracle):
     """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
     rotation) against the FP64 oracle on the same inputs: force within 1e-4
