@@ -93,9 +93,10 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
     if name == "k_rk":
-        # pairs: col + tpos read, phi + grad phi written in list order and
-        # {grad phi, source} in transposed order; nodes: own x, window, EBC flag, row
-        return nnz * (4 + 4 + r + 3 * r + 3 * r + 4) + n * (24 + 24 + 4 + 8)
+        # the semi-Lagrangian launch, gather sums fused: pairs: col + tpos read,
+        # phi written in list order and {grad phi, source} in transposed order;
+        # nodes: own x, window, EBC flag, row, own d (FP32 copy in mixed), sums written
+        return nnz * (4 + 4 + r + 3 * r + 4) + n * (24 + 24 + 4 + 8 + (16 if mixed else 24) + (48 if mixed else 80))
     return 0.0
 
 

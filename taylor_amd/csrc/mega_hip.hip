@@ -500,9 +500,18 @@ __device__ __forceinline__ void rk_moments(double *m, const double *H, const Ker
 // XCDs in contiguous ranges, kRkGroup lanes per node.  A lane's first kRkCache
 // neighbors are loaded together, and their kernel values and H are kept in
 // registers for the second pass; longer rows continue in a plain loop.
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
+// GATHER (lists rebuilt every step): the gather sums of CONSTRUCT_FINT without
+// the deformation gradient -- L0 = sum grad phi (x) d, mixed over d_j - d_i
+// with S = sum grad phi -- are taken here from the gradients as they are
+// formed, and the list-order gradients are not stored.  Each lane visits its
+// entries in the order k_gather's lanes do over the same list (kGroup ==
+// kRkGroup lanes per node, entry q on lane q mod 16), and the group sums are
+// the same, so the record equals k_gather's bit for bit.
+template <class R, bool GATHER>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
 {
+    static_assert(!GATHER || kRkGroup == kGroup, "the fused gather visits entries as k_gather does");
+    constexpr bool MIX = std::is_same<R, float>::value;
     constexpr int G = kRkGroup, U = kRkCache;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
@@ -587,8 +596,14 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
     }
     R *shp = as<R>(p.shp), *dshp = as<R>(p.dshp), *tshp = as<R>(p.tshp);
     TPair<R> *tpair = as<TPair<R>>(p.tpair);
+    R L[9], S[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L[k] = R(0);
+    S[0] = S[1] = S[2] = R(0);
+    float4 di = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (GATHER && MIX) di = p.d4[node];
     // phi_j = B.H phi_w, grad phi_j = (Bd_a.H + B_a) phi_w + (B.H) grad phi_w
-    auto emit = [&](int q, int t, const double *h, const KernelEval &e) {
+    auto emit = [&](int q, int t, int j, const double *h, const KernelEval &e) {
         double H[4] = {1.0, h[0], h[1], h[2]};
         if (p.ql)
 #pragma unroll
@@ -600,7 +615,21 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
         const R f = (R)(c0 * e.phi);
         const R gx = (R)(cx * e.phi + c0 * e.px), gy = (R)(cy * e.phi + c0 * e.py), gz = (R)(cz * e.phi + c0 * e.pz);
         shp[q] = f;
-        dshp[q] = gx; dshp[p.cap + q] = gy; dshp[2 * p.cap + q] = gz;
+        if constexpr (GATHER) {   // k_gather's sums, same operands and order
+            R dj[3];
+            if constexpr (MIX) {
+                const float4 dv = p.d4[j];
+                dj[0] = dv.x - di.x; dj[1] = dv.y - di.y; dj[2] = dv.z - di.z;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dj[k] = p.d[3 * j + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { L[k] += gx * dj[k]; L[3 + k] += gy * dj[k]; L[6 + k] += gz * dj[k]; }
+            if (MIX) { S[0] += gx; S[1] += gy; S[2] += gz; }
+        } else {
+            dshp[q] = gx; dshp[p.cap + q] = gy; dshp[2 * p.cap + q] = gz;
+        }
         if (p.tpos) {
             TPair<R> v;
             v.gx = gx; v.gy = gy; v.gz = gz; v.src = node;
@@ -614,13 +643,31 @@ __global__ __launch_bounds__(kBlock) void k_rk(Params p, const double *xs, const
         if (jv[u] >= 0) {
             KernelEval e;
             e.phi = c_ph[u]; e.px = c_px[u]; e.py = c_py[u]; e.pz = c_pz[u];
-            emit(p0 + lane + u * G, tp[u], c_h[u], e);
+            emit(p0 + lane + u * G, tp[u], jv[u], c_h[u], e);
         }
     for (int q = p0 + lane + U * G; q < p1; q += G) {
         const int j = p.col[q];
         const KernelEval e = rk_window(p, xi, xs, j);
         const double h[3] = {xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
-        emit(q, p.tpos ? p.tpos[q] : 0, h, e);
+        emit(q, p.tpos ? p.tpos[q] : 0, j, h, e);
+    }
+    if constexpr (GATHER) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L[k] = group_sum<G>(L[k]);
+        if (MIX) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S[k] = group_sum<G>(S[k]);
+        }
+        constexpr int W = red_stride<R, false>();
+        if (lane == 0) {
+            R rec[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) rec[k] = R(0);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) rec[k] = L[k];
+            if constexpr (MIX) { rec[9] = S[0]; rec[10] = S[1]; rec[11] = S[2]; }
+            store_vec<R, W>(as<R>(p.red) + (int64_t)W * node, rec);
+        }
     }
 }
 
@@ -2020,6 +2067,7 @@ struct mega_ctx {
     int64_t step = 0;
     int search_count = 0;
     int has_lists = 0, has_shapes = 0, predicted = 0;
+    int dshp_stale = 0;   // the last shapes formed the gather sums and left the list-order gradients unwritten
     double dt = 0.0;
     int has_gravity = 0;
     std::vector<Material> host_mats;
@@ -2344,28 +2392,36 @@ int build_sets(mega_ctx *c)
     return MEGA_OK;
 }
 
-int do_shapes(mega_ctx *c, const double *xs)
+// gather: also form the gather sums of do_fint(F_NONE) (then called with
+// gathered = true), without storing the list-order gradients
+int do_shapes(mega_ctx *c, const double *xs, bool gather)
 {
     if (!c->has_lists) return MEGA_ERR_ARG;
     Params p = params(c);
     Scope s(c, "k_rk");
     const int nb = (int)(((int64_t)c->n * kRkGroup + kBlock - 1) / kBlock);
     const unsigned grid = (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
-    if (c->mixed) k_rk<float><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
-    else k_rk<double><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+    if (c->mixed) {
+        if (gather) k_rk<float, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+        else k_rk<float, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+    } else {
+        if (gather) k_rk<double, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+        else k_rk<double, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+    }
     HIPCHK(hipGetLastError());
     const int r = build_sets(c);
     if (r) return r;
     c->has_shapes = 1;
+    c->dshp_stale = gather ? 1 : 0;
     return MEGA_OK;
 }
 
 template <class R, int FM>
-int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
+int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode, bool gathered)
 {
     Params p = params(c);
     int nb = 0;
-    {
+    if (!gathered) {
         Scope s(c, "k_gather");
         bool done = false;
         if constexpr (std::is_same<R, float>::value) {
@@ -2407,22 +2463,24 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode)
 }
 
 template <class R>
-int fint_kernels(mega_ctx *c, int fm, bool update_x, int force_mode)
+int fint_kernels(mega_ctx *c, int fm, bool update_x, int force_mode, bool gathered)
 {
-    if (fm == F_DIRECT) return fint_kernels_fm<R, F_DIRECT>(c, update_x, force_mode);
-    if (fm == F_INCR) return fint_kernels_fm<R, F_INCR>(c, update_x, force_mode);
-    return fint_kernels_fm<R, F_NONE>(c, update_x, force_mode);
+    if (fm == F_DIRECT) return fint_kernels_fm<R, F_DIRECT>(c, update_x, force_mode, false);
+    if (fm == F_INCR) return fint_kernels_fm<R, F_INCR>(c, update_x, force_mode, false);
+    return fint_kernels_fm<R, F_NONE>(c, update_x, force_mode, gathered);
 }
 
 // fm: F_NONE (semi-Lagrangian / small strain), F_DIRECT (F from dtot), F_INCR
 // (F carried per node); force_mode: -1 no force, 0 force only, 1 force +
-// corrector, 2 + next predictor
-int do_fint(mega_ctx *c, int fm, bool update_x, int force_mode)
+// corrector, 2 + next predictor; gathered: the gather sums are already formed
+// (do_shapes with gather, F_NONE without the coordinate update only)
+int do_fint(mega_ctx *c, int fm, bool update_x, int force_mode, bool gathered = false)
 {
     if (!c->has_shapes) return MEGA_ERR_ARG;
     if (fm == F_INCR && !c->F_valid) fm = F_DIRECT;
-    return c->mixed ? fint_kernels<float>(c, fm, update_x, force_mode)
-                    : fint_kernels<double>(c, fm, update_x, force_mode);
+    if (gathered && (fm != F_NONE || update_x)) return MEGA_ERR_ARG;
+    return c->mixed ? fint_kernels<float>(c, fm, update_x, force_mode, gathered)
+                    : fint_kernels<double>(c, fm, update_x, force_mode, gathered);
 }
 
 // UDFM_SHAPE_TENSOR for all nodes (FINT_PD.F90:227-290)
@@ -2863,7 +2921,7 @@ int mega_shapes(mega_ctx *c, const double *x_eval)
         HIPCHK(hipMemcpyAsync(c->scratch, xi.data(), sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
         xs = c->scratch;
     }
-    int r = do_shapes(c, xs);
+    int r = do_shapes(c, xs, false);
     if (r) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     return MEGA_OK;
@@ -2872,6 +2930,10 @@ int mega_shapes(mega_ctx *c, const double *x_eval)
 int mega_get_shapes(mega_ctx *c, double *shp, double *dshp, int64_t cap)
 {
     if (!c || !c->has_shapes || cap < c->nnz) return MEGA_ERR_ARG;
+    if (c->dshp_stale && dshp) {   // form them again at the coordinates the lists were taken at
+        const int r = do_shapes(c, c->x, false);
+        if (r) return r;
+    }
     const int64_t m = c->nnz;
     double *buf = nullptr;
     if (dalloc(&buf, 3 * (size_t)m)) return MEGA_ERR_NOMEM;
@@ -2993,7 +3055,7 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
         if (linit) r = do_pd_shapes(c);
         if (!r) r = do_pd_fint(c, linit ? 1 : 0, 0);
     } else {
-        if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x);
+        if (linit || !lag) r = do_shapes(c, lag ? c->X0 : c->x, false);
         else if (!c->has_shapes) r = MEGA_ERR_ARG;
         if (!r) r = do_fint(c, (lag && c->c.finite_strain) ? F_DIRECT : F_NONE, false, 0);
     }
@@ -3082,7 +3144,7 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
                 if (!r) r = do_pd_shapes(c);
                 if (!r) r = do_pd_fint(c, 1, -1);
             } else {
-                if (!r) r = do_shapes(c, lag ? c->X0 : c->x);
+                if (!r) r = do_shapes(c, lag ? c->X0 : c->x, false);
                 if (!r) r = do_fint(c, fm == F_INCR ? F_DIRECT : F_NONE, false, -1);
             }
             if (r) return r;
@@ -3118,8 +3180,8 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
                 else k_interp<double, true><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, nullptr, nullptr);
             }
             r = search_step(c, c->x, first);
-            if (!r) r = do_shapes(c, c->x);
-            if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2);
+            if (!r) r = do_shapes(c, c->x, true);   // with the gather sums
+            if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2, true);
         }
         if (r) return r;
         c->predicted = last ? 0 : 1;
