@@ -90,7 +90,10 @@ int mega_get_neighbors(mega_ctx *ctx, int32_t *gn, int32_t *gstart, int32_t *gst
  * (NULL: initial coordinates), storing shapes on the device (GSTACK_SHP,
  * GSTACK_DSHP).  RK.f90:16-453 called from FINT.F90:406-414. */
 int mega_shapes(mega_ctx *ctx, const double *x_eval);
-/* Export stored shapes: shp[nnz], dshp[3 nnz], in neighbor-list order. */
+/* Export stored shapes: shp[nnz], dshp[3 nnz], in neighbor-list order.
+ * Semi-Lagrangian steps form the gather sums from the gradients without
+ * storing them; after such steps the gradients are formed again at the
+ * current coordinates (those of the last step's search) before the export. */
 int mega_get_shapes(mega_ctx *ctx, double *shp, double *dshp, int64_t cap);
 
 /* With peridynamics on (FINT_PD.F90:14-759): UDFM_SHAPE_TENSOR results,

@@ -506,7 +506,9 @@ __device__ __forceinline__ void rk_moments(double *m, const double *H, const Ker
 // formed, and the list-order gradients are not stored.  Each lane visits its
 // entries in the order k_gather's lanes do over the same list (kGroup ==
 // kRkGroup lanes per node, entry q on lane q mod 16), and the group sums are
-// the same, so the record equals k_gather's bit for bit.
+// the same: the record is k_gather's, in mixed precision bit for bit (in FP64
+// the gradients' own products may be contracted differently, a last-bit
+// difference).
 template <class R, bool GATHER>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
 {

@@ -157,6 +157,32 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_semi_lagrangian_shapes_after_steps(mega, oracle, precision):
+    """Semi-Lagrangian steps form the gather sums inside the shape-function
+    kernel and leave the list-order gradients unstored; the export forms them
+    again at the coordinates of the last search.  Lists, shapes and gradients
+    after 12 steps against the oracle's current ones; the export does not
+    change the run (the next steps match the oracle too)."""
+    prob = small_problem(lagrangian=0)
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(12)
+    shp, dshp = g.get_shapes()
+    run = OracleRun(oracle, prob).step(12)
+    rp, col, shp_r, dshp_r = run.csr()
+    gn, _, st = g.neighbors()
+    assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    print(precision, rel_err(shp, shp_r), rel_err(dshp, dshp_r))
+    assert rel_err(shp, shp_r) < tol and rel_err(dshp, dshp_r) < tol
+    g.step(3)
+    run.step(3)
+    bar = 1e-12 if precision == "fp64" else 1e-7
+    assert rel_err(g.download("x"), run.arr["x"]) < bar
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("norm_win", [2.2, 4.4])
 def test_steps_with_wide_supports(mega, oracle, norm_win, precision):
     """Lagrangian steps with wide supports against the oracle.  The force and
