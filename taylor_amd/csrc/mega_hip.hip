@@ -152,7 +152,7 @@ struct Params {
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
     // node arrays, FP64 masters
-    const double *X0, *win, *vol, *mass;
+    const double *X0, *win, *winv, *vol, *mass;   // winv: 1 / win
     double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
     double *Fst;                                     // deformation gradient per node (9, column-major, + 1 pad)
     float4 *d4, *t4;                                 // FP32 gather copies of d and dtot (mixed)
@@ -422,15 +422,19 @@ struct KernelEval {
     double phi, px, py, pz;
 };
 
-// window of neighbor j (position xj, window w, essential-BC flag ebc) at x_i
+// window of neighbor j (position xj, essential-BC flag ebc) at x_i; w holds
+// its windows a_j (ellipsoidal support) or their reciprocals 1/a_j (box
+// support, rk_windows()), so that the box kernel divides nowhere
+__device__ __forceinline__ const double *rk_windows(const Params &p) { return p.shsup ? p.win : p.winv; }
+
 __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double *xi, const double *xj,
                                                    const double *w, int ebc)
 {
     const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-    const double w1 = w[0], w2 = w[1], w3 = w[2];
-    const double zx = dx / w1, zy = dy / w2, zz = dz / w3;
     KernelEval e;
     if (p.shsup) {
+        const double w1 = w[0], w2 = w[1], w3 = w[2];
+        const double zx = dx / w1, zy = dy / w2, zz = dz / w3;
         const double dia = sqrt(zx * zx + zy * zy + zz * zz);
         double k0, k1;
         mega::cubic_kernel(dia, k0, k1);
@@ -438,11 +442,13 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
         if (!(dia <= 1.0e-13)) { rx = dx / (w1 * w1) / dia; ry = dy / (w2 * w2) / dia; rz = dz / (w3 * w3) / dia; }
         e.phi = k0; e.px = k1 * rx; e.py = k1 * ry; e.pz = k1 * rz;
     } else {
+        const double r1 = w[0], r2 = w[1], r3 = w[2];
+        const double zx = dx * r1, zy = dy * r2, zz = dz * r3;
         double fx, fxx, fy, fyy, fz, fzz;
         mega::cubic_kernel(fabs(zx), fx, fxx);
         mega::cubic_kernel(fabs(zy), fy, fyy);
         mega::cubic_kernel(fabs(zz), fz, fzz);
-        const double iw = 1.0 / w1;   // RK.f90:183-205 divides all three by the x window
+        const double iw = r1;   // RK.f90:183-205 divides all three by the x window
         const double rx = zx == 0.0 ? 0.0 : (zx >= 0.0 ? iw : -iw);
         const double ry = zy == 0.0 ? 0.0 : (zy >= 0.0 ? iw : -iw);
         const double rz = zz == 0.0 ? 0.0 : (zz >= 0.0 ? iw : -iw);
@@ -451,9 +457,9 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
         if (ebc) {   // singular kernel at essential-BC nodes (RK.f90:217-228)
             const double wm = 1.0 / (sqrt(zx * zx + zy * zy + zz * zz) + mega::kSingularEps);
             const double pw3 = ph * wm * wm * wm;
-            px = px * wm - pw3 * zx / w1;
-            py = py * wm - pw3 * zy / w2;
-            pz = pz * wm - pw3 * zz / w3;
+            px = px * wm - pw3 * zx * r1;
+            py = py * wm - pw3 * zy * r2;
+            pz = pz * wm - pw3 * zz * r3;
             ph = ph * wm;
         }
         e.phi = ph; e.px = px; e.py = py; e.pz = pz;
@@ -464,12 +470,13 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
 __device__ __forceinline__ KernelEval rk_window(const Params &p, const double *xi, const double *xs, int j)
 {
     const double xj[3] = {xs[3 * j], xs[3 * j + 1], xs[3 * j + 2]};
-    const double w[3] = {p.win[3 * j], p.win[3 * j + 1], p.win[3 * j + 2]};
+    const double *wp = rk_windows(p);
+    const double w[3] = {wp[3 * j], wp[3 * j + 1], wp[3 * j + 2]};
     return rk_window_at(p, xi, xj, w, p.ebc_node[j]);
 }
 
-// one neighbor's terms of the moment matrix M (m[0..9]), its derivatives M_a
-// (m[10a..10a+9]) and sum phi (m[40]); H = (1, x_i - x_j).
+// one neighbor's terms of the moment matrix M (m[0..9]) and its derivatives
+// M_a (m[10a..10a+9]); H = (1, x_i - x_j), so m[0] is also sum phi.
 //   M_a(r,c) += (dH_r H_c + H_r dH_c) phi + H_r H_c phi_a, dH the unit vector e_a;
 // the product terms of dH vanish except on row or column a and are left out
 // there (their zero addends do not change the sum).
@@ -493,7 +500,6 @@ __device__ __forceinline__ void rk_moments(double *m, const double *H, const Ker
                 m[10 * a + t] += d;
             }
         }
-    m[40] += e.phi;
 }
 
 // Nodes are taken in bin order (order = the bins' node lists), blocks mapped to
@@ -523,9 +529,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     const int node = order ? order[g] : g;
     const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
     const double xi[3] = {xs[3 * node], xs[3 * node + 1], xs[3 * node + 2]};
-    double m[41];   // 10 unique entries each of M, Mx, My, Mz + sum of phi
+    double m[40];   // 10 unique entries each of M, Mx, My, Mz
 #pragma unroll
-    for (int k = 0; k < 41; ++k) m[k] = 0.0;
+    for (int k = 0; k < 40; ++k) m[k] = 0.0;
     int jv[U], tp[U];
     double c_ph[U], c_px[U], c_py[U], c_pz[U], c_h[U][3];
 #pragma unroll
@@ -538,7 +544,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     for (int u = 0; u < U; ++u) {
         const int jj = jv[u] < 0 ? node : jv[u];
         const double xj[3] = {xs[3 * jj], xs[3 * jj + 1], xs[3 * jj + 2]};
-        const double w[3] = {p.win[3 * jj], p.win[3 * jj + 1], p.win[3 * jj + 2]};
+        const double *wp = rk_windows(p);
+        const double w[3] = {wp[3 * jj], wp[3 * jj + 1], wp[3 * jj + 2]};
         const KernelEval e = rk_window_at(p, xi, xj, w, p.ebc_node[jj]);
         c_ph[u] = e.phi; c_px[u] = e.px; c_py[u] = e.py; c_pz[u] = e.pz;
         c_h[u][0] = xi[0] - xj[0]; c_h[u][1] = xi[1] - xj[1]; c_h[u][2] = xi[2] - xj[2];
@@ -554,7 +561,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         rk_moments(m, H, e);
     }
 #pragma unroll
-    for (int k = 0; k < 41; ++k) m[k] = group_sum<kRkGroup>(m[k]);
+    for (int k = 0; k < 40; ++k) m[k] = group_sum<kRkGroup>(m[k]);
     double hs[4] = {0, 0, 0, 0};
     if (p.ql) {   // quasi-linear augmentation (RK.f90:251-303, 357-377)
         const double L = p.ql_len;
@@ -577,8 +584,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 #pragma unroll
             for (int k = 0; k < 4; ++k) hs[k] += H[k];
         }
+        const double sphi = m[0];   // sum of phi: H_0 = 1
 #pragma unroll
-        for (int t = 0; t < 10; ++t) m[t] = m[t] + ms[t] * m[40] * p.ql_coef;
+        for (int t = 0; t < 10; ++t) m[t] = m[t] + ms[t] * sphi * p.ql_coef;
     }
     double Mi[16];
     mega::m44inv_sym(m, Mi);
@@ -2030,6 +2038,7 @@ struct mega_ctx {
     // FP64 node arrays
     double *X0 = nullptr, *x = nullptr, *v = nullptr, *a = nullptr, *d = nullptr, *dtot = nullptr, *uphy = nullptr;
     double *fint = nullptr, *fext = nullptr, *fext_nmo = nullptr, *win = nullptr, *vol = nullptr, *mass = nullptr;
+    double *winv = nullptr;   // 1 / win, for the box-support kernel
     double *wave_vel = nullptr, *scratch = nullptr;   // scratch: 6n doubles
     float4 *d4 = nullptr, *t4 = nullptr;             // FP32 gather copies of d, dtot (mixed)
     double *Fst = nullptr;                           // deformation gradient per node
@@ -2128,7 +2137,7 @@ Params params(mega_ctx *c)
         if (c->mixed) { p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc; p.fpair = c->fpair; }
     }
     p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
-    p.X0 = c->X0; p.win = c->win; p.vol = c->vol; p.mass = c->mass;
+    p.X0 = c->X0; p.win = c->win; p.winv = c->winv; p.vol = c->vol; p.mass = c->mass;
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
     p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
     p.d4 = c->d4; p.t4 = c->t4; p.Fst = c->Fst;
@@ -2781,7 +2790,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     else e |= dalloc(&c->wave_vel, n);
     e |= dalloc(&c->Fst, 10 * (size_t)n);
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
-    e |= dalloc(&c->win, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
+    e |= dalloc(&c->win, n3); e |= dalloc(&c->winv, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
     e |= dalloc(&c->mats, c->host_mats.size());
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
@@ -2815,7 +2824,10 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     const auto X0i = to_internal(c, X0, 3), wini = to_internal(c, win, 3), voli = to_internal(c, vol, 1);
     const auto massi = to_internal(c, mass, 3);
     const auto ebci = to_internal(c, ebc, 3), ebni = to_internal(c, ebc_node, 1), midi = to_internal(c, mid.data(), 1);
+    std::vector<double> winvi(n3);
+    for (size_t k = 0; k < n3; ++k) winvi[k] = 1.0 / wini[k];
     bool ok = up(c->X0, X0i.data(), n3 * 8) && up(c->x, X0i.data(), n3 * 8) && up(c->win, wini.data(), n3 * 8) &&
+              up(c->winv, winvi.data(), n3 * 8) &&
               up(c->vol, voli.data(), (size_t)n * 8) && up(c->mass, massi.data(), n3 * 8) &&
               up(c->ebc, ebci.data(), n3 * 4) && up(c->ebc_node, ebni.data(), (size_t)n * 4) &&
               up(c->mat, midi.data(), (size_t)n * 4) && up(c->orig, c->perm.data(), (size_t)n * 4) &&
@@ -2840,7 +2852,7 @@ int mega_destroy(mega_ctx *c)
     if (!c) return MEGA_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_pairs(c);
-    void *ptrs[] = {c->X0, c->x, c->v, c->a, c->d, c->dtot, c->uphy, c->fint, c->fext, c->fext_nmo, c->win,
+    void *ptrs[] = {c->X0, c->x, c->v, c->a, c->d, c->dtot, c->uphy, c->fint, c->fext, c->fext_nmo, c->win, c->winv,
                     c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->Fst, c->ebc, c->ebc_node, c->mat,
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
