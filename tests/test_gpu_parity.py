@@ -156,6 +156,33 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
         assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
 
 
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
+def test_steps_on_a_tiny_bar(mega, oracle, lagrangian, precision):
+    """A bar of 27 nodes: one partly filled block in every kernel, a grid
+    padded up to the XCD count with most blocks leaving at once, one source
+    set per launch, and (semi-Lagrangian) a handful of bins.  12 steps
+    against the oracle, bars as in the 12-step tests."""
+    prob = deck.taylor_problem(1.0e-3, radius=1.2e-3, length=2.5e-3, seed=0, lagrangian=lagrangian,
+                               pdsearch_steps=3)
+    assert prob.n == 27
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(12)
+    run = OracleRun(oracle, prob).step(12)
+    A = run.arr
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    print(lagrangian, precision, errs)
+    gn, _, st = g.neighbors()
+    rp, col, _, _ = run.csr()
+    assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    g.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_semi_lagrangian_shapes_after_steps(mega, oracle, precision):
     """Semi-Lagrangian steps form the gather sums inside the shape-function
