@@ -514,7 +514,8 @@ __device__ __forceinline__ void rk_moments(double *m, const double *H, const Ker
 // kRkGroup lanes per node, entry q on lane q mod 16), and the group sums are
 // the same: the record is k_gather's, in mixed precision bit for bit (in FP64
 // the gradients' own products may be contracted differently, a last-bit
-// difference).
+// difference).  Two waves per SIMD are kept by bound: the fused sums' few
+// extra registers would otherwise move the kernel to one (4-8 VGPRs spill).
 template <class R, bool GATHER>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
 {
