@@ -423,10 +423,14 @@ struct KernelEval {
 };
 
 // window of neighbor j (position xj, essential-BC flag ebc) at x_i; w holds
-// its windows a_j (ellipsoidal support) or their reciprocals 1/a_j (box
-// support, rk_windows()), so that the box kernel divides nowhere
-__device__ __forceinline__ const double *rk_windows(const Params &p) { return p.shsup ? p.win : p.winv; }
+// its windows a_j, or with RCP and the box support their reciprocals 1/a_j
+// (rk_windows<RCP>), so that the box kernel divides nowhere.  RCP is taken in
+// mixed precision only: FP64 keeps the reference's divisions, operation for
+// operation (products with 1/a differ in the last bit).
+template <bool RCP>
+__device__ __forceinline__ const double *rk_windows(const Params &p) { return (RCP && !p.shsup) ? p.winv : p.win; }
 
+template <bool RCP>
 __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double *xi, const double *xj,
                                                    const double *w, int ebc)
 {
@@ -442,13 +446,13 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
         if (!(dia <= 1.0e-13)) { rx = dx / (w1 * w1) / dia; ry = dy / (w2 * w2) / dia; rz = dz / (w3 * w3) / dia; }
         e.phi = k0; e.px = k1 * rx; e.py = k1 * ry; e.pz = k1 * rz;
     } else {
-        const double r1 = w[0], r2 = w[1], r3 = w[2];
-        const double zx = dx * r1, zy = dy * r2, zz = dz * r3;
+        const double w1 = w[0], w2 = w[1], w3 = w[2];
+        const double zx = RCP ? dx * w1 : dx / w1, zy = RCP ? dy * w2 : dy / w2, zz = RCP ? dz * w3 : dz / w3;
         double fx, fxx, fy, fyy, fz, fzz;
         mega::cubic_kernel(fabs(zx), fx, fxx);
         mega::cubic_kernel(fabs(zy), fy, fyy);
         mega::cubic_kernel(fabs(zz), fz, fzz);
-        const double iw = r1;   // RK.f90:183-205 divides all three by the x window
+        const double iw = RCP ? w1 : 1.0 / w1;   // RK.f90:183-205 divides all three by the x window
         const double rx = zx == 0.0 ? 0.0 : (zx >= 0.0 ? iw : -iw);
         const double ry = zy == 0.0 ? 0.0 : (zy >= 0.0 ? iw : -iw);
         const double rz = zz == 0.0 ? 0.0 : (zz >= 0.0 ? iw : -iw);
@@ -457,9 +461,9 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
         if (ebc) {   // singular kernel at essential-BC nodes (RK.f90:217-228)
             const double wm = 1.0 / (sqrt(zx * zx + zy * zy + zz * zz) + mega::kSingularEps);
             const double pw3 = ph * wm * wm * wm;
-            px = px * wm - pw3 * zx * r1;
-            py = py * wm - pw3 * zy * r2;
-            pz = pz * wm - pw3 * zz * r3;
+            px = px * wm - (RCP ? pw3 * zx * w1 : pw3 * zx / w1);
+            py = py * wm - (RCP ? pw3 * zy * w2 : pw3 * zy / w2);
+            pz = pz * wm - (RCP ? pw3 * zz * w3 : pw3 * zz / w3);
             ph = ph * wm;
         }
         e.phi = ph; e.px = px; e.py = py; e.pz = pz;
@@ -467,12 +471,13 @@ __device__ __forceinline__ KernelEval rk_window_at(const Params &p, const double
     return e;
 }
 
+template <bool RCP>
 __device__ __forceinline__ KernelEval rk_window(const Params &p, const double *xi, const double *xs, int j)
 {
     const double xj[3] = {xs[3 * j], xs[3 * j + 1], xs[3 * j + 2]};
-    const double *wp = rk_windows(p);
+    const double *wp = rk_windows<RCP>(p);
     const double w[3] = {wp[3 * j], wp[3 * j + 1], wp[3 * j + 2]};
-    return rk_window_at(p, xi, xj, w, p.ebc_node[j]);
+    return rk_window_at<RCP>(p, xi, xj, w, p.ebc_node[j]);
 }
 
 // one neighbor's terms of the moment matrix M (m[0..9]) and its derivatives
@@ -545,9 +550,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     for (int u = 0; u < U; ++u) {
         const int jj = jv[u] < 0 ? node : jv[u];
         const double xj[3] = {xs[3 * jj], xs[3 * jj + 1], xs[3 * jj + 2]};
-        const double *wp = rk_windows(p);
+        const double *wp = rk_windows<MIX>(p);
         const double w[3] = {wp[3 * jj], wp[3 * jj + 1], wp[3 * jj + 2]};
-        const KernelEval e = rk_window_at(p, xi, xj, w, p.ebc_node[jj]);
+        const KernelEval e = rk_window_at<MIX>(p, xi, xj, w, p.ebc_node[jj]);
         c_ph[u] = e.phi; c_px[u] = e.px; c_py[u] = e.py; c_pz[u] = e.pz;
         c_h[u][0] = xi[0] - xj[0]; c_h[u][1] = xi[1] - xj[1]; c_h[u][2] = xi[2] - xj[2];
         if (jv[u] >= 0) {
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
     for (int q = p0 + lane + U * G; q < p1; q += G) {
         const int j = p.col[q];
-        const KernelEval e = rk_window(p, xi, xs, j);
+        const KernelEval e = rk_window<MIX>(p, xi, xs, j);
         const double H[4] = {1.0, xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
         rk_moments(m, H, e);
     }
@@ -658,7 +663,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         }
     for (int q = p0 + lane + U * G; q < p1; q += G) {
         const int j = p.col[q];
-        const KernelEval e = rk_window(p, xi, xs, j);
+        const KernelEval e = rk_window<MIX>(p, xi, xs, j);
         const double h[3] = {xi[0] - xs[3 * j], xi[1] - xs[3 * j + 1], xi[2] - xs[3 * j + 2]};
         emit(q, p.tpos ? p.tpos[q] : 0, j, h, e);
     }
