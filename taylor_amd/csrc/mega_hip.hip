@@ -393,10 +393,25 @@ __global__ __launch_bounds__(kBlock) void k_interp(Params p, const double *coef,
     const double *src = UPDATE ? p.d : coef;
     const R *shp = as<R>(p.shp);
     double u0 = 0, u1 = 0, u2 = 0;
-    for (int q = p0 + lane; q < p1; q += kGroup) {
-        const int j = p.col[q];
-        const double f = (double)shp[q];
-        u0 += f * src[3 * j]; u1 += f * src[3 * j + 1]; u2 += f * src[3 * j + 2];
+    // each lane issues the loads of up to U entries before using any of them;
+    // the sums still run over its entries in list order
+    constexpr int U = kUnrollF64;
+    for (int base = p0 + lane; base < p1; base += kGroup * U) {
+        int j[U];
+        double f[U], s[U][3];
+#pragma unroll
+        for (int e = 0; e < U; ++e) {
+            const int q = base + kGroup * e;
+            j[e] = q < p1 ? p.col[q] : node;
+            f[e] = q < p1 ? (double)shp[q] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < U; ++e) {
+            s[e][0] = src[3 * j[e]]; s[e][1] = src[3 * j[e] + 1]; s[e][2] = src[3 * j[e] + 2];
+        }
+#pragma unroll
+        for (int e = 0; e < U; ++e)
+            if (base + kGroup * e < p1) { u0 += f[e] * s[e][0]; u1 += f[e] * s[e][1]; u2 += f[e] * s[e][2]; }
     }
     u0 = group_sum<kGroup>(u0); u1 = group_sum<kGroup>(u1); u2 = group_sum<kGroup>(u2);
     if (lane < 3) {
