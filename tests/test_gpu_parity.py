@@ -157,6 +157,32 @@ def test_explicit_steps_match_oracle(mega, oracle, lagrangian):
 
 
 @pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
+def test_steps_with_ellipsoidal_support(mega, oracle, lagrangian, precision):
+    """SHSUP = 1: the ellipsoidal support in the search and the radial window
+    phi(|z|) with its gradient in RK (RK.f90:136-165), against the oracle over
+    12 steps; lists equal, bars as in the 12-step tests."""
+    prob = small_problem(lagrangian=lagrangian)
+    prob.shsup = 1
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(12)
+    run = OracleRun(oracle, prob).step(12)
+    A = run.arr
+    gn, _, st = g.neighbors()
+    rp, col, _, _ = run.csr()
+    assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    print(lagrangian, precision, "neighbors", int(gn.min()), int(gn.max()), errs)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert rel_err(g.download("eps"), A["state"][:, 2]) < 1e-8
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    g.close()
+
+
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
 def test_steps_on_a_tiny_bar(mega, oracle, lagrangian, precision):
     """A bar of 27 nodes: one partly filled block in every kernel, a grid
     padded up to the XCD count with most blocks leaving at once, one source
