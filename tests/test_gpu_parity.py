@@ -182,6 +182,37 @@ def test_steps_with_ellipsoidal_support(mega, oracle, lagrangian, precision):
     g.close()
 
 
+@pytest.mark.parametrize("case,lagrangian,precision", [("no_ql", 1, "fp64"), ("no_ql", 1, "mixed"), ("no_ql", 0, "fp64"),
+                                                       ("elastic", 1, "fp64"), ("elastic", 1, "mixed"),
+                                                       ("elastic", 0, "fp64")])
+def test_steps_without_ql_and_with_elastic_material(mega, oracle, case, lagrangian, precision):
+    """QL = 0: RK1 without the quasi-linear augmentation of the moment matrix
+    (RK.f90:251-303 and 357-377 skipped); material type 1: the elastic trial
+    stress is the answer (CONSTITUTION.F90:46-52), no plastic strain.  12 steps against the
+    oracle, bars as in the 12-step tests."""
+    if case == "no_ql":
+        prob = small_problem(lagrangian=lagrangian, ql=0)
+    else:
+        prob = small_problem(lagrangian=lagrangian, material=deck.Material.elastic(117e9, 0.35, 8930.0))
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    g = mega.Mega(prob, precision=prec)
+    g.step(12)
+    A = OracleRun(oracle, prob).step(12).arr
+    errs = {k: rel_err(g.download(k), A[r])
+            for k, r in (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    eps = g.download("eps")
+    print(case, lagrangian, precision, errs, "max eps", eps.max())
+    if case == "elastic":
+        assert eps.max() == 0.0 and A["state"][:, 2].max() == 0.0
+    else:
+        assert A["state"][:, 2].max() > 0.0
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+    else:
+        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    g.close()
+
+
 @pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
 def test_steps_on_a_tiny_bar(mega, oracle, lagrangian, precision):
     """A bar of 27 nodes: one partly filled block in every kernel, a grid
