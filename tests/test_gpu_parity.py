@@ -214,6 +214,26 @@ def test_steps_without_ql_and_with_elastic_material(mega, oracle, case, lagrangi
 
 
 @pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
+def test_steps_split_over_calls(mega, lagrangian, precision):
+    """Within one mega_step call the next predictor is fused into the force
+    kernel; the last step of a call leaves it to the next call's own predictor
+    launch.  12 steps as 12, as 5 + 7 and as 1 + 4 + 7 give the same state bit
+    for bit, and the same energies."""
+    prob = small_problem(lagrangian=lagrangian)
+    prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    out = []
+    for split in ((12,), (5, 7), (1, 4, 7)):
+        g = mega.Mega(prob, precision=prec)
+        for k in split:
+            g.step(k)
+        out.append(({k: g.download(k) for k in ("x", "v", "uphy", "stress", "eps", "fint")}, g.energies()))
+        g.close()
+    for st, en in out[1:]:
+        assert all(np.array_equal(st[k], out[0][0][k]) for k in st), [k for k in st if not np.array_equal(st[k], out[0][0][k])]
+        assert en == out[0][1]
+
+
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
 def test_steps_on_a_tiny_bar(mega, oracle, lagrangian, precision):
     """A bar of 27 nodes: one partly filled block in every kernel, a grid
     padded up to the XCD count with most blocks leaving at once, one source
