@@ -84,6 +84,9 @@ static_assert(MEGA_RK_CACHE >= 1, "the shape-function kernel keeps at least one 
 #ifndef MEGA_UNROLL_SETS
 #define MEGA_UNROLL_SETS 6
 #endif
+#ifndef MEGA_UNROLL_SETS_F64
+#define MEGA_UNROLL_SETS_F64 4
+#endif
 #ifndef MEGA_UNROLL_MIX
 #define MEGA_UNROLL_MIX 4
 #endif
@@ -108,6 +111,7 @@ constexpr int kRkCache = MEGA_RK_CACHE;   // kernel evaluations per lane kept be
 constexpr int kXcd = 8;               // XCDs on MI355X
 // neighbor-list entries each lane keeps in flight in the gathers
 constexpr int kUnrollMix = MEGA_UNROLL_MIX, kUnrollF64 = MEGA_UNROLL_F64, kUnrollSets = MEGA_UNROLL_SETS;
+constexpr int kUnrollSetsF64 = MEGA_UNROLL_SETS_F64;
 constexpr int kFUnrollMix = MEGA_FUNROLL_MIX, kFUnrollF64 = MEGA_FUNROLL_F64;
 constexpr int kNodesPerBlock = kBlock / kGroup;    // nodes of one block in the gathers
 // lanes per node and nodes per block of the RK gather (k_gather)
@@ -149,6 +153,7 @@ struct Params {
     const int32_t *gset_ptr, *gset, *fset_ptr, *fset;
     const uint16_t *gloc;                            // forward slot -> position in its block's set
     const float4 *fpair;                             // mixed, with gloc: {grad phi, phi} per forward slot
+    const double2 *fpair2;                           // FP64, with gloc: {gx, gy}, {gz, phi} per forward slot
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
     // node arrays, FP64 masters
@@ -720,24 +725,34 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
 {
     constexpr bool MIX = std::is_same<R, float>::value;
     constexpr bool DIR = FM == F_DIRECT;
-    static_assert(!SETS || MIX, "source sets of the gather in mixed precision");
-    constexpr int U = SETS ? kUnrollSets : (MIX ? kUnrollMix : kUnrollF64);
+    constexpr int U = SETS ? (MIX ? kUnrollSets : kUnrollSetsF64) : (MIX ? kUnrollMix : kUnrollF64);
     constexpr int GG = gather_group(SETS);
     extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
     const int node = blk * gather_npb(SETS) + threadIdx.x / GG;
     const int lane = threadIdx.x & (GG - 1);
-    // mixed, lists that stay: the block's distinct neighbors' d (and dtot) in LDS
+    // lists that stay: the block's distinct neighbors' d (and dtot) in LDS,
+    // mixed as the FP32 copies (16 B), FP64 as {x, y}, {z, 0} (32 B)
     int m = 0;
     float4 *Ld = reinterpret_cast<float4 *>(mega_dyn);
+    double2 *Ld2 = reinterpret_cast<double2 *>(mega_dyn);
     if (SETS) {
         const int s0 = p.gset_ptr[blk];
         m = p.gset_ptr[blk + 1] - s0;
         for (int k = threadIdx.x; k < m; k += kBlock) {
             const int j = p.gset[s0 + k];
-            Ld[k] = p.d4[j];
-            if (DIR) Ld[m + k] = p.t4[j];
+            if constexpr (MIX) {
+                Ld[k] = p.d4[j];
+                if (DIR) Ld[m + k] = p.t4[j];
+            } else {
+                Ld2[2 * k] = make_double2(p.d[3 * j], p.d[3 * j + 1]);
+                Ld2[2 * k + 1] = make_double2(p.d[3 * j + 2], 0.0);
+                if (DIR) {
+                    Ld2[2 * (m + k)] = make_double2(p.dtot[3 * j], p.dtot[3 * j + 1]);
+                    Ld2[2 * (m + k) + 1] = make_double2(p.dtot[3 * j + 2], 0.0);
+                }
+            }
         }
         if (m > 0) __syncthreads();
     }
@@ -773,10 +788,17 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
         for (int e = 0; e < U; ++e) {
             const int q = base + GG * e;
             const bool ok = q < p1;
-            if (LDS) {   // mixed, lists that stay: the packed record
-                const float4 r = ok ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                g0[e] = (R)r.x; g1[e] = (R)r.y; g2[e] = (R)r.z;
-                f[e] = UPDATE_X ? (R)r.w : R(0);
+            if (LDS) {   // lists that stay: the packed record
+                if constexpr (MIX) {
+                    const float4 r = ok ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    g0[e] = (R)r.x; g1[e] = (R)r.y; g2[e] = (R)r.z;
+                    f[e] = UPDATE_X ? (R)r.w : R(0);
+                } else {
+                    const double2 a = ok ? p.fpair2[2 * (int64_t)q] : make_double2(0.0, 0.0);
+                    const double2 b = ok ? p.fpair2[2 * (int64_t)q + 1] : make_double2(0.0, 0.0);
+                    g0[e] = a.x; g1[e] = a.y; g2[e] = b.x;
+                    f[e] = UPDATE_X ? b.y : R(0);
+                }
             } else {
                 g0[e] = ok ? gx[q] : R(0);
                 g1[e] = ok ? gy[q] : R(0);
@@ -789,6 +811,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
                 if (DIR) {
                     const float4 tv = LDS ? Ld[m + j[e]] : p.t4[j[e]];
                     tj[e][0] = tv.x - ti.x; tj[e][1] = tv.y - ti.y; tj[e][2] = tv.z - ti.z;
+                }
+            } else if (LDS) {
+                const double2 a = Ld2[2 * j[e]], b = Ld2[2 * j[e] + 1];
+                dj[e][0] = a.x; dj[e][1] = a.y; dj[e][2] = b.x;
+                if (DIR) {
+                    const double2 c = Ld2[2 * (m + j[e])], d = Ld2[2 * (m + j[e]) + 1];
+                    tj[e][0] = c.x; tj[e][1] = c.y; tj[e][2] = d.x;
                 }
             } else {
 #pragma unroll
@@ -1975,6 +2004,14 @@ __global__ void k_fpair(int64_t nnz, int64_t cap, const float *shp, const float 
     out[q] = make_float4(dshp[q], dshp[cap + q], dshp[2 * cap + q], shp[q]);
 }
 
+__global__ void k_fpair2(int64_t nnz, int64_t cap, const double *shp, const double *dshp, double2 *out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    out[2 * q] = make_double2(dshp[q], dshp[cap + q]);
+    out[2 * q + 1] = make_double2(dshp[2 * cap + q], shp[q]);
+}
+
 __global__ void k_cells_ne(int64_t ncell, const int32_t *cell_start, const int32_t *pos, int32_t *cells)
 {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2082,6 +2119,8 @@ struct mega_ctx {
     int64_t gset_cap = 0, fset_cap = 0, gloc_cap = 0;
     float4 *fpair = nullptr;   // mixed: {grad phi, phi} per forward slot, beside gloc
     int64_t fpair_cap = 0;
+    double2 *fpair2 = nullptr; // FP64: {gx, gy}, {gz, phi} per forward slot, beside gloc
+    int64_t fpair2_cap = 0;
     int sets_on = 0, gset_max = 0, fset_max = 0;
     // bins
     BinGeom geom{};
@@ -2155,7 +2194,9 @@ Params params(mega_ctx *c)
     p.row_ptr = c->row_ptr; p.col = c->col; p.trow = c->trow; p.tsrc = c->tsrc; p.tpos = c->tpos;
     if (c->sets_on) {
         p.fset_ptr = c->fset_ptr; p.fset = c->fset;
-        if (c->mixed) { p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc; p.fpair = c->fpair; }
+        p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc;
+        if (c->mixed) p.fpair = c->fpair;
+        else p.fpair2 = c->fpair2;
     }
     p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.winv = c->winv; p.vol = c->vol; p.mass = c->mass;
@@ -2375,8 +2416,9 @@ int build_set(mega_ctx *c, int npb, const int32_t *rowp, const int32_t *vals, in
 }
 
 // source sets of lists that stay for the run (the Lagrangian RK lists): the
-// transposed sets for the force, in mixed precision the forward sets for the
-// gather; after the records are written, as FILL rewrites their src field
+// transposed sets for the force and the forward sets for the gather, with the
+// gather's packed {grad phi, phi} records; after the records are written, as
+// FILL rewrites their src field
 int build_sets(mega_ctx *c)
 {
     c->sets_on = 0;
@@ -2386,8 +2428,8 @@ int build_sets(mega_ctx *c)
     int e = 0;
     if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nbf + 1);
     if (!c->set_max) e |= dalloc(&c->set_max, 1);
-    if (c->mixed && !c->gset_ptr) e |= dalloc(&c->gset_ptr, nbg + 1);
-    if (c->mixed && c->gloc_cap < c->nnz) {
+    if (!c->gset_ptr) e |= dalloc(&c->gset_ptr, nbg + 1);
+    if (c->gloc_cap < c->nnz) {
         (void)hipFree(c->gloc);
         c->gloc = nullptr;
         c->gloc_cap = 0;
@@ -2400,6 +2442,13 @@ int build_sets(mega_ctx *c)
         c->fpair_cap = 0;
         e |= dalloc(&c->fpair, c->nnz_cap);
         if (!e) c->fpair_cap = c->nnz_cap;
+    }
+    if (!c->mixed && c->fpair2_cap < c->nnz) {
+        (void)hipFree(c->fpair2);
+        c->fpair2 = nullptr;
+        c->fpair2_cap = 0;
+        e |= dalloc(&c->fpair2, 2 * (size_t)c->nnz_cap);
+        if (!e) c->fpair2_cap = c->nnz_cap;
     }
     if (e) return MEGA_ERR_NOMEM;
     Scope s(c, "k_blk_set");
@@ -2418,6 +2467,14 @@ int build_sets(mega_ctx *c)
     } else {
         r = build_set<double>(c, kNodesPerFBlock, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
                               static_cast<TPair<double> *>(c->tpair), &c->fset_max);
+        if (!r)
+            r = build_set<double>(c, gather_npb(true), c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap,
+                                  c->gloc, nullptr, &c->gset_max);
+        if (!r) {
+            k_fpair2<<<grid_for(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->nnz_cap, (const double *)c->shp,
+                                                                (const double *)c->dshp, c->fpair2);
+            HIPCHK(hipGetLastError());
+        }
     }
     if (r) return r;
     c->sets_on = 1;
@@ -2456,14 +2513,13 @@ int fint_kernels_fm(mega_ctx *c, bool update_x, int force_mode, bool gathered)
     if (!gathered) {
         Scope s(c, "k_gather");
         bool done = false;
-        if constexpr (std::is_same<R, float>::value) {
-            if (p.gset_ptr) {   // lists that stay: source sets, packed records
-                const unsigned g = xcd_grid(c->n, &nb, gather_npb(true));
-                const size_t lds = (size_t)c->gset_max * (FM == F_DIRECT ? 32 : 16);
-                if (update_x) k_gather<R, FM, true, true><<<g, kBlock, lds, c->stream>>>(p, nb);
-                else k_gather<R, FM, false, true><<<g, kBlock, lds, c->stream>>>(p, nb);
-                done = true;
-            }
+        if (p.gset_ptr) {   // lists that stay: source sets, packed records
+            const unsigned g = xcd_grid(c->n, &nb, gather_npb(true));
+            const size_t row = std::is_same<R, float>::value ? 16 : 32;   // d (and dtot) per source
+            const size_t lds = (size_t)c->gset_max * (FM == F_DIRECT ? 2 * row : row);
+            if (update_x) k_gather<R, FM, true, true><<<g, kBlock, lds, c->stream>>>(p, nb);
+            else k_gather<R, FM, false, true><<<g, kBlock, lds, c->stream>>>(p, nb);
+            done = true;
         }
         if (!done) {
             const unsigned g = xcd_grid(c->n, &nb, gather_npb(false));
@@ -2879,7 +2935,7 @@ int mega_destroy(mega_ctx *c)
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
                     c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
-                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair};
+                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair, c->fpair2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
