@@ -1725,7 +1725,10 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
 // FILL writes col from row_ptr[i] on, tpos (transposed slot of each list
 // entry) and, when given, tsrc (source of each transposed slot) and tq (list
 // position of each transposed slot, for k_tsort).
-constexpr int kCandPerLane = 4;
+#ifndef MEGA_CAND_PER_LANE
+#define MEGA_CAND_PER_LANE 4
+#endif
+constexpr int kCandPerLane = MEGA_CAND_PER_LANE;   // candidates per lane in one pass of a bin's wave
 constexpr int kMaskCap = 8;   // ballot masks kept per node by COUNT for FILL: its first 512 candidates
 
 template <bool FILL>
