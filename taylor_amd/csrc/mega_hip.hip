@@ -1729,7 +1729,10 @@ __global__ void k_bin_sort(int64_t ncell, const int32_t *cell_start, const int32
 #define MEGA_CAND_PER_LANE 4
 #endif
 constexpr int kCandPerLane = MEGA_CAND_PER_LANE;   // candidates per lane in one pass of a bin's wave
-constexpr int kMaskCap = 8;   // ballot masks kept per node by COUNT for FILL: its first 512 candidates
+#ifndef MEGA_MASK_CAP
+#define MEGA_MASK_CAP 8
+#endif
+constexpr int kMaskCap = MEGA_MASK_CAP;   // ballot masks kept per node by COUNT for FILL: its first 64 kMaskCap candidates
 
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *cells, const double *x,
