@@ -24,6 +24,8 @@
 //   k_search   SOFT_SEARCH (+BIN_INODE)        BIN_SEARCH.F90:69-133,153-328
 //              and, in the same two passes, the transposed neighbor lists
 //   k_rk       RK1 shape functions/gradients   RK.f90:16-453, KERNEL.F90:3-140
+//              (lists rebuilt every step: with the gather sums of k_gather,
+//              which then does not run)
 //
 // Precision: R = double (MEGA_FP64) or float (MEGA_MIXED) for shape data,
 // stress/strain/plastic strain, the return map and the force; positions,
@@ -40,9 +42,9 @@
 // XCD's L2 holds the neighbor data its blocks gather.  For lists that stay
 // for the run (Lagrangian), each block of consecutive (Morton-ordered) nodes
 // has its set of distinct sources (k_blk_set, about a fifth of its list
-// entries): k_force stages their P rows and the mixed k_gather their
-// increments in LDS once, and the list entries name sources by set position
-// (the mixed gather reads {grad phi, phi} of an entry as one 16 B record).
+// entries): k_force stages their P rows and k_gather their increments in
+// LDS once, and the list entries name sources by set position (the gather
+// reads {grad phi, phi} of an entry as one record, 16 B mixed, 32 B FP64).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
