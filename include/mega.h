@@ -32,7 +32,10 @@ enum {
     MEGA_OK = 0,
     MEGA_ERR_RETURN_MAP = 1,   /* J2 Newton > 20 iterations       CONSTITUTE_VONMISES.F90:115-119 */
     MEGA_ERR_SINGULAR = 2,     /* singular matrix (warning only)  INVERSE.F90:41-45,93-95         */
-    MEGA_ERR_BINS = 3,         /* bin table inconsistent          BIN_SEARCH.F90:48-61            */
+    MEGA_ERR_BINS = 3,         /* bin grid (NBINSX*NBINSY*NBINSZ) or neighbor-list total beyond the
+                                  int32 index range.  The reference STOPs at 1000 nodes per bin
+                                  and 10000 candidates (BIN_SEARCH.F90:48-51,117-120); the device
+                                  tables (counting sort, CSR) have neither cap. */
     MEGA_ERR_NOMEM = 4,
     MEGA_ERR_HIP = 5,
     MEGA_ERR_ARG = 6,          /* invalid argument / unsupported option                         */
@@ -223,9 +226,14 @@ int mega_mlp_gradient(mega_mlp *m, const float *xhat, const float *that, int64_t
 
 /* Use the network in the context's steps (MEGA_MIXED only) for the J2 nodes
  * whose property row equals prop[30]: a plastic trial state inside the
- * envelope takes the network's answer when its yield residual
- * |sqrt(3/2)||dev sigma|| - sigma_y(eqps)| <= tol sigma_y(eqps) and eqps does
- * not decrease; otherwise the exact return map runs.  params NULL disables. */
+ * envelope takes the network's answer when the residual of the radial-return
+ * equations is <= tol sigma_y(eqps) and eqps does not decrease; otherwise the
+ * exact return map runs.  The residual is the largest of four parts, each in
+ * units of sigma_y(eqps): the yield condition |sqrt(3/2)||dev sigma|| -
+ * sigma_y(eqps)|, the flow direction ||dev sigma - (||dev sigma|| /
+ * ||dev sigma_trial||) dev sigma_trial||, the pressure (against the trial
+ * pressure) and the plastic multiplier |(||dev sigma_trial|| - ||dev sigma||)
+ * - 2 mu sqrt(3/2) (eqps - eqps_n)|.  params NULL disables. */
 int mega_set_surrogate(mega_ctx *ctx, const float *params, const float *norm, const double *prop, double tol);
 /* counters since the last reset: [0] plastic trial states of the trained
  * material, [1] outside the envelope, [2] evaluated by the network, [3] of

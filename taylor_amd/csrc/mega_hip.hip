@@ -2225,10 +2225,11 @@ Params params(mega_ctx *c)
     return p;
 }
 
-// event-bracketed launch for the per-kernel timing of mega_step
+// event-bracketed launch for the per-kernel timing of mega_step; keeps an
+// index into c->timers, since a nested scope may grow (and move) the vector
 struct Scope {
     mega_ctx *c;
-    mega_ctx::Timer *t = nullptr;
+    size_t idx = SIZE_MAX;
     Scope(mega_ctx *ctx, const char *name) : c(ctx)
     {
         if (!c->profile) return;
@@ -2238,13 +2239,13 @@ struct Scope {
             if (hipEventCreate(&nt.a) != hipSuccess || hipEventCreate(&nt.b) != hipSuccess) return;
             c->timers.push_back(nt);
         }
-        t = &c->timers[c->timer_used++];
-        t->name = name;
-        (void)hipEventRecord(t->a, c->stream);
+        idx = c->timer_used++;
+        c->timers[idx].name = name;
+        (void)hipEventRecord(c->timers[idx].a, c->stream);
     }
     ~Scope()
     {
-        if (t) (void)hipEventRecord(t->b, c->stream);
+        if (idx != SIZE_MAX) (void)hipEventRecord(c->timers[idx].b, c->stream);
     }
 };
 
@@ -2351,7 +2352,7 @@ int do_search(mega_ctx *c, const double *x)
     int32_t nnz32 = 0;
     HIPCHK(hipMemcpyAsync(&nnz32, c->row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (nnz32 < 0) return MEGA_ERR_ARG;
+    if (nnz32 < 0) return MEGA_ERR_BINS;   // list total beyond the int32 index range of the tables
     r = ensure_pairs(c, nnz32);
     if (r) return r;
     c->nnz = nnz32;
@@ -2495,17 +2496,19 @@ int do_shapes(mega_ctx *c, const double *xs, bool gather)
 {
     if (!c->has_lists) return MEGA_ERR_ARG;
     Params p = params(c);
-    Scope s(c, "k_rk");
-    const int nb = (int)(((int64_t)c->n * kRkGroup + kBlock - 1) / kBlock);
-    const unsigned grid = (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
-    if (c->mixed) {
-        if (gather) k_rk<float, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
-        else k_rk<float, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
-    } else {
-        if (gather) k_rk<double, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
-        else k_rk<double, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+    {
+        Scope s(c, "k_rk");
+        const int nb = (int)(((int64_t)c->n * kRkGroup + kBlock - 1) / kBlock);
+        const unsigned grid = (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
+        if (c->mixed) {
+            if (gather) k_rk<float, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+            else k_rk<float, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+        } else {
+            if (gather) k_rk<double, true><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+            else k_rk<double, false><<<grid, kBlock, 0, c->stream>>>(p, xs, c->cell_nodes, nb);
+        }
+        HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
     const int r = build_sets(c);
     if (r) return r;
     c->has_shapes = 1;
@@ -2796,7 +2799,7 @@ const char *mega_status_string(int s)
     case MEGA_OK: return "ok";
     case MEGA_ERR_RETURN_MAP: return "plasticity material model did not converge";
     case MEGA_ERR_SINGULAR: return "problem inverting matrix";
-    case MEGA_ERR_BINS: return "bin table inconsistent";
+    case MEGA_ERR_BINS: return "bin grid or neighbor-list total beyond the int32 index range";
     case MEGA_ERR_NOMEM: return "device allocation failed";
     case MEGA_ERR_HIP: return "HIP runtime error";
     case MEGA_ERR_ARG: return "invalid argument or unsupported option";
@@ -2863,7 +2866,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         c->geom.bin[k] = bin[k];
     }
     c->ncell = (int64_t)c->geom.nb[0] * c->geom.nb[1] * c->geom.nb[2];
-    if (c->ncell <= 0 || c->ncell >= INT32_MAX) { delete c; return MEGA_ERR_ARG; }
+    if (c->ncell <= 0 || c->ncell >= INT32_MAX) { delete c; return MEGA_ERR_BINS; }   // bin grid beyond int32
 
     const size_t n3 = 3 * (size_t)n, n6 = 6 * (size_t)n, r = c->rsz;
     int e = 0;
