@@ -39,7 +39,8 @@ enum {
     MEGA_ERR_NOMEM = 4,
     MEGA_ERR_HIP = 5,
     MEGA_ERR_ARG = 6,          /* invalid argument / unsupported option                         */
-    MEGA_ERR_MATERIAL = 7      /* material type not on this path  CONSTITUTION.F90:79-84          */
+    MEGA_ERR_MATERIAL = 7,     /* material type not on this path  CONSTITUTION.F90:79-84          */
+    MEGA_ERR_SCRATCH = 8       /* input larger than a no-allocation entry point's fixed scratch    */
 };
 
 /* precision modes */
@@ -141,6 +142,23 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
  * names a node outside 1..nump. */
 int mega_preprocess(int32_t nump, const double *coo, int32_t numel, const int32_t *elcon, const int32_t *hex,
                     double *win, double *vol, double *sm_len, double *dist_max);
+
+/* No counterpart in the reference: a variant of mega_preprocess for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same work, same results bit for
+ * bit, but coo, elcon, hex, win, vol and sm_len are DEVICE pointers the caller
+ * owns (hex and sm_len may be NULL), dist_max[3] is host memory, and every
+ * intermediate buffer lives in a fixed device array inside the library (about
+ * 40 MB per GPU, held until the process ends).  Sizes above
+ * MEGA_PRE_NOALLOC_MAX_NODES nodes or MEGA_PRE_NOALLOC_MAX_ELEMS elements
+ * return MEGA_ERR_SCRATCH.  Runs on the current device's null stream and
+ * synchronises it.  Calls that overlap on one GPU share that array and race;
+ * calls on different GPUs do not.  Calls no hipMalloc, hipFree or allocating
+ * library routine. */
+#define MEGA_PRE_NOALLOC_MAX_NODES 524288
+#define MEGA_PRE_NOALLOC_MAX_ELEMS 524288
+int mega_preprocess_noalloc(int32_t nump, const double *coo, int32_t numel, const int32_t *elcon,
+                            const int32_t *hex, double *win, double *vol, double *sm_len, double *dist_max);
 
 /* nsteps explicit steps of size dt, device resident: predictor, DO_INTERP,
  * coordinate update, (search + shapes when semi-Lagrangian), CONSTRUCT_FINT,

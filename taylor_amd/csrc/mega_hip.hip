@@ -2804,6 +2804,7 @@ const char *mega_status_string(int s)
     case MEGA_ERR_HIP: return "HIP runtime error";
     case MEGA_ERR_ARG: return "invalid argument or unsupported option";
     case MEGA_ERR_MATERIAL: return "invalid material type on this path";
+    case MEGA_ERR_SCRATCH: return "input larger than the fixed scratch of a no-allocation entry point";
     default: return "unknown status";
     }
 }

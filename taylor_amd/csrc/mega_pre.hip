@@ -25,6 +25,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "ddmath.hpp"
@@ -267,5 +268,151 @@ done:
                     (void *)d_cursor, (void *)d_inc, (void *)d_bad, tmp})
         if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
+    return status;
+}
+
+// ---------------------------------------------------------------------------
+// mega_preprocess_noalloc: the same kernels with every intermediate buffer in
+// one file-scope device array (no hipMalloc / hipFree, no stream creation).
+//
+// The array lives on each GPU the library's code object is loaded on, for the
+// life of the process.  Host code never passes its name to a kernel: the
+// name is a host-side stand-in, not a device address.  The device address is
+// taken once per device with hipGetSymbolAddress (device current) and kept in
+// pre_scratch_ptr[device].  Every call on one GPU shares that GPU's array, so
+// calls that overlap on one GPU race; what must start at zero (the corner
+// counts and the bad-id flag) is cleared by a fill kernel at each call.
+namespace {
+
+constexpr int64_t kNoAllocMaxNodes = MEGA_PRE_NOALLOC_MAX_NODES;
+constexpr int64_t kNoAllocMaxElems = MEGA_PRE_NOALLOC_MAX_ELEMS;
+constexpr int64_t kNoAllocCubBytes = 256 * 1024;   // hipcub scan temporary storage
+constexpr int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t kNoAllocBlocks = (kNoAllocMaxNodes + kPreBlock - 1) / kPreBlock;
+// ext[3 numel] + share[numel] (doubles), inc[8 numel], count/start/cursor[nump + 1],
+// bad, dpart[3 blocks], dmax[3], cub temporary storage
+constexpr int64_t kNoAllocBytes =
+    align256(8 * 3 * kNoAllocMaxElems) + align256(8 * kNoAllocMaxElems) + align256(4 * 8 * kNoAllocMaxElems) +
+    3 * align256(4 * (kNoAllocMaxNodes + 1)) + align256(4) + align256(8 * 3 * kNoAllocBlocks) + align256(8 * 3) +
+    kNoAllocCubBytes;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+void *pre_scratch_ptr[kMaxDevices];
+std::mutex pre_scratch_mu;
+
+// device address of this device's scratch array (looked up once per device)
+int pre_scratch(void **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return MEGA_ERR_HIP;
+    std::lock_guard<std::mutex> lk(pre_scratch_mu);
+    if (!pre_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "mega_preprocess_noalloc: hipGetSymbolAddress failed: %s\n", hipGetErrorString(e));
+            return MEGA_ERR_HIP;
+        }
+        pre_scratch_ptr[dev] = p;
+    }
+    *out = pre_scratch_ptr[dev];
+    return MEGA_OK;
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t *p, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// XDIST_MAX, YDIST_MAX, ZDIST_MAX from the per-block maxima, starting from 0
+// as PRE_MODEL_SMOOTHING_LENGTHS.F90:45-47 does; one block
+__global__ void k_pre_dmax(int nblocks, const double *dpart, double *dmax)
+{
+    __shared__ double s_max[3][kPreBlock];
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += kPreBlock)
+        for (int a = 0; a < 3; ++a)
+            if (dpart[3 * b + a] > d[a]) d[a] = dpart[3 * b + a];
+    for (int a = 0; a < 3; ++a) s_max[a][threadIdx.x] = d[a];
+    __syncthreads();
+    for (int h = kPreBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int a = 0; a < 3; ++a)
+                if (s_max[a][threadIdx.x + h] > s_max[a][threadIdx.x]) s_max[a][threadIdx.x] = s_max[a][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) dmax[a] = s_max[a][0];
+}
+
+}  // namespace
+
+extern "C" int mega_preprocess_noalloc(int32_t nump, const double *coo, int32_t numel, const int32_t *elcon,
+                                       const int32_t *hex, double *win, double *vol, double *sm_len,
+                                       double *dist_max)
+{
+    if (nump <= 0 || numel < 0 || !coo || (numel > 0 && !elcon) || !win || !vol || !dist_max) return MEGA_ERR_ARG;
+    if (nump > kNoAllocMaxNodes || numel > kNoAllocMaxElems) {
+        std::fprintf(stderr,
+                     "mega_preprocess_noalloc: %d nodes / %d elements exceed the scratch maximum of %lld / %lld "
+                     "(use mega_preprocess)\n",
+                     nump, numel, (long long)kNoAllocMaxNodes, (long long)kNoAllocMaxElems);
+        return MEGA_ERR_SCRATCH;
+    }
+    int status = MEGA_OK;
+    void *base = nullptr;
+    if ((status = pre_scratch(&base)) != MEGA_OK) return status;
+    const hipStream_t st = nullptr;   // the device's null stream
+    unsigned char *q = static_cast<unsigned char *>(base);
+    auto take = [&q](int64_t bytes) { unsigned char *p = q; q += align256(bytes); return (void *)p; };
+    double *d_ext = (double *)take(8 * 3 * kNoAllocMaxElems);
+    double *d_share = (double *)take(8 * kNoAllocMaxElems);
+    int32_t *d_inc = (int32_t *)take(4 * 8 * kNoAllocMaxElems);
+    int32_t *d_count = (int32_t *)take(4 * (kNoAllocMaxNodes + 1));
+    int32_t *d_start = (int32_t *)take(4 * (kNoAllocMaxNodes + 1));
+    int32_t *d_cursor = (int32_t *)take(4 * (kNoAllocMaxNodes + 1));
+    int32_t *d_bad = (int32_t *)take(4);
+    double *d_dpart = (double *)take(8 * 3 * kNoAllocBlocks);
+    double *d_dmax = (double *)take(8 * 3);
+    void *d_tmp = take(kNoAllocCubBytes);
+    size_t tmp_bytes = 0;
+    const int nbe = (numel + kPreBlock - 1) / kPreBlock, nbn = (nump + kPreBlock - 1) / kPreBlock;
+    int32_t bad = 0;
+    k_fill_i32<<<(unsigned)((nump + 1 + kPreBlock - 1) / kPreBlock), kPreBlock, 0, st>>>(nump + 1, d_count, 0);
+    k_fill_i32<<<1, kPreBlock, 0, st>>>(1, d_bad, 0);
+    PRECHK(hipGetLastError());
+    if (numel > 0) {
+        k_pre_elem<<<nbe, kPreBlock, 0, st>>>(numel, nump, coo, elcon, hex, d_ext, d_share, d_count, d_bad);
+        PRECHK(hipGetLastError());
+    }
+    PRECHK(hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PRECHK(hipStreamSynchronize(st));
+    if (bad) {
+        status = MEGA_ERR_ARG;   // an element names a node that does not exist
+        goto done;
+    }
+    PRECHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_count, d_start, nump + 1, st));
+    if ((int64_t)tmp_bytes > kNoAllocCubBytes) {
+        std::fprintf(stderr, "mega_preprocess_noalloc: scan storage %zu B exceeds the reserved %lld B\n", tmp_bytes,
+                     (long long)kNoAllocCubBytes);
+        status = MEGA_ERR_SCRATCH;
+        goto done;
+    }
+    PRECHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_count, d_start, nump + 1, st));
+    PRECHK(hipMemcpyAsync(d_cursor, d_start, sizeof(int32_t) * (size_t)nump, hipMemcpyDeviceToDevice, st));
+    if (numel > 0) {
+        k_pre_fill<<<nbe, kPreBlock, 0, st>>>(numel, elcon, hex, d_cursor, d_inc);
+        PRECHK(hipGetLastError());
+    }
+    k_pre_node<<<nbn, kPreBlock, 0, st>>>(nump, d_start, d_inc, d_ext, d_share, win, vol, sm_len, d_dpart);
+    PRECHK(hipGetLastError());
+    k_pre_dmax<<<1, kPreBlock, 0, st>>>(nbn, d_dpart, d_dmax);
+    PRECHK(hipGetLastError());
+    PRECHK(hipMemcpyAsync(dist_max, d_dmax, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    PRECHK(hipStreamSynchronize(st));
+done:
     return status;
 }

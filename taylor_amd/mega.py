@@ -74,6 +74,7 @@ def load_library():
         "mega_fint": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, dp],
         "mega_constitute": [i32, vp, vp, vp, vp, vp, i32, i32, i32],
         "mega_preprocess": [i32, vp, i32, vp, vp, vp, vp, vp, vp],
+        "mega_preprocess_noalloc": [i32, vp, i32, vp, vp, vp, vp, vp, vp],
         "mega_step": [vp, C.c_double, i32],
         "mega_download": [vp, i32, vp],
         "mega_upload": [vp, i32, vp],
