@@ -32,10 +32,12 @@ enum {
     MEGA_OK = 0,
     MEGA_ERR_RETURN_MAP = 1,   /* J2 Newton > 20 iterations       CONSTITUTE_VONMISES.F90:115-119 */
     MEGA_ERR_SINGULAR = 2,     /* singular matrix (warning only)  INVERSE.F90:41-45,93-95         */
-    MEGA_ERR_BINS = 3,         /* bin grid (NBINSX*NBINSY*NBINSZ) or neighbor-list total beyond the
-                                  int32 index range.  The reference STOPs at 1000 nodes per bin
-                                  and 10000 candidates (BIN_SEARCH.F90:48-51,117-120); the device
-                                  tables (counting sort, CSR) have neither cap. */
+    MEGA_ERR_BINS = 3,         /* a bin holds MAX_NEIGH = 1000 nodes, where GET_BINS stops
+                                  (BIN_SEARCH.F90:47-52, HANDELER.F90:245): nodes that left the
+                                  bar or became non-finite pile into the edge bins; or the bin
+                                  grid (NBINSX*NBINSY*NBINSZ) or neighbor-list total is beyond
+                                  the int32 index range.  The reference's 10000-candidate stop
+                                  (BIN_SEARCH.F90:117-120) has no device counterpart. */
     MEGA_ERR_NOMEM = 4,
     MEGA_ERR_HIP = 5,
     MEGA_ERR_ARG = 6,          /* invalid argument / unsupported option                         */

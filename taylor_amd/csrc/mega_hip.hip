@@ -1664,20 +1664,27 @@ struct BinGeom {
     int nb[3];
 };
 
-__global__ void k_bin_assign(int n, const double *x, BinGeom g, int32_t *ijk, int32_t *cell_count)
+// the reference's bin capacity: GET_BINS stops once a bin holds MAX_NEIGH
+// nodes (HANDELER.F90:245, BIN_SEARCH.F90:47-52)
+constexpr int kMaxNodesInBin = 1000;
+
+__global__ void k_bin_assign(int n, const double *x, BinGeom g, int32_t *ijk, int32_t *cell_count, int32_t *full)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        int s = (int)floor((x[3 * i + k] - g.mn[k]) / g.bin[k]) + 1;
-        s = s < 1 ? 1 : (s > g.nb[k] ? g.nb[k] : s);
-        c[k] = s;
-        ijk[3 * i + k] = s;
+        // clamped to 1..NBINS before the integer conversion, so that a NaN
+        // (to 1) or a far-away node stays in the table (BIN_SEARCH.F90:28-41)
+        double s = floor((x[3 * i + k] - g.mn[k]) / g.bin[k]) + 1.0;
+        s = s >= 1.0 ? s : 1.0;
+        s = s <= (double)g.nb[k] ? s : (double)g.nb[k];
+        c[k] = (int)s;
+        ijk[3 * i + k] = c[k];
     }
     const int64_t m = (int64_t)(c[2] - 1) * g.nb[0] * g.nb[1] + (int64_t)(c[1] - 1) * g.nb[0] + (c[0] - 1);
-    atomicAdd(&cell_count[m], 1);
+    if (atomicAdd(&cell_count[m], 1) + 1 >= kMaxNodesInBin) atomicOr(full, 1);
 }
 
 __global__ void k_bin_fill(int n, const int32_t *ijk, BinGeom g, int32_t *cursor, int32_t *cell_nodes)
@@ -2311,7 +2318,8 @@ int do_bins(mega_ctx *c, const double *x)
     const int n = c->n;
     Scope s(c, "k_bin");
     HIPCHK(hipMemsetAsync(c->cell_cursor, 0, sizeof(int32_t) * (c->ncell + 1), c->stream));
-    k_bin_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, x, c->geom, c->ijk, c->cell_cursor);
+    HIPCHK(hipMemsetAsync(c->flags + 3, 0, sizeof(int32_t), c->stream));
+    k_bin_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, x, c->geom, c->ijk, c->cell_cursor, c->flags + 3);
     int r = cub_scan(c, c->cell_cursor, c->cell_start, c->ncell + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(c->cell_cursor, c->cell_start, sizeof(int32_t) * c->ncell, hipMemcpyDeviceToDevice, c->stream));
@@ -2321,11 +2329,15 @@ int do_bins(mega_ctx *c, const double *x)
     r = cub_scan(c, c->cell_flag, c->cell_cursor, c->ncell + 1);   // positions of the non-empty bins
     if (r) return r;
     k_cells_ne<<<grid_for(c->ncell), kBlock, 0, c->stream>>>(c->ncell, c->cell_start, c->cell_cursor, c->cells_ne);
-    int32_t nne = 0;
+    int32_t nne = 0, full = 0;
     HIPCHK(hipMemcpyAsync(&nne, c->cell_cursor + c->ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&full, c->flags + 3, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->ncell_ne = nne;
     HIPCHK(hipGetLastError());
+    // a bin at the reference's capacity: it stops here (BIN_SEARCH.F90:48-51); the
+    // one-wave-per-bin search would also grow with the square of such a bin
+    if (full) return MEGA_ERR_BINS;
     return MEGA_OK;
 }
 
@@ -2799,7 +2811,7 @@ const char *mega_status_string(int s)
     case MEGA_OK: return "ok";
     case MEGA_ERR_RETURN_MAP: return "plasticity material model did not converge";
     case MEGA_ERR_SINGULAR: return "problem inverting matrix";
-    case MEGA_ERR_BINS: return "bin grid or neighbor-list total beyond the int32 index range";
+    case MEGA_ERR_BINS: return "a bin holds 1000 nodes, or the bin grid or list total exceeds int32";
     case MEGA_ERR_NOMEM: return "device allocation failed";
     case MEGA_ERR_HIP: return "HIP runtime error";
     case MEGA_ERR_ARG: return "invalid argument or unsupported option";

@@ -11,6 +11,7 @@ compressed .npz files.  The fixtures are data only.
     python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
     python tests/golden/make_golden.py --damage   # material types 6 and 3
     python tests/golden/make_golden.py --vtk      # the reference's VTK frames of a small deck
+    python tests/golden/make_golden.py --trajectory [--semi]   # 181k-node bar, 1,400 / 500 steps, 8 threads
 """
 from __future__ import annotations
 
@@ -367,6 +368,83 @@ def final_state_vectors(workdir):
             "eps": fr["eps"]}
 
 
+def trajectory_vectors(workdir, lagrangian, nsteps=1700, every=100, threads=8):
+    """The 181k-node Taylor bar (spacing 0.19 mm, jitter seed 1, 190 m/s, copper J2
+    linear hardening) run by the reference program for ``nsteps`` steps with a
+    full-precision frame every ``every`` steps, Lagrangian or semi-Lagrangian
+    with bins rebuilt every 20 steps (PDSEARCH = ceiling(PDSTIME/DLT),
+    MAIN.F90:430).  Stored per frame: the number of steps done, the bar's length max z (the wall plane is
+    z = 0), the impact face's radius, the largest physical speed, the
+    volume-weighted L2 norm of the plastic strain and the positions of every
+    353rd node; per step, the kinetic energy MEGA logs to
+    ENERGY_INFO.OUT (MAIN.F90:245,555, five digits); the number of steps finished and, if the
+    program stopped itself, EXIT_PROGRAM's reason.  The run is left to go
+    where it goes."""
+    h = 0.19e-3
+    mesh = deck.taylor_bar_mesh(h, seed=1)
+    prob = deck.taylor_problem(h, seed=1, lagrangian=lagrangian, pdsearch_steps=20)
+    kfile = os.path.join(workdir, "bar.k")
+    done_mark = os.path.join(workdir, "finished")
+    rc = 0
+    if os.path.exists(done_mark):   # a finished run in a kept working directory is read again
+        rc = int(open(done_mark).read())
+    else:
+        deck.write_k(kfile, mesh)
+        prob.time_end = prob.dt * (nsteps - 0.5)
+        deck.write_control(os.path.join(workdir, "Control.dat"), kfile, prob, ["BAR", "WALL"],
+                           time_output=prob.dt * (every - 0.5), out_vars="unformatted\ndisplacement\nvelocity\neps")
+        with open(os.path.join(workdir, "mega.out"), "w") as fo:
+            rc = subprocess.run([REF_MEGA, "-j", str(threads)], cwd=workdir, stdin=subprocess.DEVNULL, stdout=fo,
+                                stderr=fo, timeout=4 * 3600).returncode
+        with open(done_mark, "w") as f:
+            f.write(str(rc))
+    # MEGA prints exodusStep and STEPS at a frame; STEPS counts from 0 and is
+    # raised after the output (MAIN.F90:274,632-647), so the frame holds STEPS + 1 steps
+    steps = {}
+    with open(os.path.join(workdir, "mega.out")) as f:
+        for ln in f:
+            p = ln.split()
+            if len(p) >= 4 and p[0].isdigit() and p[1].isdigit():
+                steps[int(p[0])] = int(p[1]) + 1
+    wall = mesh.sets["WALL"] - 1
+    sample = np.arange(0, prob.n, 353)
+    rows = {"step": [], "length": [], "radius": [], "vmax": [], "eps_l2": [], "x_sample": []}
+    for k in sorted(steps):
+        path = os.path.join(workdir, "MEGAOUT_%04d.vtk" % k)
+        if k == 0 or not os.path.exists(path):
+            continue
+        fr = read_unformatted_vtk(path)
+        x = fr["points"]
+        v = np.stack([fr["velocity_x"], fr["velocity_y"], fr["velocity_z"]], 1)
+        rows["step"].append(steps[k])
+        rows["length"].append(x[:, 2].max())
+        rows["radius"].append(np.sqrt(x[wall, 0] ** 2 + x[wall, 1] ** 2).max())
+        rows["vmax"].append(np.sqrt((v * v).sum(1)).max())
+        rows["eps_l2"].append(np.sqrt((prob.vol * fr["eps"] ** 2).sum()))
+        rows["x_sample"].append(x[sample])
+    ke = []   # ENERGY_INFO.OUT: STEPS, internal, kinetic, external, total energy (MAIN.F90:245,555)
+    with open(os.path.join(workdir, "ENERGY_INFO.OUT")) as f:
+        for ln in f:
+            p = ln.split()
+            if len(p) == 5 and p[0].isdigit():
+                ke.append((int(p[0]), float(p[2])))
+    with open(os.path.join(workdir, "TIME.OUT")) as f:   # one line per finished step (MAIN.F90:653)
+        steps_done = sum(1 for ln in f if ln.strip())
+    with open(os.path.join(workdir, "mega.out")) as f:
+        txt = f.read()
+    reason = ""   # EXIT_PROGRAM's reason line, if the program stopped itself (EXIT_PROGRAM.F90)
+    if "FATAL ERROR" in txt:
+        tail = [ln.strip() for ln in txt[txt.index("FATAL ERROR"):].splitlines()]
+        reason = " ".join(t for t in tail if t and not t.startswith("*") and "FATAL" not in t)[:200]
+    out = {k: np.array(v) for k, v in rows.items()}
+    out.update({"sample": sample, "ke_step": np.array([s for s, _ in ke], dtype=np.int64),
+                "ke": np.array([e for _, e in ke]), "n": np.array([prob.n]), "dt": np.array([prob.dt]),
+                "lagrangian": np.array([lagrangian]), "pdsearch": np.array([prob.pdsearch]),
+                "nsteps": np.array([nsteps]), "returncode": np.array([rc]),
+                "steps_done": np.array([steps_done]), "stop_reason": np.array(reason)})
+    return out
+
+
 def pd_main():
     ref = load_ref()
     rng = np.random.default_rng(20260203)
@@ -393,6 +471,20 @@ def main():
         return
     if "--pd" in sys.argv:
         pd_main()
+        return
+    if "--trajectory" in sys.argv:
+        # TRAJ_DIR keeps the working directory, so that a finished run is parsed again without rerunning
+        lag = 0 if "--semi" in sys.argv else 1
+        name = "traj_h019_semilag.npz" if lag == 0 else "traj_h019_lagrangian.npz"
+        kw = dict(nsteps=500, every=50) if lag == 0 else dict(nsteps=1400, every=100)
+        keep = os.environ.get("TRAJ_DIR")
+        if keep:
+            wd = os.path.join(keep, "semi" if lag == 0 else "lag")
+            os.makedirs(wd, exist_ok=True)
+            np.savez_compressed(os.path.join(HERE, name), **trajectory_vectors(wd, lag, **kw))
+        else:
+            with tempfile.TemporaryDirectory() as wd:
+                np.savez_compressed(os.path.join(HERE, name), **trajectory_vectors(wd, lag, **kw))
         return
     if "--damage" in sys.argv:
         damage_main()
