@@ -34,6 +34,9 @@ compared with FP64 and with the reference program's final state
 
 semi_lagrangian (unless --no-semi): the same bar in the semi-Lagrangian
 formulation, FP64 and mixed rates beside the headline (not part of value);
+pd_plate (unless --no-pd): correspondence peridynamics on the 6.53M-particle
+plate (100 x 100 x 5 mm, spacing 0.2 mm), mixed and FP64 rates and the pair
+kernels' roofline fractions;
 preprocessing: the one-time geometry preprocessing of the mesh on the GPU
 (mega_preprocess) beside its host restatement.
 
@@ -50,7 +53,7 @@ a 12-step and a 2-step run (setup cancels).  Falls back to the oracle
 restatement when the reference build is absent.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--prelude P] [--modes fp64,mixed,surrogate]
-                    [--stable-steps S] [--no-validate] [--no-cpu] [--no-semi]
+                    [--stable-steps S] [--no-validate] [--no-cpu] [--no-semi] [--no-pd] [--pd-only]
 """
 from __future__ import annotations
 
@@ -105,6 +108,18 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         return nnz * (4 + 3 * r) + n * (10 * r + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 + (24 if mixed else 0))
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
+    if name == "k_pd_gather":
+        # pairs: bond index + influence weight; nodes: own X0, x, d, volume, row start,
+        # the 18 gather sums written
+        return nnz * (4 + r) + n * (24 + 24 + 24 + 8 + 4 + 18 * r)
+    if name == "k_pd_node":
+        # sums read, K^-1 read, stress/strain r/w, eps r/w, volume, mat id, T record written
+        return n * (18 * r + 72 + 24 * r + 2 * r + 8 + 4 + 10 * r)
+    if name == "k_pd_force":
+        # pairs: source + influence weight (transposed); nodes: row start, own X0, own T,
+        # g vector, then the corrector's node arrays as k_force
+        return nnz * (4 + r) + n * (4 + 24 + 10 * r + 24 + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 +
+                                    (24 if mixed else 0))
     if name == "k_rk":
         # the semi-Lagrangian launch, gather sums fused: pairs: col + tpos read,
         # phi written in list order and {grad phi, source} in transposed order;
@@ -126,6 +141,37 @@ def semi_step_bytes(n: int, nnz: int, mixed: bool) -> float:
     index + phi (RK in registers, bins amortised over 20 steps)."""
     kbar = nnz / n
     return kbar * (20 if mixed else 28) + (400 if mixed else 500)
+
+
+def pd_plate(steps: int, warmup: int, modes=("mixed", "fp64")) -> dict:
+    """Correspondence peridynamics (CONSTRUCT_FINT_PD) on the plate:
+    100 x 100 x 5 mm at spacing 0.2 mm (jitter seed 3, 6.53M particles),
+    elastic aluminium, horizon ~3 spacings, clamped lateral faces, a 5 mm
+    plug at 300 m/s; mixed (FP32 bond forces, FP64 integration) and FP64
+    node-steps/s, per-kernel times and each pair kernel's fraction of the
+    HBM roofline."""
+    t = time.time()
+    prob = deck.pd_plate_problem(preprocess_fn=mega.geometry)
+    log(f"PD plate: {prob.n} particles, deck {time.time() - t:.1f} s")
+    out = {"workload": "PD plate 100 x 100 x 5 mm, h 0.2 mm, jitter 5% seed 3, elastic aluminium, horizon 3.0 h, "
+                       "clamped edges, 5 mm plug at 300 m/s, correspondence PD (FINT_PD)",
+           "particles": prob.n, "steps": steps}
+    for name in modes:
+        wall, kt_timed, kt, nnz, mx, _ = timed_run(prob, name, steps, warmup)
+        rate = prob.n * steps / wall
+        mixed = name != "fp64"
+        kern = {}
+        for k in ("k_pd_gather", "k_pd_node", "k_pd_force"):
+            if k in kt:
+                ms = kt_timed[k][0] if k in kt_timed else kt[k][0]
+                b = kernel_bytes(k, prob.n, nnz, mixed)
+                kern[k] = {"ms": ms, "algorithmic_bytes": b, "achieved_GB_s": b / (ms * 1e-3) / 1e9,
+                           "frac": b / (ms * 1e-3) / PEAK_HBM}
+        out[name] = {"rate": rate, "ms_per_step": 1e3 * wall / steps, "bonds": nnz, "bonds_per_particle": nnz / prob.n,
+                     "max_bonds": mx, "kernels": kern}
+        log(f"PD plate {name}: {1e3 * wall / steps:.3f} ms/step, {rate:.4e} node-steps/s, "
+            + ", ".join(f"{k} {v['ms']:.3f} ms ({v['frac']:.2f} of HBM)" for k, v in kern.items()))
+    return out
 
 
 def preprocessing() -> dict:
@@ -352,6 +398,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-semi", action="store_true",
                     help="skip the side measurements (semi-Lagrangian step, preprocessing)")
+    ap.add_argument("--no-pd", action="store_true", help="skip the peridynamic plate side measurement")
+    ap.add_argument("--pd-only", action="store_true",
+                    help="only the peridynamic plate side measurement (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -363,6 +412,9 @@ def main():
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     modes = [m for m in args.modes.split(",") if m]
+    if args.pd_only:
+        print(json.dumps({"pd_plate": pd_plate(min(args.steps, 20), 3)}))
+        return
 
     t = time.time()
     prob = deck.taylor_problem(H, v0=V0, seed=SEED)
@@ -406,6 +458,7 @@ def main():
     traffic = read_traffic(kname, mixed_prec)
 
     semi = None if args.no_semi else semi_lagrangian(min(args.steps, 50), 3)
+    pd = None if args.no_pd else pd_plate(min(args.steps, 20), 3)
     pre = None if args.no_semi else preprocessing()
     if pre:
         log("preprocessing:", pre)
@@ -459,6 +512,7 @@ def main():
         "impact_start_window": {"v0_m_s": V0_IMPACT, "steps": f"{args.warmup + 1}-{args.warmup + args.steps}",
                                 "modes": impact},
         "semi_lagrangian": semi,
+        "pd_plate": pd,
         "preprocessing": pre,
         "validation": validation,
         "cpu_baseline": cpu,

@@ -351,10 +351,52 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
     return prob
 
 
+def aluminium_elastic(mm_units: bool = False) -> Material:
+    """Aluminium, elastic (type 1): E = 70 GPa, nu = 0.25, rho = 2700 kg/m^3;
+    in mm / tonne / s / MPa with mm_units."""
+    if mm_units:
+        return Material.elastic(E=70e3, nu=0.25, rho=2.7e-9)
+    return Material.elastic(E=70e9, nu=0.25, rho=2700.0)
+
+
+def pd_plate_problem(h: float = 0.2e-3, lx: float = 100e-3, ly: float = 100e-3, lz: float = 5e-3, seed: int = 3,
+                     v0: float = 300.0, impact_radius: float = 5e-3, horizon: float = 3.015, dt_frac: float = 0.3,
+                     time_end: float = 20e-6, mm_units: bool = True, preprocess_fn=None) -> Problem:
+    """Correspondence-peridynamic plate (CONSTRUCT_FINT_PD, *PERIDYNAMICS 1):
+    a jittered cubic lattice [0,lx]x[0,ly]x[0,lz] (plate_mesh) of elastic
+    aluminium, the four lateral faces clamped (set EDGE, fixity 1 1 1: the
+    particles within half a spacing of them) and the particles within
+    impact_radius of the plate's vertical centre axis (set IMPACT) given
+    v_z = -v0 as *INITIAL CONDITION: the footprint of a striker, as the
+    reference's cards can express it (the reference has no contact).
+    Spherical support (*SUPPORT 1, SHSUP) of horizon ~ horizon * h: the
+    normalised window is horizon / 1.086, 1.086 h being the mean nodal window
+    of a lattice jittered by 5 %.  dt = dt_frac * h / c_p.  Arguments are SI;
+    mm_units (default) writes the model in mm / tonne / s / MPa, where the
+    shape tensor is O(1) (in SI at sub-mm spacing INVERSE takes it for
+    singular, INVERSE.F90:41-45).  preprocess_fn(mesh) -> Geometry replaces
+    the host preprocessing (e.g. taylor_amd.mega.geometry, on the device)."""
+    s = 1e3 if mm_units else 1.0
+    mesh = plate_mesh(h * s, lx * s, ly * s, lz * s, seed=seed)
+    x, y = mesh.coo[:, 0], mesh.coo[:, 1]
+    r = np.sqrt((x - 0.5 * lx * s) ** 2 + (y - 0.5 * ly * s) ** 2)
+    ids = np.arange(1, mesh.coo.shape[0] + 1, dtype=np.int32)
+    mesh.sets["IMPACT"] = ids[r <= impact_radius * s]
+    half = 0.5 * h * s
+    edge = (x < half) | (x > lx * s - half) | (y < half) | (y > ly * s - half)
+    mesh.sets["EDGE"] = ids[edge]
+    mat = aluminium_elastic(mm_units)
+    prob = assemble(mesh, {"EDGE": (1, 1, 1)}, {"IMPACT": (0.0, 0.0, -v0 * s)}, mat, norm_win=horizon / 1.086,
+                    dt=dt_frac * h * s / p_wave_speed(mat), time_end=time_end, lagrangian=1, pdsearch=1, ql=0,
+                    shsup=1, geometry=None if preprocess_fn is None else preprocess_fn(mesh))
+    prob.peridynamics = 1
+    return prob
+
+
 def assemble(mesh: Mesh, fixity: dict, velocity: dict, mat: Material, norm_win: float, dt: float,
              time_end: float, lagrangian: int = 1, pdsearch: int = 1, ql: int = 1, ql_coef: float = 0.001,
-             shsup: int = 0) -> Problem:
-    geo = preprocess(mesh)
+             shsup: int = 0, geometry: "Geometry | None" = None) -> Problem:
+    geo = geometry if geometry is not None else preprocess(mesh)
     n = mesh.coo.shape[0]
     # QL length from the unscaled window (PRE_CONTROL.F90:803-828), then scale
     ql_len = float(geo.win0[0, 0])

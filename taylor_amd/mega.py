@@ -324,6 +324,14 @@ def preprocess(coo, elcon, hex=None):
     return win, vol, sm, dmax
 
 
+def geometry(mesh):
+    """deck.Geometry of a deck.Mesh from the device preprocessing (the same
+    windows and volumes as deck.preprocess, bit for bit)."""
+    from taylor_amd import deck
+    win, vol, sm, dmax = preprocess(mesh.coo, mesh.elcon)
+    return deck.Geometry(win0=win, vol=vol, sm_len=sm, dist_max=dmax)
+
+
 class Mlp:
     """Trainer / batch inference of the 13-64-64-64-7 network on the GPU."""
 

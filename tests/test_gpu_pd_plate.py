@@ -1,0 +1,114 @@
+"""Correspondence peridynamics (CONSTRUCT_FINT_PD, FINT_PD.F90:16-759) on the
+plate workload: 100 x 100 x 5 mm at spacing 0.2 mm (jitter seed 3, 6.53M
+particles), elastic aluminium, horizon ~3.0 spacings (spherical support),
+lateral faces clamped, a 5 mm-radius plug under the plate's centre given
+300 m/s (deck.pd_plate_problem; model in mm / tonne / s / MPa).
+
+* A reduced plate (10 x 10 x 2 mm, 28.6k particles) against the oracle's
+  MAIN loop over 20 steps: bond lists equal in the reference's order, FP64
+  positions within 1e-12, velocity, force and stress within 1e-9; mixed
+  precision (FP32 bond forces, FP64 integration) within the stated bars.
+* The full plate, FP64: one CONSTRUCT_FINT_PD call under a uniform affine
+  deformation x = A X.  The deformation gradient S K^-1 reproduces A at
+  every particle, whatever its bond family, so the stress is one uniform
+  tensor to round-off; the bond forces sum to zero.
+* The full plate, 300 m/s impact: mixed precision against FP64 over 30
+  steps, the bond forces summing to zero in both.
+"""
+import numpy as np
+import pytest
+
+from oracle_bind import OracleRun
+from taylor_amd import deck, mega
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return mega.load_library()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reduced_plate_matches_oracle(lib, oracle, precision):
+    prob = deck.pd_plate_problem(lx=10e-3, ly=10e-3, lz=2e-3, impact_radius=2e-3)
+    nsteps = 20
+    g = mega.Mega(prob, precision=mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
+    g.step(nsteps)
+    run = OracleRun(oracle, prob, nthreads=8).step(nsteps)
+    A = run.arr
+    gn, _, st = g.neighbors()
+    rp, col, _, _ = run.csr()
+    assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
+    errs = {k: rel_err(g.download(k), A[r]) for k, r in
+            (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    fsum = float(np.abs(g.download("fint").sum(0)).max() / np.abs(g.download("fint")).sum(0).max())
+    g.close()
+    print(f"reduced plate {prob.n} particles, {int(rp[-1])} bonds, {precision}, {nsteps} steps vs oracle: {errs}; "
+          f"|sum fint| / sum |fint| {fsum:.1e}", flush=True)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert fsum < 1e-12
+    else:
+        # measured: x 3.6e-10, v 1.3e-7, fint 2.1e-7, stress 2.8e-7, force sum 2.2e-10
+        assert errs["x"] < 4e-9 and errs["v"] < 1.5e-6 and errs["fint"] < 2e-6 and errs["stress"] < 3e-6
+        assert fsum < 3e-9
+
+
+@pytest.fixture(scope="module")
+def plate():
+    prob = deck.pd_plate_problem(preprocess_fn=mega.geometry)
+    print(f"plate: {prob.n} particles, impact set {len(prob.sets['IMPACT'])}, clamped {len(prob.sets['EDGE'])}",
+          flush=True)
+    return prob
+
+
+def test_full_plate_affine_deformation_gives_uniform_stress(lib, plate):
+    prob = plate
+    A = np.array([[1.0004, 0.0002, -0.0001], [0.0001, 0.9997, 0.0003], [-0.0002, 0.0001, 1.0005]])
+    X = prob.coo
+    dtot = X @ (A - np.eye(3)).T
+    n = prob.n
+    g = mega.Mega(prob, precision=mega.MEGA_FP64)
+    g.search()
+    g.shapes()
+    nnz, _, _ = g.stats()
+    stress, strain, state = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 20))
+    fint, _, _, status = g.construct_fint(0, dtot, dtot, X + dtot, stress, strain, state)
+    g.close()
+    spread = float((stress - stress.mean(0)).__abs__().max() / np.abs(stress.mean(0)).max())
+    fsum = float(np.abs(fint.sum(0)).max() / np.abs(fint).sum(0).max())
+    print(f"plate affine: {nnz} bonds ({nnz / n:.1f} per particle); mean stress {stress.mean(0)} MPa, "
+          f"spread {spread:.1e}; |sum fint| / sum |fint| {fsum:.1e}", flush=True)
+    assert status == 0
+    assert np.isfinite(stress).all() and np.abs(stress.mean(0)).max() > 1.0
+    assert spread < 1e-9    # measured 9.2e-11
+    assert fsum < 5e-14     # measured 5.5e-15
+
+
+def test_full_plate_impact_mixed_vs_fp64(lib, plate):
+    prob = plate
+    nsteps = 30
+    out = {}
+    for name, prec in (("fp64", mega.MEGA_FP64), ("mixed", mega.MEGA_MIXED)):
+        g = mega.Mega(prob, precision=prec)
+        st = g.step(nsteps)
+        f = g.download("fint")
+        out[name] = {"x": g.download("x"), "v": g.download("v"), "fint": f, "stress": g.download("stress"),
+                     "fsum": float(np.abs(f.sum(0)).max() / np.abs(f).sum(0).max()), "status": st}
+        g.close()
+    errs = {k: rel_err(out["mixed"][k], out["fp64"][k]) for k in ("x", "v", "fint", "stress")}
+    print(f"plate impact, {nsteps} steps: mixed vs FP64 {errs}; |sum fint| / sum |fint| "
+          f"fp64 {out['fp64']['fsum']:.1e} mixed {out['mixed']['fsum']:.1e}", flush=True)
+    for r in out.values():
+        assert r["status"] == 0 and np.isfinite(r["x"]).all() and np.isfinite(r["fint"]).all()
+    # measured: force sums 8.2e-16 (FP64), 2.4e-10 (mixed); mixed vs FP64 x 7.4e-11, v 1.6e-7,
+    # fint 2.2e-7, stress 3.7e-7
+    assert out["fp64"]["fsum"] < 1e-14 and out["mixed"]["fsum"] < 3e-9
+    assert np.abs(out["fp64"]["v"][:, 2]).max() > 100e3   # the plug still moves (mm/s)
+    assert errs["x"] < 1e-9 and errs["v"] < 2e-6 and errs["fint"] < 2e-6 and errs["stress"] < 4e-6
