@@ -163,6 +163,7 @@ struct Params {
     double *x, *v, *a, *d, *dtot, *uphy, *fint, *fext, *fext_nmo, *wave_vel;
     double *Fst;                                     // deformation gradient per node (9, column-major, + 1 pad)
     float4 *d4, *t4;                                 // FP32 gather copies of d and dtot (mixed)
+    float4 *u4;                                      // peridynamics, mixed: FP32 x - X0 for the gather
     // state and scratch in R
     void *stress, *strain, *eps, *red, *P;
     void *st4;   // R[4n]: the damage models' EPS_SHEAR, EPS_VOL, D, DAM_SV (STATE(1),(2),(4),(5))
@@ -1475,6 +1476,17 @@ __global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
             const double om = px * py * pz, vj = p.vol[j];
             w[q] = (R)om;
             tw[p.tpos[q]] = (R)om;
+            if constexpr (std::is_same<R, float>::value) {
+                // mixed: the bond records of the gather and the force, w V (X_j - X_i) in list
+                // order (x, y, z blocks of dshp) and w V_i (X_i - X_j) with the source in the
+                // transposed slot
+                float *r = as<float>(p.dshp);
+                const double c = om * vj;
+                r[q] = (float)(c * h0); r[p.cap + q] = (float)(c * h1); r[2 * p.cap + q] = (float)(c * h2);
+                const double ci = -om * p.vol[node];
+                as<TPair<float>>(p.tpair)[p.tpos[q]] = TPair<float>{(float)(ci * h0), (float)(ci * h1),
+                                                                    (float)(ci * h2), (int32_t)node};
+            }
             K[0] += h0 * h0 * om * vj; K[1] += h0 * h1 * om * vj; K[2] += h0 * h2 * om * vj;
             K[3] += h1 * h1 * om * vj; K[4] += h1 * h2 * om * vj; K[5] += h2 * h2 * om * vj;
             g[0] -= om * vj * h0; g[1] -= om * vj * h1; g[2] -= om * vj * h2;
@@ -1496,6 +1508,15 @@ __global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
     }
 }
 
+// mixed peridynamics: FP32 u = x - X0 of every particle for the gather
+__global__ void k_pd_u4(int n, const double *x, const double *X0, float4 *u4)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u4[i] = make_float4((float)(x[3 * i] - X0[3 * i]), (float)(x[3 * i + 1] - X0[3 * i + 1]),
+                        (float)(x[3 * i + 2] - X0[3 * i + 2]), 0.f);
+}
+
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
 {
@@ -1506,7 +1527,35 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
     const int lane = threadIdx.x & (kGroup - 1);
     const bool live = node < p.n;
     double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (live) {
+    if constexpr (std::is_same<R, float>::value) {
+        // mixed: S - K = sum w V_j (u_j - u_i) (X_j - X_i)^T with u = x - X0 and
+        // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d and the
+        // bond records w V_j (X_j - X_i); k_pd_node adds K back as F = I + (S - K) K^-1
+        if (live) {
+            const float4 ui = p.u4[node], di = p.d4[node];
+            const float *rx = as<float>(p.dshp), *ry = rx + p.cap, *rz = ry + p.cap;
+            const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
+            for (int q = q0 + lane; q < q1; q += 2 * kGroup) {
+                const bool two = q + kGroup < q1;
+                const int j0 = p.col[q], j1 = two ? p.col[q + kGroup] : j0;
+                const float r0[3] = {rx[q], ry[q], rz[q]};
+                const float r1[3] = {two ? rx[q + kGroup] : 0.f, two ? ry[q + kGroup] : 0.f,
+                                     two ? rz[q + kGroup] : 0.f};
+                const float4 u0 = p.u4[j0], d0 = p.d4[j0], u1 = p.u4[j1], d1 = p.d4[j1];
+                const float a0[3] = {u0.x - ui.x, u0.y - ui.y, u0.z - ui.z};
+                const float b0[3] = {d0.x - di.x, d0.y - di.y, d0.z - di.z};
+                const float a1[3] = {u1.x - ui.x, u1.y - ui.y, u1.z - ui.z};
+                const float b1[3] = {d1.x - di.x, d1.y - di.y, d1.z - di.z};
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        S[a + 3 * b] += (double)a0[a] * r0[b] + (double)a1[a] * r1[b];
+                        L[a + 3 * b] += (double)b0[a] * r0[b] + (double)b1[a] * r1[b];
+                    }
+            }
+        }
+    } else if (live) {
         const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
         const double x[3] = {p.x[3 * node], p.x[3 * node + 1], p.x[3 * node + 2]};
         const double d[3] = {p.d[3 * node], p.d[3 * node + 1], p.d[3 * node + 2]};
@@ -1548,6 +1597,9 @@ __global__ __launch_bounds__(kBlock) void k_pd_node(Params p)
     double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, IF[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, det = 1.0;
     if (!p.pd_linit && p.finite_strain) {
         mega::mm3(S, Ki, F);
+        if constexpr (std::is_same<R, float>::value) {   // mixed: the gather gave S - K
+            F[0] += 1.0; F[4] += 1.0; F[8] += 1.0;
+        }
         det = mega::det3_abs(F);
         if (!mega::inverse_gj3(F, IF)) atomicAdd(&p.flags[2], 1);
     }
@@ -1619,7 +1671,26 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
     const bool live = node < p.n;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const R *Pb = as<R>(p.P);
-    if (live) {
+    if constexpr (std::is_same<R, float>::value) {
+        // mixed: one 16 B record per transposed slot holds w V_k (X_k - X_i) and the source k
+        if (live) {
+            const TPair<float> *tp = as<const TPair<float>>(p.tpair);
+            const int t0 = p.trow[node], t1 = p.trow[node + 1];
+            for (int t = t0 + lane; t < t1; t += 2 * kGroup) {
+                const bool two = t + kGroup < t1;
+                const TPair<float> e0 = tp[t], e1 = two ? tp[t + kGroup] : TPair<float>{0.f, 0.f, 0.f, e0.src};
+                R T0[10], T1[10];
+                load_P<R>(Pb, e0.src, T0);
+                load_P<R>(Pb, e1.src, T1);
+                a0 += (double)(T0[0] * e0.gx + T0[3] * e0.gy + T0[6] * e0.gz) +
+                      (double)(T1[0] * e1.gx + T1[3] * e1.gy + T1[6] * e1.gz);
+                a1 += (double)(T0[1] * e0.gx + T0[4] * e0.gy + T0[7] * e0.gz) +
+                      (double)(T1[1] * e1.gx + T1[4] * e1.gy + T1[7] * e1.gz);
+                a2 += (double)(T0[2] * e0.gx + T0[5] * e0.gy + T0[8] * e0.gz) +
+                      (double)(T1[2] * e1.gx + T1[5] * e1.gy + T1[8] * e1.gz);
+            }
+        }
+    } else if (live) {
         const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
         const R *tw = as<R>(p.tshp);
         for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
@@ -2114,6 +2185,7 @@ struct mega_ctx {
     double *winv = nullptr;   // 1 / win, for the box-support kernel
     double *wave_vel = nullptr, *scratch = nullptr;   // scratch: 6n doubles
     float4 *d4 = nullptr, *t4 = nullptr;             // FP32 gather copies of d, dtot (mixed)
+    float4 *u4 = nullptr;                            // FP32 x - X0 (peridynamics, mixed)
     double *Fst = nullptr;                           // deformation gradient per node
     int F_valid = 0;                                 // Fst consistent with dtot
     int32_t *ebc = nullptr, *ebc_node = nullptr, *mat = nullptr;
@@ -2217,7 +2289,7 @@ Params params(mega_ctx *c)
     p.X0 = c->X0; p.win = c->win; p.winv = c->winv; p.vol = c->vol; p.mass = c->mass;
     p.x = c->x; p.v = c->v; p.a = c->a; p.d = c->d; p.dtot = c->dtot; p.uphy = c->uphy;
     p.fint = c->fint; p.fext = c->fext; p.fext_nmo = c->fext_nmo; p.wave_vel = c->wave_vel;
-    p.d4 = c->d4; p.t4 = c->t4; p.Fst = c->Fst;
+    p.d4 = c->d4; p.t4 = c->t4; p.u4 = c->u4; p.Fst = c->Fst;
     p.stress = c->stress; p.strain = c->strain; p.eps = c->eps; p.red = c->red; p.P = c->P; p.st4 = c->st4;
     p.ebc = c->ebc; p.ebc_node = c->ebc_node; p.mat = c->mat; p.mats = c->mats;
     p.wpart = c->wpart; p.epart = c->epart; p.flags = c->flags;
@@ -2616,6 +2688,7 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     const unsigned g = xcd_grid(c->n, &nb);
     {
         Scope s(c, "k_pd_gather");
+        if (c->mixed) k_pd_u4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->x, c->X0, c->u4);
         k_pd_gather<R><<<g, kBlock, 0, c->stream>>>(p, nb);
     }
     {
@@ -2888,6 +2961,7 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     e |= dalloc(&c->fext, n3); e |= dalloc(&c->scratch, n6);
     if (c->has_gravity) e |= dalloc(&c->fext_nmo, n3);
     if (c->mixed) { e |= dalloc(&c->d4, n); e |= dalloc(&c->t4, n); }
+    if (c->mixed && c->pd) e |= dalloc(&c->u4, n);
     else e |= dalloc(&c->wave_vel, n);
     e |= dalloc(&c->Fst, 10 * (size_t)n);
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
@@ -2954,7 +3028,7 @@ int mega_destroy(mega_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_pairs(c);
     void *ptrs[] = {c->X0, c->x, c->v, c->a, c->d, c->dtot, c->uphy, c->fint, c->fext, c->fext_nmo, c->win, c->winv,
-                    c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->Fst, c->ebc, c->ebc_node, c->mat,
+                    c->vol, c->mass, c->wave_vel, c->scratch, c->d4, c->t4, c->u4, c->Fst, c->ebc, c->ebc_node, c->mat,
                     c->mats, c->stress, c->strain, c->eps, c->red, c->P, c->row_ptr, c->trow, c->counts,
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
