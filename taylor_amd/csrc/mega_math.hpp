@@ -8,6 +8,10 @@
 //   singular-kernel epsilon is the REAL(4) 1.0E-015  RK.f90:219
 //   J2 Newton tolerance uses the REAL(4) 1.0E-05     CONSTITUTE_VONMISES.F90:109
 // Small matrices are column-major: a[i + 3*j] = A(i+1, j+1).
+// The constitutive per-point functions keep the reference's unfused
+// arithmetic (no contraction of products into fused multiply-adds), so that
+// in FP64 they give the reference's bits; the kernel and the moment matrix are
+// sums over neighbors, whose order already differs.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -65,6 +69,7 @@ MEGA_HD void cubic_kernel(T x, T &phi, T &dphi)
 template <class T>
 MEGA_HD void inv3_ref(const T *a, T *r)
 {
+#pragma clang fp contract(off)
 #define A_(i, j) a[((i)-1) + 3 * ((j)-1)]
 #define R_(i, j) r[((i)-1) + 3 * ((j)-1)]
     T det = A_(1,1)*A_(2,2)*A_(3,3) - A_(1,1)*A_(2,3)*A_(3,2) - A_(1,2)*A_(2,1)*A_(3,3)
@@ -86,6 +91,7 @@ MEGA_HD void inv3_ref(const T *a, T *r)
 template <class T>
 MEGA_HD T det3_abs(const T *a)
 {
+#pragma clang fp contract(off)
 #define A_(i, j) a[((i)-1) + 3 * ((j)-1)]
     T d = A_(1,1)*(A_(2,2)*A_(3,3) - A_(3,2)*A_(2,3))
         + A_(1,2)*(A_(3,1)*A_(2,3) - A_(2,1)*A_(3,3))
@@ -97,6 +103,7 @@ MEGA_HD T det3_abs(const T *a)
 template <class T>
 MEGA_HD void mm3(const T *a, const T *b, T *c)
 {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -110,6 +117,7 @@ MEGA_HD void mm3(const T *a, const T *b, T *c)
 // Returns 1 on success, 0 on that early stop.
 MEGA_HD int inverse_gj3(const double *A, double *Ai)
 {
+#pragma clang fp contract(off)
 #define G_(i, j) a[(i) + 3 * (j)]
     double a[9];
 #pragma unroll
@@ -182,6 +190,7 @@ MEGA_HD void m44inv_sym(const double *m, double *r)
 // shears, d = Voigt(sym G).
 MEGA_HD void hughes_winget(const double *L, double *R, double *de, double *d)
 {
+#pragma clang fp contract(off)
     double A[9], Ai[9], G[9], W[9], IW[9], IWi[9], T[9];
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -212,6 +221,7 @@ MEGA_HD void hughes_winget(const double *L, double *R, double *de, double *d)
 // ROTATE_TENSOR: v <- R T(v) R^T (ROTATE_TENSOR.F90:21-87), Voigt XX,YY,ZZ,YZ,XZ,XY
 MEGA_HD void rotate_voigt(const double *R, double *v)
 {
+#pragma clang fp contract(off)
     double T[9] = {v[0], v[5], v[4], v[5], v[1], v[3], v[4], v[3], v[2]};
     double RT[9], RTRt[9], Rt[9];
 #pragma unroll
@@ -228,6 +238,7 @@ MEGA_HD void rotate_voigt(const double *R, double *v)
 template <class T>
 MEGA_HD T hard_pow(T base, T e)
 {
+#pragma clang fp contract(off)
     if (e == T(1)) return base;
     if (e == T(0)) return T(1);
     return pow(base, e);
@@ -239,6 +250,7 @@ MEGA_HD T hard_pow(T base, T e)
 template <class T>
 MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
 {
+#pragma clang fp contract(off)
     const T mu = T(m.mu), sig0 = T(m.sig0), B = T(m.B), CE = T(m.CE);
     const T kk = pred[0] + pred[1] + pred[2];
     const T p3 = kk / T(3);
@@ -269,9 +281,8 @@ MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
             }
             eps = epst + r23 * dg;
         }
-        const T s = dg * T(2) * mu / tn;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) dev[k] = dev[k] - s * dev[k];
+        for (int k = 0; k < 6; ++k) dev[k] = dev[k] - dg * T(2) * mu * dev[k] / tn;   // the reference's order
     }
     stress[0] = p3 + dev[0];
     stress[1] = p3 + dev[1];
@@ -288,6 +299,7 @@ MEGA_HD int von_mises(T *stress, const T *pred, T &eps, const Material &m)
 template <class T>
 MEGA_HD void effective_trial(const T *stress, T dam, const T *inc, const Material &m, T *eff)
 {
+#pragma clang fp contract(off)
     const T lam = T(m.lam), mu = T(m.mu), l2m = T(m.l2m);
 #pragma unroll
     for (int k = 0; k < 6; ++k) eff[k] = stress[k] / (T(1) - dam);
@@ -305,6 +317,7 @@ MEGA_HD void effective_trial(const T *stress, T dam, const T *inc, const Materia
 template <class T>
 MEGA_HD void damage_update(const T *total, T &dam, T &dam_sv, const Material &m)
 {
+#pragma clang fp contract(off)
     const T k_i = T(m.k_i), k_c = T(m.k_c);
     const T eta = sqrt(total[0] * total[0] + total[1] * total[1] + total[2] * total[2] +
                        T(2) * (total[3] * total[3] + total[4] * total[4] + total[5] * total[5]));
@@ -324,6 +337,7 @@ MEGA_HD void damage_update(const T *total, T &dam, T &dam_sv, const Material &m)
 template <class T>
 MEGA_HD int von_mises_dam(T *stress, const T *total, const T *inc, T &eps, T &dam, T &dam_sv, const Material &m)
 {
+#pragma clang fp contract(off)
     const T mu = T(m.shear), sig0 = T(m.sig0), B = T(m.B), CE = T(m.CE);
     T eff[6];
     effective_trial(stress, dam, inc, m, eff);
@@ -376,6 +390,7 @@ template <class T>
 MEGA_HD int druck_prag(T *stress, const T *total, const T *inc, T &eps_shear, T &eps, T &dam, T &dam_sv,
                        const Material &m)
 {
+#pragma clang fp contract(off)
     const T q_phi = T(m.q_phi), k_phi = T(m.k_phi), q_psi = T(m.q_psi);
     T eff[6];
     effective_trial(stress, dam, inc, m, eff);
@@ -412,6 +427,7 @@ MEGA_HD int druck_prag(T *stress, const T *total, const T *inc, T &eps_shear, T 
 template <class T>
 MEGA_HD T trial_excess(const T *pred, T eps, const Material &m)
 {
+#pragma clang fp contract(off)
     const T kk = pred[0] + pred[1] + pred[2];
     const T p3 = kk / T(3);
     const T d0 = pred[0] - p3, d1 = pred[1] - p3, d2 = pred[2] - p3;
@@ -423,6 +439,7 @@ MEGA_HD T trial_excess(const T *pred, T eps, const Material &m)
 template <class T>
 MEGA_HD void elastic_trial(const T *stress, const T *de, const Material &m, T *pred)
 {
+#pragma clang fp contract(off)
     const T lam = T(m.lam), mu = T(m.mu), l2m = T(m.l2m);
     pred[0] = stress[0] + (l2m * de[0] + lam * de[1] + lam * de[2]);
     pred[1] = stress[1] + (lam * de[0] + l2m * de[1] + lam * de[2]);
@@ -435,11 +452,13 @@ MEGA_HD void elastic_trial(const T *stress, const T *de, const Material &m, T *p
 template <class T>
 MEGA_HD T tnorm(const T *v)
 {
+#pragma clang fp contract(off)
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + T(2) * (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
 }
 template <class T>
 MEGA_HD T tnorm_strain(const T *v)
 {
+#pragma clang fp contract(off)
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
 }
 
@@ -474,6 +493,7 @@ MEGA_HD T return_residual(const T *pred, const T *s, T eps_n, T eps, const Mater
 // GMAX_WVEL from secant moduli (FINT.F90:850-870, ESTIMATE_MODULI.F90:16-86)
 MEGA_HD double wave_speed(const double *si, const double *ei, const Material &m)
 {
+#pragma clang fp contract(off)
     double shear = m.shear, bulk = m.bulk;
     if (m.type > 1) {
         const double sk = (si[0] + si[1] + si[2]) / 3.0, ek = (ei[0] + ei[1] + ei[2]) / 3.0;

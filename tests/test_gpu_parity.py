@@ -18,6 +18,55 @@ def rel_err(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
+# Mixed precision against the FP64 oracle: bars about ten times the errors
+# measured for each case (relative to each field's largest value), so that an
+# order-of-magnitude regression of the FP32 path fails.
+MIXED_BARS = {
+    # measured: x 1.1e-09, v 1.1e-07, fint 9.3e-08, stress 2.5e-07
+    "ellipsoidal L1": {"x": 2e-08, "v": 2e-06, "fint": 1e-06, "stress": 3e-06},
+    # measured: x 7.6e-10, v 9.6e-08, fint 9.5e-08, stress 2.3e-07
+    "ellipsoidal L0": {"x": 1e-08, "v": 1e-06, "fint": 1e-06, "stress": 3e-06},
+    # measured: x 1.4e-09, v 1.3e-07, fint 1.7e-07, stress 1.8e-07
+    "no_ql L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "stress": 2e-06},
+    # measured: x 1.1e-09, v 1.5e-07, fint 1.7e-07, stress 2.4e-07
+    "elastic L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "stress": 3e-06},
+    # measured: x 2.8e-09, v 1.6e-07, fint 2.8e-07, stress 4.1e-07
+    "tiny bar L1": {"x": 3e-08, "v": 2e-06, "fint": 3e-06, "stress": 5e-06},
+    # measured: x 2.3e-09, v 6.9e-08, fint 2.5e-07, stress 3.5e-07
+    "tiny bar L0": {"x": 3e-08, "v": 1e-06, "fint": 3e-06, "stress": 5e-06},
+    # measured: shp 3.1e-08, dshp 4.3e-08
+    "semi shapes": {"shp": 5e-07, "dshp": 5e-07},
+    # measured: x 8.5e-10
+    "semi shapes, 3 more steps": {"x": 1e-08},
+    # measured: x 2.7e-10, uphy 3.2e-08, v 8.6e-08, fint 2.3e-07, stress 3.0e-07
+    "wide support 2.2": {"x": 3e-09, "uphy": 5e-07, "v": 1e-06, "fint": 3e-06, "stress": 5e-06},
+    # measured: x 3.9e-10, uphy 4.7e-08, v 4.1e-08, fint 1.7e-07, stress 2.4e-07
+    "wide support 4.4": {"x": 5e-09, "uphy": 5e-07, "v": 5e-07, "fint": 2e-06, "stress": 3e-06},
+    # measured: x 1.1e-09, v 1.3e-07, fint 1.3e-07, fext 1.0e-07, stress 2.8e-07
+    "gravity L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "fext": 2e-06, "stress": 3e-06},
+    # measured: x 1.1e-09, v 1.7e-07, fint 1.3e-07, stress 3.0e-07
+    "two materials L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "stress": 5e-06},
+    # measured: x 1.1e-09, v 1.1e-07, fint 1.3e-07, stress 2.8e-07, energy 2.2e-09
+    "strain0 L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "stress": 3e-06, "energy": 3e-08},
+    # measured: x 1.1e-09, v 1.3e-07, fint 1.3e-07, stress 2.8e-07, energy 1.3e-09
+    "strain1 L1": {"x": 2e-08, "v": 2e-06, "fint": 2e-06, "stress": 3e-06, "energy": 2e-08},
+    # measured: fint 1.6e-07, stress 1.2e-07, eps 4.3e-08
+    "one CONSTRUCT_FINT": {"fint": 2e-06, "stress": 2e-06, "eps": 5e-07},
+    # measured: x 1.1e-09, uphy 6.7e-08, v 1.3e-07, stress 2.6e-07
+    "12 steps L1": {"x": 2e-08, "uphy": 1e-06, "v": 2e-06, "stress": 3e-06},
+    # measured: x 6.2e-10, uphy 3.7e-08, v 7.4e-08, stress 1.7e-07
+    "12 steps L0": {"x": 1e-08, "uphy": 5e-07, "v": 1e-06, "stress": 2e-06},
+}
+
+
+def check_mixed(case, errs):
+    print("mixed errors", case, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bars = MIXED_BARS[case]
+    for k, v in errs.items():
+        if k in bars:
+            assert v < bars[k], (case, k, v, bars[k])
+
+
 @pytest.fixture(scope="module")
 def mega():
     from taylor_amd import mega as m
@@ -43,9 +92,63 @@ def test_constitute_matches_oracle(mega, oracle):
             d = np.zeros(6)
             oracle.or_constitute_point(np.ascontiguousarray(L[i]), s_ref[i], e_ref[i], st_ref[i], mat.props, 2, 1, d)
         mega.constitute(L, stress, strain, state, mat.props, 2, 1)
+        print("constitute: values differing from the oracle's bits:", int((stress != s_ref).sum()),
+              int((strain != e_ref).sum()), int((state[:, 2] != st_ref[:, 2]).sum()), "of", stress.size, flush=True)
+        if mat.props[5] == 1.0:
+            # linear hardening: the reference's own arithmetic, bit for bit (with a power law the
+            # device's pow and the host C library's may differ in the last bit)
+            assert np.array_equal(stress, s_ref) and np.array_equal(strain, e_ref)
+            assert np.array_equal(state[:, 2], st_ref[:, 2])
         assert rel_err(stress, s_ref) < 1e-12
         assert rel_err(strain, e_ref) < 1e-12
         assert rel_err(state[:, 2], st_ref[:, 2]) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_return_map_failure_is_reported(mega, oracle, precision):
+    """VON_MISES stops the program when its Newton loop has not met the
+    absolute tolerance sqrt(2/3) sigma_0 1e-5 in 20 iterations
+    (CONSTITUTE_VONMISES.F90:105-119).  A trial stress of ~1e12 sigma_0 (the
+    state of a run that has lost stability) leaves a residual at the rounding
+    level of its norm, above that tolerance, for about a fifth of such points
+    in FP64; with a zero velocity-gradient increment the trial stress is the
+    given stress.  Which points fail depends on the last bits: the oracle
+    (bit-exact to VON_MISES) and the device (which keeps its unfused
+    arithmetic) fail on the same points, the device reports
+    MEGA_ERR_RETURN_MAP, and the points that converge agree bit for bit.
+    FP32 (mixed) resolves a deviatoric norm above ~8 sigma_0 more coarsely
+    than that tolerance, so at 1e8 sigma_0 (below the overflow of its
+    squares) its return map fails where FP64's converges; the device reports
+    it (which points fail is not observable there: every FP32 result differs
+    from its FP64 input)."""
+    rng = np.random.default_rng(11)
+    n = 256
+    mat = deck.copper_linear()
+    L = np.zeros((n, 9))
+    amp = 1e12 if precision == "fp64" else 1e8
+    stress = rng.normal(size=(n, 6)) * mat.props[3] * amp
+    stress0 = stress.copy()
+    strain = np.zeros((n, 6))
+    state = np.zeros((n, 20))
+    s_ref, e_ref, st_ref = stress.copy(), strain.copy(), state.copy()
+    ok = np.zeros(n, bool)
+    for i in range(n):
+        d = np.zeros(6)
+        ok[i] = oracle.or_constitute_point(np.ascontiguousarray(L[i]), s_ref[i], e_ref[i], st_ref[i], mat.props,
+                                           2, 1, d) >= 0
+    st = mega.constitute(L, stress, strain, state, mat.props, 2, 1,
+                         mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
+    moved = (stress != stress0).any(axis=1)   # FP64: a failed point keeps its trial stress
+    print(f"{precision}, trial ~{amp:.0e} sigma_0: oracle fails {int((~ok).sum())} of {n}, device returns status {st} "
+          f"and leaves {int((~moved).sum())} points at their trial stress", flush=True)
+    assert st == 1 and "converge" in mega._status_string(st)
+    if precision == "fp64":
+        # the reference's unfused arithmetic: the same points fail, the others agree bit for bit
+        assert 0 < (~ok).sum() < n
+        assert np.array_equal(moved, ok)
+        assert np.array_equal(stress[ok], s_ref[ok])
+    else:
+        assert ok.all()
 
 
 @pytest.mark.parametrize("norm_win,shsup", [(1.84, 0), (5.0, 0), (5.0, 1)])
@@ -178,7 +281,7 @@ def test_steps_with_ellipsoidal_support(mega, oracle, lagrangian, precision):
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
         assert rel_err(g.download("eps"), A["state"][:, 2]) < 1e-8
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        check_mixed(f"ellipsoidal L{lagrangian}", errs)
     g.close()
 
 
@@ -209,7 +312,7 @@ def test_steps_without_ql_and_with_elastic_material(mega, oracle, case, lagrangi
     if precision == "fp64":
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        check_mixed(f"{case} L{lagrangian}", errs)
     g.close()
 
 
@@ -256,7 +359,7 @@ def test_steps_on_a_tiny_bar(mega, oracle, lagrangian, precision):
     if precision == "fp64":
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        check_mixed(f"tiny bar L{lagrangian}", errs)
     g.close()
 
 
@@ -276,13 +379,17 @@ def test_semi_lagrangian_shapes_after_steps(mega, oracle, precision):
     rp, col, shp_r, dshp_r = run.csr()
     gn, _, st = g.neighbors()
     assert np.array_equal(gn, np.diff(rp)) and np.array_equal(st, col + 1)
-    tol = 1e-10 if precision == "fp64" else 1e-5
     print(precision, rel_err(shp, shp_r), rel_err(dshp, dshp_r))
-    assert rel_err(shp, shp_r) < tol and rel_err(dshp, dshp_r) < tol
+    if precision == "fp64":
+        assert rel_err(shp, shp_r) < 1e-10 and rel_err(dshp, dshp_r) < 1e-10
+    else:
+        check_mixed("semi shapes", {"shp": rel_err(shp, shp_r), "dshp": rel_err(dshp, dshp_r)})
     g.step(3)
     run.step(3)
-    bar = 1e-12 if precision == "fp64" else 1e-7
-    assert rel_err(g.download("x"), run.arr["x"]) < bar
+    if precision == "fp64":
+        assert rel_err(g.download("x"), run.arr["x"]) < 1e-12
+    else:
+        check_mixed("semi shapes, 3 more steps", {"x": rel_err(g.download("x"), run.arr["x"])})
     g.close()
 
 
@@ -314,7 +421,7 @@ def test_steps_with_wide_supports(mega, oracle, norm_win, precision):
         assert errs["x"] < 1e-12 and errs["uphy"] < 1e-10
         assert errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
     else:
-        assert errs["x"] < 1e-7 and errs["uphy"] < 1e-4 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        check_mixed(f"wide support {norm_win}", errs)
     g.close()
 
 
@@ -341,7 +448,7 @@ def test_steps_with_gravity(mega, oracle, lagrangian, precision):
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["fext"] < 1e-9
         assert errs["stress"] < 1e-9
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["fext"] < 1e-5 and errs["stress"] < 1e-3
+        check_mixed(f"gravity L{lagrangian}", errs)
     g.close()
 
 
@@ -372,7 +479,7 @@ def test_steps_with_two_materials_and_damping(mega, oracle, lagrangian, precisio
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
         assert rel_err(eps, A["state"][:, 2]) < 1e-8
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+        check_mixed(f"two materials L{lagrangian}", errs)
     g.close()
 
 
@@ -407,15 +514,13 @@ def test_steps_small_strain_and_energies(mega, oracle, finite, lagrangian, preci
         assert rel_err(g.download("eps"), A["state"][:, 2]) < 1e-8
         assert e_err.max() < 1e-12
     else:
-        assert errs["x"] < 1e-7 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
-        assert e_err.max() < 1e-6
+        check_mixed(f"strain{finite} L{lagrangian}", {**errs, "energy": float(e_err.max())})
     g.close()
 
 
 def test_construct_fint_mixed_precision(mega, oracle):
     """Mixed mode (FP32 shapes, stress, return map, force; FP64 F, F^-1 and
-    rotation) against the FP64 oracle on the same inputs: force within 1e-4
-    normwise, stress within 1e-5."""
+    rotation) against the FP64 oracle on the same inputs (bars in MIXED_BARS)."""
     prob = small_problem(lagrangian=1)
     n = prob.n
     rng = np.random.default_rng(5)
@@ -438,17 +543,15 @@ def test_construct_fint_mixed_precision(mega, oracle):
     f_r, _, _, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, dinc, dtot, prob.coo + dtot, s_r, e_r, st_r)
     s_g, e_g, st_g = stress.copy(), strain.copy(), state.copy()
     f_g, _, _, _ = g.construct_fint(0, dinc, dtot, prob.coo + dtot, s_g, e_g, st_g)
-    assert rel_err(f_g, f_r) < 1e-4
-    assert rel_err(s_g, s_r) < 1e-5
-    assert rel_err(st_g[:, 2], st_r[:, 2]) < 1e-4
+    check_mixed("one CONSTRUCT_FINT", {"fint": rel_err(f_g, f_r), "stress": rel_err(s_g, s_r),
+                                       "eps": rel_err(st_g[:, 2], st_r[:, 2])})
 
 
 @pytest.mark.parametrize("lagrangian", [1, 0])
 def test_explicit_steps_mixed_precision(mega, oracle, lagrangian):
     """Mixed mode over 12 steps against the FP64 oracle.  FP32 force and
-    stress noise (~1e-6 relative per step) enters the velocities and, through
-    them, the positions; bounds: positions 1e-7, displacement 1e-4, velocity
-    and stress 1e-3 (relative to each field's max)."""
+    stress rounding (~1e-7 relative) enters the velocities and, through them,
+    the positions (bars in MIXED_BARS, relative to each field's max)."""
     prob = small_problem(lagrangian=lagrangian)
     nsteps = 12
     g = mega.Mega(prob, precision=mega.MEGA_MIXED)
@@ -456,8 +559,4 @@ def test_explicit_steps_mixed_precision(mega, oracle, lagrangian):
     run = OracleRun(oracle, prob).step(nsteps)
     A = run.arr
     errs = {k: rel_err(g.download(k), A[r]) for k, r in (("x", "x"), ("uphy", "dphy_tot"), ("v", "v"), ("stress", "stress"))}
-    print("mixed step errors", lagrangian, errs)
-    assert errs["x"] < 1e-7
-    assert errs["uphy"] < 1e-4
-    assert errs["v"] < 1e-3
-    assert errs["stress"] < 1e-3
+    check_mixed(f"12 steps L{lagrangian}", errs)
