@@ -351,6 +351,14 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
     return prob
 
 
+def aluminium_powerlaw() -> Material:
+    """The reference's shipped deck material (Control.dat *MATERIALS): J2 with
+    power-law hardening sigma_0 (1 + B eps)^CE, E = 78.2 GPa, nu = 0.3,
+    rho = 2700 kg/m^3, sigma_0 = 290 MPa, B = 125, CE = 0.1; VON_MISES then
+    iterates Newton (CONSTITUTE_VONMISES.F90:94-119)."""
+    return Material.j2(E=78.2e9, nu=0.3, rho=2700.0, sig0=0.29e9, B=125.0, CE=0.1)
+
+
 def aluminium_elastic(mm_units: bool = False) -> Material:
     """Aluminium, elastic (type 1): E = 70 GPa, nu = 0.25, rho = 2700 kg/m^3;
     in mm / tonne / s / MPa with mm_units."""

@@ -7,7 +7,13 @@ in for VON_MISES (CONSTITUTE_VONMISES.F90:21-158); the step falls back to the
 exact return map outside the training envelope and wherever the network's
 answer misses the yield surface by more than tol * sigma_y.
 
-    python -m taylor_amd.surrogate --out taylor_amd/data/j2_copper.npz
+    python -m taylor_amd.surrogate                       # copper, linear hardening
+    python -m taylor_amd.surrogate --material powerlaw   # the reference deck's aluminium, power law
+
+Training runs: the 181k-node bar (spacing 0.19 mm, jitter seed 1) in the
+semi-Lagrangian formulation with bins rebuilt every 20 steps, at 150, 190 and
+250 m/s, every 4th step of the first 2,000 (or until the run leaves the range
+in which the formulation stays finite).
 """
 from __future__ import annotations
 
@@ -26,20 +32,25 @@ from taylor_amd import deck, mega  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "data", "j2_copper.npz")
+POWERLAW_WEIGHTS = os.path.join(HERE, "data", "j2_powerlaw.npz")
+MATERIALS = {"copper": deck.copper_linear, "powerlaw": deck.aluminium_powerlaw}
 
 
-def harvest(prob, steps: int, every: int, cap: int, precision: int = mega.MEGA_FP64, log=None):
+def harvest(prob, steps: int, every: int, cap: int, precision: int = mega.MEGA_FP64, log=None, chunk: int = 100):
     """Run `steps` explicit steps and return the plastic return-map tuples of
     every `every`-th step: float32 rows (de[6], rotated stress[6], eqps_n,
-    stress[6], eqps)."""
+    stress[6], eqps).  The formulation loses stability at fine spacing: a
+    chunk in which a return map fails, the state stops being finite or a node
+    moves faster than 5 times the impact speed is dropped and the run ends."""
     g = mega.Mega(prob, precision=precision)
     g.harvest(cap, every=every)
+    v0 = float(np.abs(prob.vinit).max())
     out, done = [], 0
     while done < steps:
-        k = min(250, steps - done)
+        k = min(chunk, steps - done)
         try:
-            g.step(k)
-        except mega.MegaError as e:   # a return map that no longer converges
+            st = g.step(k)
+        except mega.MegaError as e:   # e.g. nodes piling into the edge bins
             if log:
                 log(f"    stopped at step {done + k}: {e}")
             break
@@ -47,10 +58,12 @@ def harvest(prob, steps: int, every: int, cap: int, precision: int = mega.MEGA_F
         tup, count = g.harvest_take(cap)
         if count > cap:
             raise RuntimeError(f"harvest buffer overflow: {count} tuples > {cap}")
-        if not (np.isfinite(g.download("x")).all() and np.isfinite(tup).all()):
+        v = g.download("v")
+        vmax = float(np.sqrt((v * v).sum(1)).max())
+        if st != 0 or not (np.isfinite(g.download("x")).all() and np.isfinite(tup).all()) or not vmax <= 5.0 * v0:
             # the formulation's own instability has set in: keep what came before
             if log:
-                log(f"    run left the finite range by step {done}; chunk dropped")
+                log(f"    run left the finite range by step {done} (status {st}, max speed {vmax:.0f}); chunk dropped")
             break
         out.append(tup)
         if log and (done % 1000 == 0 or done == steps):
@@ -155,16 +168,19 @@ def attach(g: "mega.Mega", weights=None, tol: float = 1e-3):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--material", choices=sorted(MATERIALS), default="copper")
+    ap.add_argument("--out", default=None)
     ap.add_argument("--velocities", type=float, nargs="+", default=[150.0, 190.0, 250.0])
     ap.add_argument("--h", type=float, default=0.19e-3, help="lattice spacing of the harvest runs")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--lagrangian", type=int, default=0, help="1: Lagrangian harvest runs (default semi-Lagrangian)")
+    ap.add_argument("--pdsearch", type=int, default=20, help="bin rebuild period of the semi-Lagrangian runs")
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=4)
-    ap.add_argument("--h2", type=float, default=0.5e-3, help="spacing of the full-length (80 us) harvest runs")
+    ap.add_argument("--h2", type=float, default=0.0, help="spacing of optional full-length (80 us) runs (0: none)")
     ap.add_argument("--every2", type=int, default=2)
     ap.add_argument("--cap", type=int, default=20_000_000)
-    ap.add_argument("--iters", type=int, default=40000)
+    ap.add_argument("--iters", type=int, default=300000)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr0", type=float, default=2e-3)
     ap.add_argument("--lr1", type=float, default=1e-5)
@@ -173,11 +189,15 @@ def main():
     def log(*a):
         print(*a, flush=True)
 
-    mat = deck.copper_linear()
+    out_path = args.out or (DEFAULT_WEIGHTS if args.material == "copper" else POWERLAW_WEIGHTS)
+    mat = MATERIALS[args.material]()
     sets = []
+    runs = [(args.h, args.seed, args.steps, args.every, args.lagrangian)]
+    if args.h2 > 0:
+        runs.append((args.h2, 0, None, args.every2, 1))
     for v in args.velocities:
-        for h, seed, steps, every in ((args.h, args.seed, args.steps, args.every), (args.h2, 0, None, args.every2)):
-            prob = deck.taylor_problem(h, v0=v, seed=seed, material=mat)
+        for h, seed, steps, every, lag in runs:
+            prob = deck.taylor_problem(h, v0=v, seed=seed, material=mat, lagrangian=lag, pdsearch_steps=args.pdsearch)
             n_steps = steps if steps else int(np.floor(80e-6 / prob.dt)) + 1
             log(f"harvest: h {h * 1e3:.3f} mm, {prob.n} nodes, {v} m/s, {n_steps} steps, every {every}")
             t0 = time.perf_counter()
@@ -189,7 +209,8 @@ def main():
     hist = res["loss"]
     log(f"trained {args.iters} iterations in {res['train_seconds']:.1f} s: loss {hist[:100].mean():.4e} -> "
         f"{hist[-500:].mean():.4e}")
-    meta = {"velocities": args.velocities, "h": args.h, "seed": args.seed, "steps": args.steps, "every": args.every,
+    meta = {"material": args.material, "lagrangian": args.lagrangian, "pdsearch": args.pdsearch,
+            "velocities": args.velocities, "h": args.h, "seed": args.seed, "steps": args.steps, "every": args.every,
             "h2": args.h2, "every2": args.every2, "tuples": int(tup.shape[0]), "iters": args.iters,
             "batch": args.batch, "lr0": args.lr0, "lr1": args.lr1, "final_loss": float(hist[-500:].mean()),
             "train_seconds": res["train_seconds"]}
@@ -198,13 +219,14 @@ def main():
     sample = tup[rng.choice(tup.shape[0], size=min(tup.shape[0], 2_000_000), replace=False)]
     meta["train_eval"] = evaluate(w, sample)
     log("training-set check:", meta["train_eval"])
-    prob = deck.taylor_problem(args.h, v0=220.0, seed=args.seed, material=mat)
+    prob = deck.taylor_problem(args.h, v0=220.0, seed=args.seed, material=mat, lagrangian=args.lagrangian,
+                               pdsearch_steps=args.pdsearch)
     held = harvest(prob, min(args.steps, 600), 5, args.cap)
     meta["heldout_220"] = evaluate(w, held)
     log("held-out 220 m/s check:", meta["heldout_220"])
-    save(args.out, res["params"], res["norm"], mat.props, meta)
-    np.save(os.path.splitext(args.out)[0] + "_loss.npy", hist[:: max(1, len(hist) // 2000)])
-    log(f"saved {args.out}")
+    save(out_path, res["params"], res["norm"], mat.props, meta)
+    np.save(os.path.splitext(out_path)[0] + "_loss.npy", hist[:: max(1, len(hist) // 2000)])
+    log(f"saved {out_path}")
 
 
 if __name__ == "__main__":
