@@ -127,6 +127,16 @@ static_assert(kFGroup <= 16 && kGroup <= 16 && kGGroupSets <= 16, "groups inside
 // 40 B (mixed) plus the force kernel's own 1 KB let 7 of its blocks (its
 // register limit) share a CU's 160 KB of LDS; larger sets read global memory.
 constexpr int kSetCap = MEGA_SET_CAP;
+// peridynamics (mixed): 32 particles per block, 8 lanes each, and sets of up
+// to kPdSetCap sources (a 32-particle brick names ~600-900 within a horizon of
+// three spacings); the force stages 40 B T rows, the gather u and d (32 B)
+#ifndef MEGA_PD_SET_CAP
+#define MEGA_PD_SET_CAP 1024
+#endif
+constexpr int kPdSetCap = MEGA_PD_SET_CAP;
+constexpr int kPdG = 8;
+constexpr int kPdNpb = kBlock / kPdG;
+constexpr int kPdUnroll = 4;
 constexpr int kSetSort = 4096;   // list entries of one block the set builder sorts in LDS
 static_assert(kSetCap <= 65535, "set positions are 16-bit");
 
@@ -1521,38 +1531,80 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
 {
     constexpr int W = std::is_same<R, float>::value ? 21 : 18;
+    constexpr bool MIX = std::is_same<R, float>::value;
+    constexpr int G = MIX ? kPdG : kGroup;   // lanes per particle
+    extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
-    const int lane = threadIdx.x & (kGroup - 1);
+    const int node = blk * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x & (G - 1);
     const bool live = node < p.n;
     double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if constexpr (std::is_same<R, float>::value) {
+    if constexpr (MIX) {
         // mixed: S - K = sum w V_j (u_j - u_i) (X_j - X_i)^T with u = x - X0 and
         // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d and the
-        // bond records w V_j (X_j - X_i); k_pd_node adds K back as F = I + (S - K) K^-1
+        // packed bond records {w V_j (X_j - X_i), j's position in the block's source
+        // set, or j}; k_pd_node adds K back as F = I + (S - K) K^-1.  A block with a
+        // source set stages u and d of its sources in LDS.
+        float4 *Lu = reinterpret_cast<float4 *>(mega_dyn);
+        int m = 0;
+        if (p.gset_ptr) {
+            const int s0 = p.gset_ptr[blk];
+            m = p.gset_ptr[blk + 1] - s0;
+            for (int k = threadIdx.x; k < m; k += kBlock) {
+                const int j = p.gset[s0 + k];
+                Lu[k] = p.u4[j];
+                Lu[m + k] = p.d4[j];
+            }
+            if (m > 0) __syncthreads();
+        }
         if (live) {
             const float4 ui = p.u4[node], di = p.d4[node];
-            const float *rx = as<float>(p.dshp), *ry = rx + p.cap, *rz = ry + p.cap;
             const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
-            for (int q = q0 + lane; q < q1; q += 2 * kGroup) {
-                const bool two = q + kGroup < q1;
-                const int j0 = p.col[q], j1 = two ? p.col[q + kGroup] : j0;
-                const float r0[3] = {rx[q], ry[q], rz[q]};
-                const float r1[3] = {two ? rx[q + kGroup] : 0.f, two ? ry[q + kGroup] : 0.f,
-                                     two ? rz[q + kGroup] : 0.f};
-                const float4 u0 = p.u4[j0], d0 = p.d4[j0], u1 = p.u4[j1], d1 = p.d4[j1];
-                const float a0[3] = {u0.x - ui.x, u0.y - ui.y, u0.z - ui.z};
-                const float b0[3] = {d0.x - di.x, d0.y - di.y, d0.z - di.z};
-                const float a1[3] = {u1.x - ui.x, u1.y - ui.y, u1.z - ui.z};
-                const float b1[3] = {d1.x - di.x, d1.y - di.y, d1.z - di.z};
+            if (p.fpair) {
+                for (int base = q0 + lane; base < q1; base += G * kPdUnroll) {
+                    float4 rec[kPdUnroll], uj[kPdUnroll], dj[kPdUnroll];
 #pragma unroll
-                for (int b = 0; b < 3; ++b)
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        S[a + 3 * b] += (double)a0[a] * r0[b] + (double)a1[a] * r1[b];
-                        L[a + 3 * b] += (double)b0[a] * r0[b] + (double)b1[a] * r1[b];
+                    for (int e = 0; e < kPdUnroll; ++e) {
+                        const int q = base + G * e;
+                        rec[e] = q < q1 ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(m > 0 ? 0 : node));
                     }
+#pragma unroll
+                    for (int e = 0; e < kPdUnroll; ++e) {
+                        const int j = __float_as_int(rec[e].w);
+                        uj[e] = m > 0 ? Lu[j] : p.u4[j];
+                        dj[e] = m > 0 ? Lu[m + j] : p.d4[j];
+                    }
+#pragma unroll
+                    for (int e = 0; e < kPdUnroll; ++e) {
+                        const float r[3] = {rec[e].x, rec[e].y, rec[e].z};
+                        const float a[3] = {uj[e].x - ui.x, uj[e].y - ui.y, uj[e].z - ui.z};
+                        const float b[3] = {dj[e].x - di.x, dj[e].y - di.y, dj[e].z - di.z};
+#pragma unroll
+                        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                            for (int aa = 0; aa < 3; ++aa) {
+                                S[aa + 3 * bb] += (double)a[aa] * r[bb];
+                                L[aa + 3 * bb] += (double)b[aa] * r[bb];
+                            }
+                    }
+                }
+            } else {   // before the records are packed (first call): SoA records and indices
+                const float *rx = as<float>(p.dshp), *ry = rx + p.cap, *rz = ry + p.cap;
+                for (int q = q0 + lane; q < q1; q += G) {
+                    const int j = p.col[q];
+                    const float r[3] = {rx[q], ry[q], rz[q]};
+                    const float4 u = p.u4[j], d = p.d4[j];
+                    const float a[3] = {u.x - ui.x, u.y - ui.y, u.z - ui.z};
+                    const float b[3] = {d.x - di.x, d.y - di.y, d.z - di.z};
+#pragma unroll
+                    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                        for (int aa = 0; aa < 3; ++aa) {
+                            S[aa + 3 * bb] += (double)a[aa] * r[bb];
+                            L[aa + 3 * bb] += (double)b[aa] * r[bb];
+                        }
+                }
             }
         }
     } else if (live) {
@@ -1576,7 +1628,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
         }
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) { S[k] = group_sum<kGroup>(S[k]); L[k] = group_sum<kGroup>(L[k]); }
+    for (int k = 0; k < 9; ++k) { S[k] = group_sum<G>(S[k]); L[k] = group_sum<G>(L[k]); }
     if (live && lane == 0) {
         R *red = as<R>(p.red) + (int64_t)W * node;
 #pragma unroll
@@ -1664,30 +1716,59 @@ template <class R, bool CORRECT, bool PREDICT>
 __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
 {
     __shared__ double s_red[kBlock / 64];
+    constexpr bool MIX = std::is_same<R, float>::value;
+    constexpr int G = MIX ? kPdG : kGroup;   // lanes per particle
+    extern __shared__ __align__(16) unsigned char mega_dyn[];
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int node = blk * (kBlock / kGroup) + threadIdx.x / kGroup;
-    const int lane = threadIdx.x & (kGroup - 1);
+    const int node = blk * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x & (G - 1);
     const bool live = node < p.n;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const R *Pb = as<R>(p.P);
-    if constexpr (std::is_same<R, float>::value) {
-        // mixed: one 16 B record per transposed slot holds w V_k (X_k - X_i) and the source k
+    if constexpr (MIX) {
+        // mixed: one 16 B record per transposed slot holds w V_k (X_k - X_i) and the
+        // source k, or k's position in the block's source set, whose T rows (the 9
+        // used values, 40 B) a block with a set stages in LDS
+        float2 *Lt = reinterpret_cast<float2 *>(mega_dyn);
+        int m = 0;
+        if (p.fset_ptr) {
+            const int s0 = p.fset_ptr[blk];
+            m = p.fset_ptr[blk + 1] - s0;
+            for (int r = threadIdx.x; r < m; r += kBlock) {
+                const float4 *q = reinterpret_cast<const float4 *>(Pb) + 3 * (int64_t)p.fset[s0 + r];
+                const float4 a = q[0], b = q[1], c = q[2];
+                Lt[5 * r] = make_float2(a.x, a.y); Lt[5 * r + 1] = make_float2(a.z, a.w);
+                Lt[5 * r + 2] = make_float2(b.x, b.y); Lt[5 * r + 3] = make_float2(b.z, b.w);
+                Lt[5 * r + 4] = make_float2(c.x, c.y);
+            }
+            if (m > 0) __syncthreads();
+        }
         if (live) {
             const TPair<float> *tp = as<const TPair<float>>(p.tpair);
             const int t0 = p.trow[node], t1 = p.trow[node + 1];
-            for (int t = t0 + lane; t < t1; t += 2 * kGroup) {
-                const bool two = t + kGroup < t1;
-                const TPair<float> e0 = tp[t], e1 = two ? tp[t + kGroup] : TPair<float>{0.f, 0.f, 0.f, e0.src};
-                R T0[10], T1[10];
-                load_P<R>(Pb, e0.src, T0);
-                load_P<R>(Pb, e1.src, T1);
-                a0 += (double)(T0[0] * e0.gx + T0[3] * e0.gy + T0[6] * e0.gz) +
-                      (double)(T1[0] * e1.gx + T1[3] * e1.gy + T1[6] * e1.gz);
-                a1 += (double)(T0[1] * e0.gx + T0[4] * e0.gy + T0[7] * e0.gz) +
-                      (double)(T1[1] * e1.gx + T1[4] * e1.gy + T1[7] * e1.gz);
-                a2 += (double)(T0[2] * e0.gx + T0[5] * e0.gy + T0[8] * e0.gz) +
-                      (double)(T1[2] * e1.gx + T1[5] * e1.gy + T1[8] * e1.gz);
+            for (int base = t0 + lane; base < t1; base += G * kPdUnroll) {
+                TPair<float> e[kPdUnroll];
+#pragma unroll
+                for (int u = 0; u < kPdUnroll; ++u) {
+                    const int t = base + G * u;
+                    e[u] = t < t1 ? tp[t] : TPair<float>{0.f, 0.f, 0.f, 0};
+                }
+#pragma unroll
+                for (int u = 0; u < kPdUnroll; ++u) {
+                    float T[10];
+                    if (m > 0) {
+                        const float2 *r = Lt + 5 * e[u].src;
+                        const float2 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3], v4 = r[4];
+                        T[0] = v0.x; T[1] = v0.y; T[2] = v1.x; T[3] = v1.y; T[4] = v2.x; T[5] = v2.y;
+                        T[6] = v3.x; T[7] = v3.y; T[8] = v4.x;
+                    } else {
+                        load_P<float>(Pb, e[u].src, T);
+                    }
+                    a0 += (double)(T[0] * e[u].gx + T[3] * e[u].gy + T[6] * e[u].gz);
+                    a1 += (double)(T[1] * e[u].gx + T[4] * e[u].gy + T[7] * e[u].gz);
+                    a2 += (double)(T[2] * e[u].gx + T[5] * e[u].gy + T[8] * e[u].gz);
+                }
             }
         }
     } else if (live) {
@@ -1704,7 +1785,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
             a2 += c * ((double)T[2] * dx[0] + (double)T[5] * dx[1] + (double)T[8] * dx[2]);
         }
     }
-    a0 = group_sum<kGroup>(a0); a1 = group_sum<kGroup>(a1); a2 = group_sum<kGroup>(a2);
+    a0 = group_sum<G>(a0); a1 = group_sum<G>(a1); a2 = group_sum<G>(a2);
     double ke = 0.0, ie = 0.0, ee = 0.0;
     if (live && lane < 3) {
         R T[10];
@@ -2088,6 +2169,20 @@ __global__ void k_fpair(int64_t nnz, int64_t cap, const float *shp, const float 
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nnz) return;
     out[q] = make_float4(dshp[q], dshp[cap + q], dshp[2 * cap + q], shp[q]);
+}
+
+// peridynamics (mixed): the gather's bond record {w V_j (X_j - X_i), j's position
+// in the block's source set (or j, for a block without a set)}, kPdG lanes per particle
+__global__ void k_pd_fpair(int n, const int32_t *row_ptr, const int32_t *col, int64_t cap, const float *rec,
+                           const uint16_t *loc, const int32_t *gset_ptr, float4 *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kPdG), lane = (int)(t % kPdG);
+    if (i >= n) return;
+    const int b = i / kPdNpb;
+    const bool set = gset_ptr[b + 1] > gset_ptr[b];
+    for (int q = row_ptr[i] + lane; q < row_ptr[i + 1]; q += kPdG)
+        out[q] = make_float4(rec[q], rec[cap + q], rec[2 * cap + q], __int_as_float(set ? (int)loc[q] : col[q]));
 }
 
 __global__ void k_fpair2(int64_t nnz, int64_t cap, const double *shp, const double *dshp, double2 *out)
@@ -2481,12 +2576,12 @@ int search_step(mega_ctx *c, const double *x, int first)
 // one list's sets: rowp/vals the CSR, loc (forward) or the records (transposed)
 template <class R>
 int build_set(mega_ctx *c, int npb, const int32_t *rowp, const int32_t *vals, int32_t *setp, int32_t **set,
-              int64_t *set_cap, uint16_t *loc, TPair<R> *tp, int *maxm)
+              int64_t *set_cap, uint16_t *loc, TPair<R> *tp, int *maxm, int cap = kSetCap)
 {
     const int nb = (c->n + npb - 1) / npb;
     HIPCHK(hipMemsetAsync(c->set_max, 0, sizeof(int32_t), c->stream));
     HIPCHK(hipMemsetAsync(c->counts + nb, 0, sizeof(int32_t), c->stream));
-    k_blk_set<false, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, kSetCap, c->counts, c->set_max,
+    k_blk_set<false, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, cap, c->counts, c->set_max,
                                                       nullptr, nullptr, nullptr, nullptr);
     int r = cub_scan(c, c->counts, setp, nb + 1);
     if (r) return r;
@@ -2501,7 +2596,7 @@ int build_set(mega_ctx *c, int npb, const int32_t *rowp, const int32_t *vals, in
         if (dalloc(set, tot)) return MEGA_ERR_NOMEM;
         *set_cap = tot;
     }
-    k_blk_set<true, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, kSetCap, nullptr, nullptr, setp, *set,
+    k_blk_set<true, R><<<nb, kBlock, 0, c->stream>>>(c->n, npb, rowp, vals, cap, nullptr, nullptr, setp, *set,
                                                      loc, tp);
     HIPCHK(hipGetLastError());
     *maxm = mx;
@@ -2570,6 +2665,48 @@ int build_sets(mega_ctx *c)
         }
     }
     if (r) return r;
+    c->sets_on = 1;
+    return MEGA_OK;
+}
+
+// source sets of the peridynamic bond lists (mixed; the lists stay for the
+// run): the force's transposed records get their source's set position (as
+// build_sets does for the RK force), the gather's records are packed with
+// theirs.  Built after k_pd_shape has written the records.
+int build_pd_sets(mega_ctx *c)
+{
+    c->sets_on = 0;
+    if (!c->pd || !c->mixed || !c->tsrc || c->nnz == 0) return MEGA_OK;
+    const int nb = (c->n + kPdNpb - 1) / kPdNpb;
+    int e = 0;
+    if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nb + 1);
+    if (!c->set_max) e |= dalloc(&c->set_max, 1);
+    if (!c->gset_ptr) e |= dalloc(&c->gset_ptr, nb + 1);
+    if (c->gloc_cap < c->nnz) {
+        (void)hipFree(c->gloc);
+        c->gloc = nullptr;
+        c->gloc_cap = 0;
+        e |= dalloc(&c->gloc, c->nnz_cap);
+        if (!e) c->gloc_cap = c->nnz_cap;
+    }
+    if (c->fpair_cap < c->nnz) {
+        (void)hipFree(c->fpair);
+        c->fpair = nullptr;
+        c->fpair_cap = 0;
+        e |= dalloc(&c->fpair, c->nnz_cap);
+        if (!e) c->fpair_cap = c->nnz_cap;
+    }
+    if (e) return MEGA_ERR_NOMEM;
+    Scope s(c, "k_blk_set");
+    int r = build_set<float>(c, kPdNpb, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
+                             static_cast<TPair<float> *>(c->tpair), &c->fset_max, kPdSetCap);
+    if (!r)
+        r = build_set<float>(c, kPdNpb, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc, nullptr,
+                             &c->gset_max, kPdSetCap);
+    if (r) return r;
+    k_pd_fpair<<<grid_for((int64_t)c->n * kPdG), kBlock, 0, c->stream>>>(
+        c->n, c->row_ptr, c->col, c->nnz_cap, (const float *)c->dshp, c->gloc, c->gset_ptr, c->fpair);
+    HIPCHK(hipGetLastError());
     c->sets_on = 1;
     return MEGA_OK;
 }
@@ -2675,6 +2812,8 @@ int do_pd_shapes(mega_ctx *c)
     if (c->mixed) k_pd_shape<float><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p);
     else k_pd_shape<double><<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p);
     HIPCHK(hipGetLastError());
+    const int r = build_pd_sets(c);
+    if (r) return r;
     c->pd_shapes = 1;
     return MEGA_OK;
 }
@@ -2682,14 +2821,18 @@ int do_pd_shapes(mega_ctx *c)
 template <class R>
 int pd_kernels(mega_ctx *c, int linit, int force_mode)
 {
+    constexpr bool MIX = std::is_same<R, float>::value;
     Params p = params(c);
     p.pd_linit = linit;
     int nb = 0;
-    const unsigned g = xcd_grid(c->n, &nb);
+    const unsigned g = xcd_grid(c->n, &nb, MIX ? kPdNpb : kNodesPerBlock);
+    // mixed with source sets: u and d (32 B) per gather source, T rows (40 B) per force source
+    const size_t lds_g = (MIX && p.gset_ptr) ? (size_t)c->gset_max * 32 : 0;
+    const size_t lds_f = (MIX && p.fset_ptr) ? (size_t)c->fset_max * 40 : 0;
     {
         Scope s(c, "k_pd_gather");
         if (c->mixed) k_pd_u4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->x, c->X0, c->u4);
-        k_pd_gather<R><<<g, kBlock, 0, c->stream>>>(p, nb);
+        k_pd_gather<R><<<g, kBlock, lds_g, c->stream>>>(p, nb);
     }
     {
         Scope s(c, "k_pd_node");
@@ -2697,9 +2840,9 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     }
     if (force_mode >= 0) {
         Scope s(c, "k_pd_force");
-        if (force_mode == 0) k_pd_force<R, false, false><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else if (force_mode == 1) k_pd_force<R, true, false><<<g, kBlock, 0, c->stream>>>(p, nb);
-        else k_pd_force<R, true, true><<<g, kBlock, 0, c->stream>>>(p, nb);
+        if (force_mode == 0) k_pd_force<R, false, false><<<g, kBlock, lds_f, c->stream>>>(p, nb);
+        else if (force_mode == 1) k_pd_force<R, true, false><<<g, kBlock, lds_f, c->stream>>>(p, nb);
+        else k_pd_force<R, true, true><<<g, kBlock, lds_f, c->stream>>>(p, nb);
     }
     HIPCHK(hipGetLastError());
     return MEGA_OK;
