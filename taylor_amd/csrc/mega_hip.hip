@@ -133,10 +133,16 @@ constexpr int kSetCap = MEGA_SET_CAP;
 #ifndef MEGA_PD_SET_CAP
 #define MEGA_PD_SET_CAP 1024
 #endif
+#ifndef MEGA_PD_GROUP
+#define MEGA_PD_GROUP 8
+#endif
+#ifndef MEGA_PD_UNROLL
+#define MEGA_PD_UNROLL 4
+#endif
 constexpr int kPdSetCap = MEGA_PD_SET_CAP;
-constexpr int kPdG = 8;
-constexpr int kPdNpb = kBlock / kPdG;
-constexpr int kPdUnroll = 4;
+constexpr int kPdG = MEGA_PD_GROUP;       // lanes per particle
+constexpr int kPdNpb = kBlock / kPdG;     // particles per block (and per source set)
+constexpr int kPdUnroll = MEGA_PD_UNROLL; // bonds per lane in flight
 constexpr int kSetSort = 4096;   // list entries of one block the set builder sorts in LDS
 static_assert(kSetCap <= 65535, "set positions are 16-bit");
 
