@@ -108,6 +108,16 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         return nnz * (4 + 3 * r) + n * (10 * r + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 + (24 if mixed else 0))
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
+    if name == "k_search_count":
+        # one wave per bin tests the 27 bins' candidates; per node: own x and window
+        # (a candidate's are re-reads), its slot in the bin list, the count, the kept
+        # ballot masks (8 x 8 B) and the 27 per-source-bin counts of its transposed row
+        return n * (24 + 24 + 4 + 4 + 64 + 27 * 4)
+    if name == "k_search_fill":
+        # the same walk over the kept masks (tests again beyond them): per node the masks,
+        # row start, the 27 transposed group cursors read and written; per pair the
+        # neighbor index and its transposed slot written
+        return n * (64 + 4 + 24 + 24 + 2 * 27 * 4) + nnz * (4 + 4)
     if name == "k_pd_gather":
         # pairs: bond index + influence weight; nodes: own X0, x, d, volume, row start,
         # the 18 gather sums written
@@ -399,6 +409,8 @@ def main():
     ap.add_argument("--no-semi", action="store_true",
                     help="skip the side measurements (semi-Lagrangian step, preprocessing)")
     ap.add_argument("--no-pd", action="store_true", help="skip the peridynamic plate side measurement")
+    ap.add_argument("--no-impact", action="store_true",
+                    help="skip the impact-start window beside the headline (profiling runs)")
     ap.add_argument("--pd-only", action="store_true",
                     help="only the peridynamic plate side measurement (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -436,7 +448,7 @@ def main():
     impact = {}
     v_unit = prob.vinit / V0
     prob.vinit = v_unit * V0_IMPACT
-    for name in modes:
+    for name in ([] if args.no_impact else modes):
         wall, _, _, _, _, st = timed_run(prob, name, args.steps, args.warmup, 0)
         impact[name] = {"rate": n * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
                         "surrogate_stats": st}

@@ -3054,6 +3054,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (ctrl->surrogate && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     // the reference's CONSTRUCT_FINT_PD is Lagrangian only (FINT_PD.F90:408-412)
     if (ctrl->peridynamics && (!ctrl->lagrangian || ctrl->surrogate)) return MEGA_ERR_ARG;
+    // the search's per-source-bin counters (27 per node) are indexed in int32
+    if ((int64_t)n * 27 > INT32_MAX) return MEGA_ERR_BINS;
     *out = nullptr;
     mega_ctx *c = new mega_ctx();
     c->c = *ctrl;
