@@ -167,6 +167,18 @@ int mega_preprocess_noalloc(int32_t nump, const double *coo, int32_t numel, cons
  * EBC and corrector.  MAIN.F90:304-658. */
 int mega_step(mega_ctx *ctx, double dt, int32_t nsteps);
 
+/* nsteps explicit steps with the automatic time step (IAUTO_TS = 1,
+ * peridynamics only; PRE_CONTROL.F90:522-524, MAIN.F90:348-350,428): each
+ * step's DLT is dlt_fac (DLT_FAC) times the smallest, over all particles, of
+ * the shortest bond at the current coordinates over the particle's wave speed
+ * (FINT_PD.F90:690-754), formed after the internal force and used by that
+ * step's corrector and the next step's predictor.  The first step of a
+ * context has no predictor.  dlt (host, nsteps entries, may be NULL) receives
+ * each step's DLT, so that the caller keeps TIME = TIME + DLT (MAIN.F90:641).
+ * MEGA_ERR_ARG for a context without peridynamics: the RKPM CONSTRUCT_FINT
+ * leaves DLT_FINT unset (FINT.F90:1250-1326 are commented out). */
+int mega_step_auto(mega_ctx *ctx, double dlt_fac, int32_t nsteps, double *dlt);
+
 /* fields for mega_download / mega_upload */
 enum {
     MEGA_F_X = 0,        /* current coordinates 3n (LOCAL_COO_CURRENT)     */

@@ -992,6 +992,35 @@ int or_construct_fint_pd(or_fint_io *io, const or_control *c)
         free(ycur); free(lst);
     }
     io->ret_fail = fails;
+    if (io->auto_ts) {
+        /* FINT_PD.F90:690-754: GCHAR_DIST(I) the shortest DSQRT((XJ-XI)**2 +
+         * (YJ-YI)**2 + (ZJ-ZI)**2) over the neighbors JJ /= I at the current
+         * coordinates (left as it was when there is none); DLT_FINT the
+         * smallest GCHAR_DIST(I) / GMAX_WVEL(I), in particle order, times DLT_FAC */
+        double dlt = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const int64_t p0 = io->row_ptr[i];
+            const int ln = (int)(io->row_ptr[i + 1] - p0);
+            const double xi = io->coo_cur[3 * i], yi = io->coo_cur[3 * i + 1], zi = io->coo_cur[3 * i + 2];
+            int first = 1;
+            for (int J = 0; J < ln; ++J) {
+                const int jj = io->col[p0 + J];
+                if (jj == i) continue;
+                const double xj = io->coo_cur[3 * jj], yj = io->coo_cur[3 * jj + 1], zj = io->coo_cur[3 * jj + 2];
+                const double dist = sqrt((xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi));
+                if (first) {
+                    io->char_dist[i] = dist;
+                    first = 0;
+                } else {
+                    io->char_dist[i] = fmin(io->char_dist[i], dist);
+                }
+            }
+            const double t = io->char_dist[i] / io->wave_vel[i];
+            if (i == 0) dlt = t;
+            else if (t < dlt) dlt = t;
+        }
+        io->dlt_fint = dlt * io->dlt_fac;
+    }
     return bad_type ? -2 : (fails ? -1 : 0);
 }
 
@@ -1031,8 +1060,10 @@ static int model_fint(or_model *m, int linit)
     io.stress = m->stress; io.strain = m->strain; io.state = m->state;
     io.fint = m->fint; io.fext = m->fext; io.wave_vel = m->wave_vel; io.strain_eq = m->strain_eq;
     io.int_work = m->int_work; io.linit = linit; io.nthreads = m->nthreads; io.invk = m->invk;
+    io.auto_ts = m->auto_ts && m->c.peridynamics; io.dlt_fac = m->dlt_fac; io.char_dist = m->char_dist;
     int r = m->c.peridynamics ? or_construct_fint_pd(&io, &m->c) : or_construct_fint(&io, &m->c);
     m->int_work = io.int_work;
+    if (io.auto_ts && !linit) m->dt = io.dlt_fint;   /* IF (AUTO_TS) DLT = LOCAL_DLT (MAIN.F90:428) */
     m->ret_fail += io.ret_fail;
     return r;
 }
@@ -1045,6 +1076,8 @@ int or_model_init(or_model *m)
     m->col = NULL; m->shp = NULL; m->dshp = NULL; m->nnz = 0; m->nnz_cap = 0;
     m->invk = m->c.peridynamics ? (double *)calloc(9 * (size_t)n, sizeof(double)) : NULL;
     if (m->c.peridynamics && !m->invk) return -1;
+    m->char_dist = m->auto_ts ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
+    if (m->auto_ts && (!m->c.peridynamics || !m->char_dist || !m->wave_vel)) return -1;
     m->step = 0; m->time = 0.0; m->int_work = 0.0; m->ke = m->ie = m->ee = 0.0;
     m->search_count = 0; m->ret_fail = 0;
     if (!m->row_ptr || !m->ijk) return -1;
@@ -1062,7 +1095,6 @@ int or_model_init(or_model *m)
 int or_model_step(or_model *m)
 {
     const int n = m->n;
-    const double dt = m->dt;
     const int first = (m->step == 0);
     double *fext_nmo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
     if (!fext_nmo) return -1;
@@ -1073,11 +1105,17 @@ int or_model_step(or_model *m)
         r = model_fint(m, 1);
         if (r == -2) { free(fext_nmo); return r; }
     }
-    /* predictor (MAIN.F90:353-355), fixed time step */
-    for (int64_t q = 0; q < 3 * (int64_t)n; ++q) {
-        m->d[q] = dt * m->v[q] + dt * dt * 0.5 * m->a[q];
-        m->dtot[q] = m->dtot[q] + m->d[q];
-        m->v[q] = m->v[q] + dt * 0.5 * m->a[q];
+    /* predictor (MAIN.F90:348-355) with the step size of the last step; with
+     * the automatic time step the first step has none (zero increment) */
+    if (first && m->auto_ts) {
+        for (int64_t q = 0; q < 3 * (int64_t)n; ++q) m->d[q] = 0.0;
+    } else {
+        const double dt = m->dt;
+        for (int64_t q = 0; q < 3 * (int64_t)n; ++q) {
+            m->d[q] = dt * m->v[q] + dt * dt * 0.5 * m->a[q];
+            m->dtot[q] = m->dtot[q] + m->d[q];
+            m->v[q] = m->v[q] + dt * 0.5 * m->a[q];
+        }
     }
     /* DO_INTERP (HANDELER.F90:114-133) and coordinate update (MAIN.F90:377-386) */
     if (m->c.peridynamics) {   /* HANDELER.F90:134-137: physical = generalized */
@@ -1100,6 +1138,7 @@ int or_model_step(or_model *m)
         if (model_search(m, first)) { free(fext_nmo); return -1; }
     r = model_fint(m, 0);
     if (r == -2) { free(fext_nmo); return r; }
+    const double dt = m->dt;   /* this step's DLT when it is automatic */
     /* EBC, damping, corrector (MAIN.F90:432-488) */
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
@@ -1132,6 +1171,7 @@ int or_model_step(or_model *m)
 
 void or_model_free(or_model *m)
 {
-    free(m->row_ptr); free(m->ijk); free(m->col); free(m->shp); free(m->dshp); free(m->invk);
+    free(m->row_ptr); free(m->ijk); free(m->col); free(m->shp); free(m->dshp); free(m->invk); free(m->char_dist);
     m->row_ptr = NULL; m->ijk = NULL; m->col = NULL; m->shp = NULL; m->dshp = NULL; m->invk = NULL;
+    m->char_dist = NULL;
 }

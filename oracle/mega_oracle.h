@@ -88,6 +88,12 @@ typedef struct or_fint_io {
     int linit;
     int nthreads;                                /* 1 = reference summation order       */
     int64_t ret_fail;                            /* count of non-converged return maps  */
+    /* automatic time step (CONSTRUCT_FINT_PD only, FINT_PD.F90:690-754) */
+    int auto_ts;                                 /* AUTO_TS                             */
+    double dlt_fac;                              /* DLT_FAC                             */
+    double *char_dist;                           /* n GCHAR_DIST (in/out; kept where a
+                                                    particle has no other neighbor)     */
+    double dlt_fint;                             /* DLT_FINT out                        */
 } or_fint_io;
 int or_construct_fint(or_fint_io *io, const or_control *c);
 
@@ -128,6 +134,12 @@ typedef struct or_model {
     double *invk;                      /* 9n (peridynamics)                    */
     double geom[6]; int32_t nb[3]; int32_t *ijk; int search_count;
     int64_t ret_fail;
+    /* automatic time step (IAUTO_TS = 1, peridynamics): dt is then set by each
+     * step's internal force (MAIN.F90:428) and the first step has no predictor
+     * (MAIN.F90:348-350) */
+    int auto_ts;
+    double dlt_fac;
+    double *char_dist;                 /* n, allocated by or_model_init        */
 } or_model;
 
 /* All arrays in the model struct are caller-owned except the CSR/shape/bin

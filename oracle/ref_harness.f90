@@ -28,8 +28,22 @@ module ref_harness
   integer, parameter :: max_neigh = 1000
   integer, save, allocatable :: ispace(:), jspace(:), kspace(:)
   integer, save, allocatable :: nodes_in_bin(:), nodelist_in_bin(:,:)
+  ! DLT_FINT of the last CONSTRUCT_FINT_PD call (set when AUTO_TS is on)
+  double precision, save :: last_dlt_fint = 0.0d0
 
 contains
+
+  ! AUTO_TS and DLT_FAC as PRE_CONTROL.F90:522-524 sets them (after ref_set_control)
+  subroutine ref_set_auto_ts(iauto, fac) bind(C, name="ref_set_auto_ts")
+    integer(c_int), value :: iauto
+    real(c_double), value :: fac
+    AUTO_TS = (iauto /= 0)
+    DLT_FAC = fac
+  end subroutine
+
+  real(c_double) function ref_last_dlt_fint() bind(C, name="ref_last_dlt_fint")
+    ref_last_dlt_fint = last_dlt_fint
+  end function
 
   subroutine ref_set_control(rk_degree_, rk_cont_, rk_impl_, itype_int_, iql, ql_coef_, ql_len_, &
                              ishsup, ilag, ifinite, grav) bind(C, name="ref_set_control")
@@ -335,6 +349,7 @@ contains
         ghost, gprop, gstate, gstress, gstrain, h_s, s_s, gdinc, gdinc_tot, gmat_type, lebc, &
         gn, gstart, d, gstack, gshp, dshp, ddshp, mx, ginvk, lin, &
         fint, dlt_fint, chard, gmax_wvel, dxs, dys, dzs, xm, ym, zm, bod, work, dl)
+    last_dlt_fint = dlt_fint
     deallocate(sm_len, sm_area, sm_vol, nsni_fac, h_s, s_s, dshp, ddshp, dxs, dys, dzs, xm, ym, zm, chard, bod)
     deallocate(ghost, lebc)
   end subroutine

@@ -204,6 +204,7 @@ struct Params {
     int shsup, finite_strain, has_gravity;
     double grav[3];
     double dt;
+    const double *dt_dev;                            // automatic time step: this step's DLT (device), else dt
     // correspondence peridynamics
     int pd, pd_linit;
     double *Kinv;                                    // 9 per node, inverse shape tensor
@@ -342,9 +343,11 @@ __device__ __forceinline__ int xcd_block()
 // ---------------------------------------------------------------------------
 // predictor (MAIN.F90:353-355): d = dt v + dt^2/2 a; dtot += d; v += dt/2 a
 
+__device__ __forceinline__ double step_dt(const Params &p) { return p.dt_dev ? *p.dt_dev : p.dt; }
+
 __device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
 {
-    const double dt = p.dt;
+    const double dt = step_dt(p);
     const double v = p.v[q], a = p.a[q];
     const double d = dt * v + dt * dt * 0.5 * a;
     const double t = p.dtot[q] + d;
@@ -1285,14 +1288,15 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
         p.fext[q] = fx;
         return;
     }
+    const double dt = step_dt(p);
     double v = in.v, acc = 0.0;
     if (in.ebc == 1) {
         v = 0.0;
         fx = -f;
     } else {
         const double c = in.damp * in.mass;
-        acc = (fx - f - c * v) / (in.mass + 0.5 * p.dt * c);
-        v = v + 0.5 * p.dt * acc;
+        acc = (fx - f - c * v) / (in.mass + 0.5 * dt * c);
+        v = v + 0.5 * dt * acc;
     }
     p.a[q] = acc;
     p.fext[q] = fx;
@@ -1301,7 +1305,6 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
     if (!p.finite_strain) ie = 0.5 * f * in.dtot;
     ee = 0.5 * (fx + in.fxo) * in.d;
     if (PREDICT) {
-        const double dt = p.dt;
         const double d = dt * v + dt * dt * 0.5 * acc;
         const double t = in.dtot + d;
         p.d[q] = d;
@@ -1522,6 +1525,78 @@ __global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
         double *go = p.gvec + 3 * (int64_t)node;
         go[0] = g[0]; go[1] = g[1]; go[2] = g[2];
     }
+}
+
+// Automatic time step (IAUTO_TS = 1; FINT_PD.F90:690-754): per particle the
+// shortest bond at the current coordinates (neighbors other than itself; a
+// particle with none keeps its previous distance), over its wave speed; the
+// smallest over all particles, times DLT_FAC, is the step's DLT (MAIN.F90:428).
+// The distance is DSQRT((XJ-XI)**2 + (YJ-YI)**2 + (ZJ-ZI)**2) unfused and in
+// that order; the minimum is taken over squared distances, whose square root
+// is the same since a correctly rounded sqrt is monotone.  Non-negative
+// doubles order as their bit patterns, so the blocks meet in one unsigned
+// 64-bit atomicMin; a NaN quotient is no candidate.
+__global__ __launch_bounds__(kBlock) void k_pd_dt(Params p, double *char_dist, unsigned long long *dmin)
+{
+#pragma clang fp contract(off)
+    __shared__ unsigned long long s_min[kBlock / 64];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kGroup), lane = (int)(t % kGroup);
+    double best = INFINITY;
+    int found = 0;
+    if (i < p.n) {
+        const double xi = p.x[3 * (int64_t)i], yi = p.x[3 * (int64_t)i + 1], zi = p.x[3 * (int64_t)i + 2];
+        const int e = p.row_ptr[i + 1];
+        for (int q = p.row_ptr[i] + lane; q < e; q += kGroup) {
+            const int j = p.col[q];
+            if (j == i) continue;
+            const double dx = p.x[3 * (int64_t)j] - xi, dy = p.x[3 * (int64_t)j + 1] - yi,
+                         dz = p.x[3 * (int64_t)j + 2] - zi;
+            best = fmin(best, dx * dx + dy * dy + dz * dz);
+            found = 1;
+        }
+    }
+#pragma unroll
+    for (int m = kGroup / 2; m >= 1; m >>= 1) {
+        best = fmin(best, __shfl_xor(best, m, kGroup));
+        found |= __shfl_xor(found, m, kGroup);
+    }
+    unsigned long long key = ~0ull;
+    if (i < p.n && lane == 0) {
+        double cd;
+        if (found) {
+            cd = sqrt(best);
+            char_dist[i] = cd;
+        } else {
+            cd = char_dist[i];
+        }
+        const double w = cd / p.wave_vel[i];
+        if (w == w) key = (unsigned long long)__double_as_longlong(w);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const unsigned long long o = __shfl_xor(key, m, 64);
+        key = o < key ? o : key;
+    }
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long k = s_min[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) k = s_min[w] < k ? s_min[w] : k;
+        if (k != ~0ull) atomicMin(dmin, k);
+    }
+}
+
+// DLT = DLT_FAC * the smallest quotient (FINT_PD.F90:750); the minimum is
+// reset for the next step
+__global__ void k_pd_dt_fin(unsigned long long *dmin, double dlt_fac, double *dt_dev, double *hist, int s)
+{
+    if (threadIdx.x != 0) return;
+    const double d = __longlong_as_double((long long)*dmin) * dlt_fac;
+    dt_dev[0] = d;
+    hist[s] = d;
+    *dmin = ~0ull;
 }
 
 // mixed peridynamics: FP32 u = x - X0 of every particle for the gather
@@ -2328,6 +2403,13 @@ struct mega_ctx {
     int has_lists = 0, has_shapes = 0, predicted = 0;
     int dshp_stale = 0;   // the last shapes formed the gather sums and left the list-order gradients unwritten
     double dt = 0.0;
+    // automatic time step (peridynamics): this step's DLT, each step's DLT of the
+    // call, the characteristic distances and the running minimum
+    int auto_on = 0, auto_step = 0;
+    double dlt_fac = 0.0;
+    double *dt_dev = nullptr, *dt_hist = nullptr, *char_dist = nullptr;
+    unsigned long long *dt_min = nullptr;
+    int64_t dt_hist_cap = 0;
     int has_gravity = 0;
     std::vector<Material> host_mats;
     // learned return map
@@ -2398,6 +2480,7 @@ Params params(mega_ctx *c)
     p.shsup = c->c.shsup; p.finite_strain = c->c.finite_strain; p.has_gravity = c->has_gravity;
     for (int k = 0; k < 3; ++k) p.grav[k] = c->c.gravity[k];
     p.dt = c->dt;
+    p.dt_dev = c->auto_on ? c->dt_dev : nullptr;
     p.surr_tiles = c->surr_tiles; p.surr_bias = c->surr_bias; p.surr_norm = c->surr_norm;
     p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
@@ -2844,6 +2927,11 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
         Scope s(c, "k_pd_node");
         k_pd_node<R><<<grid_for(c->n), kBlock, 0, c->stream>>>(p);
     }
+    if (c->auto_on && force_mode >= 1) {   // this step's DLT before the corrector uses it
+        Scope s(c, "k_pd_dt");
+        k_pd_dt<<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->char_dist, c->dt_min);
+        k_pd_dt_fin<<<1, 64, 0, c->stream>>>(c->dt_min, c->dlt_fac, c->dt_dev, c->dt_hist, c->auto_step);
+    }
     if (force_mode >= 0) {
         Scope s(c, "k_pd_force");
         if (force_mode == 0) k_pd_force<R, false, false><<<g, kBlock, lds_f, c->stream>>>(p, nb);
@@ -3113,7 +3201,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (c->has_gravity) e |= dalloc(&c->fext_nmo, n3);
     if (c->mixed) { e |= dalloc(&c->d4, n); e |= dalloc(&c->t4, n); }
     if (c->mixed && c->pd) e |= dalloc(&c->u4, n);
-    else e |= dalloc(&c->wave_vel, n);
+    // wave speeds: FP64, and peridynamics in both precisions (the automatic time step)
+    if (!c->mixed || c->pd) e |= dalloc(&c->wave_vel, n);
     e |= dalloc(&c->Fst, 10 * (size_t)n);
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
     e |= dalloc(&c->win, n3); e |= dalloc(&c->winv, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
@@ -3184,7 +3273,8 @@ int mega_destroy(mega_ctx *c)
                     c->ijk, c->cell_start, c->cell_cursor, c->cell_nodes, c->wpart, c->epart, c->flags,
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
                     c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
-                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair, c->fpair2};
+                    c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair, c->fpair2, c->dt_dev,
+                    c->dt_hist, c->char_dist, c->dt_min};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -3460,10 +3550,8 @@ int mega_constitute(int32_t n, const double *lmat, double *stress, double *strai
     return st;
 }
 
-int mega_step(mega_ctx *c, double dt, int32_t nsteps)
+static int run_steps(mega_ctx *c, int32_t nsteps)
 {
-    if (!c || nsteps < 0 || !(dt > 0.0)) return MEGA_ERR_ARG;
-    c->dt = dt;
     if (c->profile) c->timer_used = 0;
     const int n = c->n;
     const bool lag = c->c.lagrangian;
@@ -3489,8 +3577,9 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
             }
             if (r) return r;
         }
-        // the predictor is fused into the previous k_force within one call
-        if (!c->predicted) r = do_predict(c);
+        // the predictor is fused into the previous k_force within one call; with
+        // the automatic time step the first step has none (MAIN.F90:348-350)
+        if (!c->predicted && !(c->auto_on && first)) r = do_predict(c);
         if (r) return r;
         if (last) {   // LOCAL_VEL_PHY, LOCAL_ACL_PHY as DO_INTERP forms them (HANDELER.F90:114-137)
             if (c->pd) {
@@ -3509,6 +3598,7 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
             }
         }
         if (c->pd) {
+            c->auto_step = s;
             r = do_pd_fint(c, 0, last ? 1 : 2);   // DO_INTERP is the identity (HANDELER.F90:134-137)
         } else if (lag) {
             r = do_fint(c, fm, true, last ? 1 : 2);   // DO_INTERP fused into the gather
@@ -3530,6 +3620,48 @@ int mega_step(mega_ctx *c, double dt, int32_t nsteps)
     }
     c->hv_active = 0;
     return check_flags(c);
+}
+
+int mega_step(mega_ctx *c, double dt, int32_t nsteps)
+{
+    if (!c || nsteps < 0 || !(dt > 0.0)) return MEGA_ERR_ARG;
+    c->dt = dt;
+    c->auto_on = 0;
+    return run_steps(c, nsteps);
+}
+
+int mega_step_auto(mega_ctx *c, double dlt_fac, int32_t nsteps, double *dlt)
+{
+    // the RKPM CONSTRUCT_FINT leaves DLT_FINT unset (its estimate is commented
+    // out, FINT.F90:1250-1326), so only the peridynamic path has one
+    if (!c || !c->pd || !c->wave_vel || nsteps < 0 || !(dlt_fac > 0.0) || !(dlt_fac < INFINITY)) return MEGA_ERR_ARG;
+    if (!c->dt_dev) {
+        if (dalloc(&c->dt_dev, 1) || dalloc(&c->dt_min, 1) || dalloc(&c->char_dist, c->n)) return MEGA_ERR_NOMEM;
+        HIPCHK(hipMemsetAsync(c->dt_min, 0xff, sizeof(unsigned long long), c->stream));
+        HIPCHK(hipMemsetAsync(c->char_dist, 0, sizeof(double) * c->n, c->stream));
+        HIPCHK(hipMemsetAsync(c->dt_dev, 0, sizeof(double), c->stream));
+    }
+    if (nsteps > c->dt_hist_cap) {
+        if (c->dt_hist) (void)hipFree(c->dt_hist);
+        c->dt_hist = nullptr;
+        c->dt_hist_cap = 0;
+        if (dalloc(&c->dt_hist, (size_t)nsteps)) return MEGA_ERR_NOMEM;
+        c->dt_hist_cap = nsteps;
+    }
+    c->dlt_fac = dlt_fac;
+    c->auto_on = 1;
+    int r = run_steps(c, nsteps);
+    if (r == MEGA_OK || r == MEGA_ERR_RETURN_MAP) {
+        std::vector<double> h((size_t)nsteps);
+        if (nsteps > 0) {
+            HIPCHK(hipMemcpyAsync(h.data(), c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->dt = h[nsteps - 1];
+        }
+        if (dlt)
+            for (int32_t k = 0; k < nsteps; ++k) dlt[k] = h[k];
+    }
+    return r;
 }
 
 static int field_width(int f)

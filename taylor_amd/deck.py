@@ -491,14 +491,15 @@ def read_k(path: str) -> Mesh:
 
 
 def write_control(path: str, kfile: str, prob: Problem, set_names: list, time_output: float,
-                  out_vars: str = "unformatted") -> None:
-    """Control.dat with the cards the reference reads (PRE_CONTROL.F90:132-588)."""
+                  out_vars: str = "unformatted", dlt_fac: float | None = None) -> None:
+    """Control.dat with the cards the reference reads (PRE_CONTROL.F90:132-588).
+    dlt_fac: IAUTO_TS = 1 with that DLT_FAC in place of the fixed DT."""
     mat = prob.materials[0]
     p = mat.props
     rows = [p[0:5], p[5:10], p[10:15], p[15:20], p[20:25], np.zeros(5)]
     fx_set = [nm for nm in set_names if nm == "WALL"]
     txt = ["*MODEL", "# Name", os.path.basename(kfile), "",
-           "*TIME PARAMETERS", "#TIME_END", "  %.17e" % prob.time_end, "#IAUTO_TS", "  0", "#DT", "  %.17e" % prob.dt, "",
+           "*TIME PARAMETERS", "#TIME_END", "  %.17e" % prob.time_end, "#IAUTO_TS", "  %d" % (dlt_fac is not None), "#DT", "  %.17e" % (prob.dt if dlt_fac is None else dlt_fac), "",
            "*QL PARAMETERS", "# USE, COEFFICIENT", "   %d          %.17g" % (prob.ql, prob.ql_coef), "",
            "*RK PARAMETERS", "# RK_DEGREE, RK_CONT, RK_IMPL", "   1          3        0", "",
            "*PERIDYNAMICS", "# SWITCH", "  %d" % int(prob.peridynamics), "",

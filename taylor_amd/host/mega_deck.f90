@@ -49,6 +49,7 @@ module mega_deck
   type :: deck_model
      integer(c_int32_t) :: n = 0, numel = 0
      real(c_double) :: dt = 0.0d0, time_end = 0.0d0, time_output = 0.0d0
+     logical :: auto_ts = .false.                     ! IAUTO_TS = 1: dt holds DLT_FAC
      type(mega_control) :: ctrl
      real(c_double), allocatable :: x0(:), win(:), vol(:), mass(:), prop(:), vinit(:), norm_win(:)
      real(c_double), allocatable :: vinit_card(:)     ! MODEL_VINIT as the cards give it (fixed dofs too)
@@ -389,7 +390,12 @@ contains
     end if
     if (.not. have_outv) m%outv([1, 2, 7, 8, 12, 16]) = .true.   ! defaults (PRE_CONTROL.F90:624-636)
     if (.not. have_time) call deck_fail('card *TIME PARAMETERS not found')
-    if (iauto /= 0) call deck_fail('IAUTO_TS = 1 (automatic time step) is not on this path')
+    ! IAUTO_TS = 1: DLT_VAL is DLT_FAC (PRE_CONTROL.F90:522-531); only the peridynamic
+    ! internal force sets a step (FINT_PD.F90:690-754; FINT.F90:1250-1326 are commented out)
+    if (iauto /= 0 .and. iauto /= 1) call deck_fail('choose IAUTO_TS = (0,1) in card *TIME PARAMETERS')
+    if (iauto == 1 .and. iperi /= 1) &
+         call deck_fail('IAUTO_TS = 1 needs *PERIDYNAMICS 1: the RKPM internal force sets no time step')
+    m%auto_ts = (iauto == 1)
     m%dt = dltval
     if (m%time_output < 0.0d0) m%time_output = m%time_end * 0.01d0
     if (len_trim(kfile) == 0) call deck_fail('card *MODEL not found')

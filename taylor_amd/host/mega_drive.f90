@@ -10,6 +10,8 @@
 !   mega_drive --deck <Control.dat> <result.bin> [precision [nsteps [chunk]]]
 !       precision 0 = FP64 (default), 1 = mixed; nsteps 0 or absent = the
 !       reference's count (TIME += DLT until TIME > TIME_END, MAIN.F90:644-656).
+!       With IAUTO_TS = 1 (peridynamic decks) each step's DLT comes from the
+!       device (mega_step_auto) and TIME is summed here as MAIN sums it.
 !       With 'unformatted' among the output variables, frames MEGAOUT_*.vtk
 !       are written next to Control.dat on the reference's schedule: frame 0
 !       before the loop, then after each step where the time since the last
@@ -56,7 +58,7 @@ program mega_drive
      n = dm%n
      dt = dm%dt
      ctrl = dm%ctrl
-     if (nsteps <= 0) then
+     if (nsteps <= 0 .and. .not. dm%auto_ts) then
         t = 0.0d0
         do
            nsteps = nsteps + 1
@@ -81,42 +83,11 @@ program mega_drive
   call mega_check(mega_create(ctrl, n, x0, win, vol, mass, ebc, ebc_node, mat_type, prop, vinit, ctx), 'mega_create')
   frames = .false.
   if (allocated(dm%norm_win)) frames = dm%unf
-  if (frames) then
-     call vtk_notice_skipped(dm)
-     allocate(outstep(nsteps))
-     tc = 0.0d0
-     do j = 1, nsteps          ! TIME_COUNTER: tested after the step, then advanced
-        outstep(j) = tc > dm%time_output
-        if (outstep(j)) tc = 0.0d0
-        tc = tc + dt
-     end do
-     call get_frame(.true.)
-     call vtk_frame(0, dm, ff)
-     frame = 1
+  if (dm%auto_ts) then
+     call auto_loop()
+  else
+     call fixed_loop()
   end if
-  done = 0
-  do while (done < nsteps)
-     k = min(chunk, nsteps - done)
-     if (frames) then          ! stop at the next frame
-        do j = done + 1, done + k
-           if (outstep(j)) then
-              k = j - done
-              exit
-           end if
-        end do
-     end if
-     call mega_check(mega_step(ctx, dt, k), 'mega_step')
-     done = done + k
-     if (frames) then
-        if (outstep(done)) then
-           call get_frame(.false.)
-           call vtk_frame(frame, dm, ff)
-           frame = frame + 1
-        end if
-     end if
-     call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
-     write(*, '(A,I8,A,3ES14.6)') 'step ', done, '  energies (int, kin, ext)', e_int, e_kin, e_ext
-  end do
   allocate(x(3*n), eps(n))
   call mega_check(mega_download(ctx, MEGA_F_X, x), 'mega_download')
   call mega_check(mega_download(ctx, MEGA_F_EPS, eps), 'mega_download')
@@ -137,6 +108,100 @@ program mega_drive
 
 contains
 
+  ! fixed DLT: the step count and the frame steps are known in advance
+  subroutine fixed_loop()
+    if (frames) then
+       call vtk_notice_skipped(dm)
+       allocate(outstep(nsteps))
+       tc = 0.0d0
+       do j = 1, nsteps          ! TIME_COUNTER: tested after the step, then advanced
+          outstep(j) = tc > dm%time_output
+          if (outstep(j)) tc = 0.0d0
+          tc = tc + dt
+       end do
+       call get_frame(.true.)
+       call vtk_frame(0, dm, ff)
+       frame = 1
+    end if
+    done = 0
+    do while (done < nsteps)
+       k = min(chunk, nsteps - done)
+       if (frames) then          ! stop at the next frame
+          do j = done + 1, done + k
+             if (outstep(j)) then
+                k = j - done
+                exit
+             end if
+          end do
+       end if
+       call mega_check(mega_step(ctx, dt, k), 'mega_step')
+       done = done + k
+       if (frames) then
+          if (outstep(done)) then
+             call get_frame(.false.)
+             call vtk_frame(frame, dm, ff)
+             frame = frame + 1
+          end if
+       end if
+       call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
+       write(*, '(A,I8,A,3ES14.6)') 'step ', done, '  energies (int, kin, ext)', e_int, e_kin, e_ext
+    end do
+  end subroutine fixed_loop
+
+  ! IAUTO_TS = 1: each step's DLT comes back from the device; TIME and
+  ! TIME_COUNTER are summed here in step order (MAIN.F90:641-642).  A frame
+  ! follows a step when TIME_COUNTER, before that step's DLT is added, exceeds
+  ! TIME_OUTPUT (MAIN.F90:557), so with frames the steps go one at a time;
+  ! without them a call takes at most half the remaining time at the last DLT.
+  subroutine auto_loop()
+    real(c_double), allocatable :: dl(:)
+    real(c_double) :: dlast
+    integer(c_int32_t) :: kk
+    integer :: jj
+    logical :: out_now
+    allocate(dl(max(chunk, 1)))
+    if (frames) then
+       call vtk_notice_skipped(dm)
+       call get_frame(.true.)
+       call vtk_frame(0, dm, ff)
+       frame = 1
+    end if
+    t = 0.0d0; tc = 0.0d0; dlast = 0.0d0; done = 0; out_now = .false.
+    do
+       kk = 1
+       if (.not. frames .and. dlast > 0.0d0) &
+            kk = int(max(1.0d0, min(dble(max(chunk, 1)), 0.5d0 * (dm%time_end - t) / dlast)), c_int32_t)
+       if (nsteps > 0) kk = min(kk, nsteps - done)
+       call mega_check(mega_step_auto(ctx, dm%dt, kk, dl), 'mega_step_auto')
+       do jj = 1, kk
+          out_now = tc > dm%time_output
+          if (out_now) tc = 0.0d0
+          t = t + dl(jj)
+          tc = tc + dl(jj)
+       end do
+       done = done + kk
+       dlast = dl(kk)
+       if (frames .and. out_now) then
+          call get_frame(.false.)
+          call vtk_frame(frame, dm, ff)
+          frame = frame + 1
+       end if
+       if (nsteps > 0) then
+          if (done >= nsteps) exit
+       else if (t > dm%time_end) then
+          exit
+       end if
+       if (.not. frames .or. mod(done, max(chunk, 1)) == 0) then
+          call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
+          write(*, '(A,I8,A,ES14.6,A,3ES14.6)') 'step ', done, '  DLT', dlast, '  energies (int, kin, ext)', &
+               e_int, e_kin, e_ext
+       end if
+    end do
+    call mega_check(mega_energies(ctx, e_int, e_kin, e_ext), 'mega_energies')
+    write(*, '(A,I0,A,ES24.16,A,ES14.6,A,3ES14.6)') 'automatic time step: ', done, ' steps, TIME ', t, &
+         ', last DLT', dlast, '; energies (int, kin, ext)', e_int, e_kin, e_ext
+  end subroutine auto_loop
+
   ! the fields of one frame that the requested variables need; before the
   ! first step the reference writes the generalized displacement, velocity
   ! and acceleration (MAIN.F90:256-261)
@@ -145,9 +210,9 @@ contains
     if (.not. allocated(ff%x)) then
        allocate(ff%x(3*n), ff%dsp(3*n), ff%vel(3*n), ff%acl(3*n), ff%fint(3*n), ff%stress(6*n), ff%strain(6*n), &
                 ff%eps(n), ff%strain_eq(n), ff%wave(n), ff%dstate(4*n))
-       ff%have_wave = ctrl%precision == MEGA_FP64
+       ff%have_wave = ctrl%precision == MEGA_FP64 .or. ctrl%peridynamics == 1
        if (dm%outv(19) .and. .not. ff%have_wave) &
-            write(*, '(A)') 'output variable max_wave_vel is not computed in mixed precision; left out'
+            write(*, '(A)') 'output variable max_wave_vel is not computed in mixed-precision RKPM; left out'
     end if
     call mega_check(mega_download(ctx, MEGA_F_X, ff%x), 'mega_download')
     if (initial) then

@@ -87,6 +87,14 @@ module mega_iface
        integer(c_int32_t), value :: nsteps
      end function mega_step
 
+     integer(c_int) function mega_step_auto(ctx, dlt_fac, nsteps, dlt) bind(C, name="mega_step_auto")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dlt_fac
+       integer(c_int32_t), value :: nsteps
+       real(c_double), intent(out) :: dlt(*)
+     end function mega_step_auto
+
      integer(c_int) function mega_download(ctx, field, host) bind(C, name="mega_download")
        import :: c_int, c_int32_t, c_double, c_ptr
        type(c_ptr), value :: ctx

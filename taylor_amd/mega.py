@@ -76,6 +76,7 @@ def load_library():
         "mega_preprocess": [i32, vp, i32, vp, vp, vp, vp, vp, vp],
         "mega_preprocess_noalloc": [i32, vp, i32, vp, vp, vp, vp, vp, vp],
         "mega_step": [vp, C.c_double, i32],
+        "mega_step_auto": [vp, C.c_double, i32, vp],
         "mega_download": [vp, i32, vp],
         "mega_upload": [vp, i32, vp],
         "mega_energies": [vp, dp, dp, dp],
@@ -231,6 +232,14 @@ class Mega:
         st = self.lib.mega_step(self.ctx, float(dt or self.prob.dt), int(nsteps))
         _chk(st, "mega_step", allow=(1, 2))
         return st
+
+    def step_auto(self, dlt_fac: float, nsteps: int = 1):
+        """nsteps with the automatic time step (IAUTO_TS = 1, peridynamics);
+        returns (status, each step's DLT)."""
+        dlt = np.zeros(max(int(nsteps), 1))
+        st = self.lib.mega_step_auto(self.ctx, float(dlt_fac), int(nsteps), _p(dlt))
+        _chk(st, "mega_step_auto", allow=(1, 2))
+        return st, dlt[:int(nsteps)]
 
     def download(self, name: str) -> np.ndarray:
         w = WIDTH.get(name, 3)

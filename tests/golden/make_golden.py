@@ -8,6 +8,7 @@ compressed .npz files.  The fixtures are data only.
 
     python tests/golden/make_golden.py            # dense, geometry, search/RK/FINT, runs
     python tests/golden/make_golden.py --pd       # peridynamics vectors and run
+    python tests/golden/make_golden.py --pd-auto  # peridynamics with the automatic time step
     python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
     python tests/golden/make_golden.py --damage   # material types 6 and 3
     python tests/golden/make_golden.py --vtk      # the reference's VTK frames of a small deck
@@ -464,7 +465,84 @@ def pd_main():
         np.savez_compressed(os.path.join(HERE, "run_pd.npz"), **mega_run(prob, mesh, 40, (1, 10, 40), wd))
 
 
+PD_DLT_FAC = 0.3
+
+
+def pd_auto_main():
+    """The automatic time step (IAUTO_TS = 1) of CONSTRUCT_FINT_PD: DLT_FINT of
+    the reference's call on pd.npz's deformed state, and 40 steps of the
+    reference program on the small peridynamic bar (mm deck, DLT_FAC 0.3) with
+    every step's TIME (TIME.OUT, MAIN.F90:653) and frames 1, 10 and 40.  The
+    deck's TIME_END lies midway between the oracle's times after steps 39 and
+    40; the program's own TIME.OUT says how many steps it took."""
+    from oracle_bind import OracleRun, load_oracle
+    ref = load_ref()
+    g = np.load(os.path.join(HERE, "pd.npz"))
+    out = {}
+    with tempfile.TemporaryDirectory() as wd:
+        cwd = os.getcwd()
+        os.chdir(wd)   # the reference's WARN appends to MEGA.log and WARN.log in the working directory
+        open("MEGA.log", "a").close()
+        open("WARN.log", "a").close()
+        try:
+            prob = small(peridynamics=1, mm_units=True)
+            ref_set_control(ref, prob)
+            ref.ref_set_auto_ts(1, PD_DLT_FAC)
+            n = prob.n
+            coo, win, vol = (np.ascontiguousarray(prob.coo), np.ascontiguousarray(prob.win),
+                             np.ascontiguousarray(prob.vol))
+            gn, st = g["mm_gn"], g["mm_gstack"]
+            gs = (np.concatenate([[0], np.cumsum(gn)[:-1]]) + 1).astype(np.int32)
+            nnz = int(gn.sum())
+            props = np.ascontiguousarray(prob.props_table()[prob.mat_id])
+            ebc = np.ascontiguousarray(prob.ebc_node, dtype=np.int32)
+            state = np.zeros((n, 20))
+            state[:, 2] = g["pd0_eps0"]
+            s, e = g["pd0_stress0"].copy(), g["pd0_strain0"].copy()
+            dtot = g["pd0_dtot"]
+            f, wv = np.zeros((n, 3)), np.zeros(n)
+            ref.ref_construct_fint_pd(n, win, vol, coo, np.ascontiguousarray(coo + dtot), props, state, s, e,
+                                      np.ascontiguousarray(g["pd0_dinc"]), np.ascontiguousarray(dtot),
+                                      np.full(n, 2, np.int32), ebc, np.ascontiguousarray(gn), gs, nnz,
+                                      np.ascontiguousarray(st), g["mm_omega"].copy(),
+                                      np.ascontiguousarray(g["mm_invk"]).copy(), int(gn.max()), 0, f, wv, prob.dt)
+            out.update({"call_dlt_fac": np.array([PD_DLT_FAC]), "call_dlt": np.array([ref.ref_last_dlt_fint()]),
+                        "call_wave": wv, "call_f": f})
+        finally:
+            os.chdir(cwd)
+    prob = small(peridynamics=1, mm_units=True)
+    run = OracleRun(load_oracle(), prob, dlt_fac=PD_DLT_FAC).step(40)
+    t = np.cumsum(run.dlt)
+    mesh = deck.taylor_bar_mesh(H * 1e3, radius=3.8e-3 * 1e3, length=LEN * 1e3, seed=0)
+    with tempfile.TemporaryDirectory() as wd:
+        kfile = os.path.join(wd, "bar.k")
+        deck.write_k(kfile, mesh)
+        prob.time_end = 0.5 * (t[38] + t[39])
+        deck.write_control(os.path.join(wd, "Control.dat"), kfile, prob, ["BAR", "WALL"], time_output=-1.0,
+                           out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps",
+                           dlt_fac=PD_DLT_FAC)
+        with open(os.path.join(wd, "mega.out"), "w") as fo:
+            subprocess.run([REF_MEGA, "-j", "1"], cwd=wd, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
+                           check=True, timeout=3600)
+        with open(os.path.join(wd, "TIME.OUT")) as f:
+            times = [float(ln.split()[0].replace("D", "E")) for ln in f if ln.strip()]
+        out.update({"run_dlt_fac": np.array([PD_DLT_FAC]), "run_time_end": np.array([prob.time_end]),
+                    "run_time": np.array(times)})
+        for k in (1, 10, 40):
+            fr = read_unformatted_vtk(os.path.join(wd, "MEGAOUT_%04d.vtk" % k))
+            out[f"step{k}_x"] = fr["points"]
+            out[f"step{k}_u"] = np.stack([fr["displacement_x"], fr["displacement_y"], fr["displacement_z"]], 1)
+            out[f"step{k}_vphy"] = np.stack([fr["velocity_x"], fr["velocity_y"], fr["velocity_z"]], 1)
+            out[f"step{k}_fint"] = np.stack([fr["fint_x"], fr["fint_y"], fr["fint_z"]], 1)
+            out[f"step{k}_stress"] = np.stack([fr[f"stress_{c}"] for c in ("xx", "yy", "zz", "yz", "xz", "xy")], 1)
+            out[f"step{k}_eps"] = fr["eps"]
+    np.savez_compressed(os.path.join(HERE, "pd_auto.npz"), **out)
+
+
 def main():
+    if "--pd-auto" in sys.argv:
+        pd_auto_main()
+        return
     if "--final" in sys.argv:
         with tempfile.TemporaryDirectory() as wd:
             np.savez_compressed(os.path.join(HERE, "taylor_h05_final.npz"), **final_state_vectors(wd))

@@ -33,7 +33,8 @@ class OrFintIO(C.Structure):
                 ("stress", C.c_void_p), ("strain", C.c_void_p), ("state", C.c_void_p),
                 ("fint", C.c_void_p), ("fext", C.c_void_p), ("wave_vel", C.c_void_p), ("strain_eq", C.c_void_p),
                 ("invk", C.c_void_p),
-                ("int_work", C.c_double), ("linit", C.c_int), ("nthreads", C.c_int), ("ret_fail", C.c_int64)]
+                ("int_work", C.c_double), ("linit", C.c_int), ("nthreads", C.c_int), ("ret_fail", C.c_int64),
+                ("auto_ts", C.c_int), ("dlt_fac", C.c_double), ("char_dist", C.c_void_p), ("dlt_fint", C.c_double)]
 
 
 class OrModel(C.Structure):
@@ -51,7 +52,8 @@ class OrModel(C.Structure):
                 ("row_ptr", C.POINTER(C.c_int64)), ("col", C.POINTER(C.c_int32)),
                 ("shp", C.POINTER(C.c_double)), ("dshp", C.POINTER(C.c_double)), ("invk", C.POINTER(C.c_double)),
                 ("geom", C.c_double * 6), ("nb", C.c_int32 * 3), ("ijk", C.POINTER(C.c_int32)),
-                ("search_count", C.c_int), ("ret_fail", C.c_int64)]
+                ("search_count", C.c_int), ("ret_fail", C.c_int64),
+                ("auto_ts", C.c_int), ("dlt_fac", C.c_double), ("char_dist", C.POINTER(C.c_double))]
 
 
 def _ptr(a):
@@ -127,6 +129,8 @@ def load_ref():
     lib.ref_udfm.argtypes = [dp, C.c_int, dp, dp, dp, ip, C.c_int, ip, C.c_int, dp, dp]
     lib.ref_construct_fint_pd.argtypes = ([C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip, ip, ip,
                                            C.c_int, ip, dp, dp, C.c_int, C.c_int, dp, dp, C.c_double])
+    lib.ref_set_auto_ts.argtypes = [C.c_int, C.c_double]
+    lib.ref_last_dlt_fint.restype = C.c_double
     return lib
 
 
@@ -148,7 +152,9 @@ def ref_set_control(ref, prob):
 class OracleRun:
     """Whole explicit run of the oracle (MAIN.F90:304-658) on a deck.Problem."""
 
-    def __init__(self, lib, prob, nthreads: int = 1):
+    def __init__(self, lib, prob, nthreads: int = 1, dlt_fac: float | None = None):
+        """dlt_fac: the automatic time step's DLT_FAC (IAUTO_TS = 1,
+        peridynamics); each step's DLT is then kept in ``dlt``."""
         n = prob.n
         self.lib, self.prob = lib, prob
         f = lambda *s: np.zeros(s, dtype=np.float64)
@@ -170,6 +176,9 @@ class OracleRun:
         for k, a in self.arr.items():
             setattr(m, k, _ptr(a))
         m.nmat = len(prob.materials)
+        if dlt_fac is not None:
+            m.auto_ts, m.dlt_fac = 1, float(dlt_fac)
+        self.dlt = []
         self.m = m
         if lib.or_model_init(C.byref(m)) != 0:
             raise RuntimeError("oracle init failed")
@@ -179,6 +188,7 @@ class OracleRun:
             r = self.lib.or_model_step(C.byref(self.m))
             if r == -2:
                 raise RuntimeError("unsupported material")
+            self.dlt.append(self.m.dt)
         return self
 
     def csr(self):
@@ -254,9 +264,10 @@ def oracle_udfm(lib, prob, rp, col):
 
 
 def oracle_fint_pd(lib, prob, rp, col, omega, invk, linit, dinc, dinc_tot, x_cur, stress, strain, state,
-                   nthreads=1):
+                   nthreads=1, dlt_fac=None, info=None):
     """One or_construct_fint_pd call; omega/invk/stress/strain/state updated in
-    place.  Returns fint, wave_vel, status."""
+    place.  Returns fint, wave_vel, status.  With dlt_fac the automatic time
+    step is on and info["dlt"] receives DLT_FINT."""
     n = prob.n
     io = OrFintIO()
     keep = dict(win=np.ascontiguousarray(prob.win), vol=np.ascontiguousarray(prob.vol),
@@ -275,7 +286,12 @@ def oracle_fint_pd(lib, prob, rp, col, omega, invk, linit, dinc, dinc_tot, x_cur
         setattr(io, k, _ptr(a))
     io.fint, io.fext, io.wave_vel = _ptr(fint), _ptr(fext), _ptr(wv)
     io.linit, io.nthreads = int(linit), nthreads
+    cd = np.zeros(n)
+    if dlt_fac is not None:
+        io.auto_ts, io.dlt_fac, io.char_dist = 1, float(dlt_fac), _ptr(cd)
     st = lib.or_construct_fint_pd(C.byref(io), C.byref(control(prob)))
+    if info is not None:
+        info["dlt"] = io.dlt_fint
     return fint, wv, st
 
 

@@ -38,7 +38,8 @@ def run(ctl, tmp_path):
     (lambda t: t.replace("#SET\nWALL\n#FIXITY", "#SET\nBAR\n#FIXITY"), None),
     (lambda t: t + "*BOUNDARY CONDITION\n#SET\nWALL\n#FIXITY\n1 0 0\n", "multiple values of fixity"),
     (lambda t: t.replace("#SET\nWALL\n#FIXITY", "#SET\nNOSUCHSET\n#FIXITY"), "set name not found"),
-    (lambda t: t.replace("#IAUTO_TS\n  0", "#IAUTO_TS\n  1"), "automatic time step"),
+    (lambda t: t.replace("#IAUTO_TS\n  0", "#IAUTO_TS\n  1"), "IAUTO_TS = 1 needs *PERIDYNAMICS 1"),
+    (lambda t: t.replace("#IAUTO_TS\n  0", "#IAUTO_TS\n  2"), "choose IAUTO_TS = (0,1)"),
     (lambda t: t.replace("#TYPE\n2\n", "#TYPE\nJ2\n"), "error reading MAT_TYPE"),
 ])
 def test_reader_refuses_malformed_decks(tmp_path, edit, message):

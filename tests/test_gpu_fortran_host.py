@@ -186,3 +186,35 @@ def test_fortran_host_writes_the_reference_frames(tmp_path):
         worst = compare_frames(ours, str(text), 1e-9)
         print(name, "largest relative difference", worst)
     assert not os.path.exists(tmp_path / "MEGAOUT_0003.vtk")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVE), reason="Fortran host not built")
+def test_fortran_host_automatic_time_step_follows_the_reference_program(tmp_path):
+    """A peridynamic deck with IAUTO_TS = 1 (DLT_FAC 0.3, the small mm bar of
+    tests/golden/pd_auto.npz, TIME_END as the reference program ran it): the
+    Fortran host sums TIME from the device's DLT of every step, stops after
+    the reference program's 40 steps, and ends at its TIME and positions."""
+    a = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pd_auto.npz"))
+    h, length = 0.7, 6.0   # mm
+    prob = deck.taylor_problem(0.7e-3, length=6e-3, seed=0, pdsearch_steps=3, peridynamics=1, mm_units=True)
+    prob.time_end = float(a["run_time_end"][0])
+    mesh = deck.taylor_bar_mesh(h, radius=3.8, length=length, seed=0)
+    assert np.array_equal(mesh.coo, prob.coo)
+    kfile = str(tmp_path / "bar.k")
+    deck.write_k(kfile, mesh)
+    ctl = str(tmp_path / "Control.dat")
+    deck.write_control(ctl, kfile, prob, ["BAR", "WALL"], time_output=1.0, dlt_fac=float(a["run_dlt_fac"][0]))
+    result = str(tmp_path / "result.bin")
+    out = subprocess.run([DRIVE, "--deck", ctl, result, "0", "0", "16"], capture_output=True, text=True, timeout=120)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    line = [ln for ln in out.stdout.splitlines() if ln.startswith("automatic time step:")][0]
+    steps, t = int(line.split()[3]), float(line.split()[6].rstrip(","))
+    r = np.fromfile(result, dtype=np.float64)
+    x = r[:3 * prob.n].reshape(prob.n, 3)
+    e_t = abs(t - a["run_time"][-1]) / a["run_time"][-1]
+    e_x = float(np.abs(x - a["step40_x"]).max() / np.abs(a["step40_x"]).max())
+    print(f"steps {steps}, TIME {t:.17e} (reference {a['run_time'][-1]:.17e}, {e_t:.1e}), positions {e_x:.1e}")
+    assert steps == 40 == len(a["run_time"])
+    assert e_t < 1e-14 and e_x < 1e-12
+

@@ -89,6 +89,58 @@ def test_pd_steps_match_reference_program():
     m.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pd_automatic_time_step_matches_reference_program(precision):
+    """IAUTO_TS = 1 (DLT_FAC 0.3): 40 steps against the reference program's
+    frames and its TIME after every step (tests/golden/pd_auto.npz).  The first
+    step has no predictor, so its frame shows no displacement."""
+    a = load("pd_auto.npz")
+    fac = float(a["run_dlt_fac"][0])
+    ref_dlt = np.diff(np.concatenate([[0.0], a["run_time"]]))
+    prob = small(mm_units=True)
+    m = mega.Mega(prob, precision=mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
+    done, dlt, worst = 0, [], {}
+    for k in (1, 10, 40):
+        st, d = m.step_auto(fac, k - done)
+        assert st == 0
+        dlt.extend(d.tolist())
+        done = k
+        errs = {"x": rel_err(m.download("x"), a[f"step{k}_x"]),
+                "u": rel_err(m.download("uphy"), a[f"step{k}_u"]) if k > 1 else float(np.abs(m.download("uphy")).max()),
+                "v": rel_err(m.download("v") - 0.5 * dlt[-1] * m.download("a"), a[f"step{k}_vphy"]),
+                "fint": rel_err(m.download("fint"), a[f"step{k}_fint"]),
+                "stress": rel_err(m.download("stress"), a[f"step{k}_stress"])}
+        for q, e in errs.items():
+            worst[q] = max(worst.get(q, 0.0), e)
+        print(f"{precision} step {k}: " + " ".join(f"{q} {e:.2e}" for q, e in errs.items()), flush=True)
+    m.close()
+    # TIME as the program sums it
+    t, tt = [], 0.0
+    for d in dlt:
+        tt = tt + d
+        t.append(tt)
+    e_dlt = float(np.abs(np.array(dlt) - ref_dlt).max() / ref_dlt.max())
+    e_t = float(np.abs(np.array(t) - a["run_time"]).max() / a["run_time"].max())
+    print(f"{precision}: DLT {e_dlt:.2e}, TIME {e_t:.2e} (first DLT {dlt[0]:.6e}, last {dlt[-1]:.6e})", flush=True)
+    if precision == "fp64":
+        assert e_dlt < 1e-14 and e_t < 1e-14
+        assert worst["x"] < 1e-12 and worst["u"] < 1e-10
+        assert worst["v"] < 1e-9 and worst["fint"] < 1e-9 and worst["stress"] < 1e-9
+    else:
+        # measured: DLT 1.1e-8, TIME 2.4e-9, x 6.5e-9, v 1.2e-7, fint 1.5e-7, stress 1.8e-7
+        assert e_dlt < 1.2e-7 and e_t < 2.5e-8
+        assert worst["x"] < 7e-8 and worst["v"] < 1.2e-6 and worst["fint"] < 1.6e-6 and worst["stress"] < 1.8e-6
+
+
+def test_automatic_time_step_needs_peridynamics():
+    """The RKPM CONSTRUCT_FINT leaves DLT_FINT unset: MEGA_ERR_ARG."""
+    prob = deck.taylor_problem(H, length=LEN, seed=0, pdsearch_steps=3)
+    m = mega.Mega(prob)
+    dlt = np.zeros(2)
+    assert m.lib.mega_step_auto(m.ctx, 0.3, 2, dlt.ctypes.data) == 6   # MEGA_ERR_ARG
+    m.close()
+
+
 def test_pd_si_units_degenerate_like_the_reference(oracle):
     """In SI units at sub-mm spacing the shape tensor is below INVERSE's pivot
     threshold: the reference warns and keeps the unreduced tensor; so does the
@@ -114,4 +166,5 @@ def test_pd_mixed_precision_steps():
         m.close()
     errs = {k: rel_err(out[mega.MEGA_MIXED][k], out[mega.MEGA_FP64][k]) for k in out[mega.MEGA_FP64]}
     print("mixed vs FP64 after 40 steps:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert errs["x"] < 1e-7 and errs["uphy"] < 1e-4 and errs["v"] < 1e-3 and errs["stress"] < 1e-3
+    # measured: x 7.0e-9, uphy 7.3e-8, v 1.3e-7, stress 2.5e-7
+    assert errs["x"] < 7e-8 and errs["uphy"] < 7e-7 and errs["v"] < 1.3e-6 and errs["stress"] < 2.5e-6

@@ -60,6 +60,26 @@ def test_reduced_plate_matches_oracle(lib, oracle, precision):
         assert fsum < 3e-9
 
 
+def test_reduced_plate_automatic_time_step_matches_oracle(lib, oracle):
+    """IAUTO_TS = 1 (DLT_FAC 0.5) on the reduced plate, FP64, 20 steps against
+    the oracle: every step's DLT within 1e-14 and the state as with a fixed step."""
+    prob = deck.pd_plate_problem(lx=10e-3, ly=10e-3, lz=2e-3, impact_radius=2e-3)
+    nsteps, fac = 20, 0.5
+    g = mega.Mega(prob, precision=mega.MEGA_FP64)
+    st, dlt = g.step_auto(fac, nsteps)
+    run = OracleRun(oracle, prob, nthreads=8, dlt_fac=fac).step(nsteps)
+    A = run.arr
+    e_dlt = float(np.abs(dlt - np.array(run.dlt)).max() / max(run.dlt))
+    errs = {k: rel_err(g.download(k), A[r]) for k, r in
+            (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    g.close()
+    print(f"reduced plate, automatic step: DLT {dlt[0]:.4e} .. {dlt[-1]:.4e} (fixed {prob.dt:.4e}); "
+          f"DLT vs oracle {e_dlt:.1e}; {errs}", flush=True)
+    assert st == 0
+    assert e_dlt < 1e-14
+    assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+
+
 @pytest.fixture(scope="module")
 def plate():
     prob = deck.pd_plate_problem(preprocess_fn=mega.geometry)

@@ -187,6 +187,64 @@ def test_whole_run_pd_bit_exact(oracle):
         assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
 
 
+def test_pd_automatic_time_step_one_call_bit_exact(oracle):
+    """DLT_FINT of the reference's CONSTRUCT_FINT_PD with AUTO_TS on (FINT_PD.F90:
+    690-754) on the deformed yielding state of pd.npz: the shortest bond per
+    particle at the current coordinates over its secant wave speed, smallest
+    over the particles, times DLT_FAC."""
+    g, a = load("pd.npz"), load("pd_auto.npz")
+    prob = small(peridynamics=1, mm_units=True)
+    n = prob.n
+    rp, col, _ = oracle_search(oracle, prob)
+    st = np.zeros((n, 20))
+    st[:, 2] = g["pd0_eps0"]
+    s, e = g["pd0_stress0"].copy(), g["pd0_strain0"].copy()
+    dtot = g["pd0_dtot"]
+    info = {}
+    f, wv, status = oracle_fint_pd(oracle, prob, rp, col, g["mm_omega"].copy(), np.ascontiguousarray(g["mm_invk"]).copy(),
+                                   0, g["pd0_dinc"], dtot, prob.coo + dtot, s, e, st,
+                                   dlt_fac=float(a["call_dlt_fac"][0]), info=info)
+    assert status == 0
+    assert np.array_equal(wv, a["call_wave"]) and np.array_equal(f, a["call_f"])
+    assert info["dlt"] == float(a["call_dlt"][0])
+    # the rule itself: brute force over all pairs of the list at the current coordinates
+    x = prob.coo + dtot
+    best = np.full(n, np.inf)
+    for i in range(n):
+        j = col[rp[i]:rp[i + 1]]
+        j = j[j != i]
+        best[i] = np.sqrt(((x[j] - x[i]) ** 2).sum(1)).min()
+    assert np.isclose((best / wv).min() * float(a["call_dlt_fac"][0]), info["dlt"], rtol=1e-14, atol=0)
+
+
+def test_whole_run_pd_automatic_time_step_bit_exact(oracle):
+    """40 steps of the reference program with IAUTO_TS = 1 (DLT_FAC 0.3) on the
+    peridynamic mm deck: the first step has no predictor (MAIN.F90:348-350),
+    each step's DLT comes from its own internal force (MAIN.F90:428); every
+    step's TIME and frames 1, 10 and 40 bit-exact."""
+    a = load("pd_auto.npz")
+    run = OracleRun(oracle, small(peridynamics=1, mm_units=True), dlt_fac=float(a["run_dlt_fac"][0]))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        assert np.array_equal(A["x"], a[f"step{k}_x"])
+        assert np.array_equal(A["dphy_tot"], a[f"step{k}_u"])
+        assert np.array_equal(A["vphy"], a[f"step{k}_vphy"])
+        assert np.array_equal(A["fint"], a[f"step{k}_fint"])
+        assert np.array_equal(A["stress"], a[f"step{k}_stress"])
+        assert np.array_equal(A["state"][:, 2], a[f"step{k}_eps"])
+    # TIME = TIME + DLT in step order, as the program logs it
+    t, tt = [], 0.0
+    for d in run.dlt:
+        tt = tt + d
+        t.append(tt)
+    assert len(a["run_time"]) == 40 and np.array_equal(np.array(t), a["run_time"])
+    # the first step did not move the particles: its predictor is skipped
+    assert np.array_equal(a["step1_u"], np.zeros_like(a["step1_u"]))
+
+
 @pytest.mark.parametrize("tag,fn", [("dam6", "or_von_mises_dam"), ("dp3", "or_druck_prag")])
 def test_damage_models_bit_exact(oracle, tag, fn):
     """VON_MISES_DAM (type 6) and DRUCK_PRAG (type 3) of the compiled reference
