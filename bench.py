@@ -118,6 +118,10 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         # row start, the 27 transposed group cursors read and written; per pair the
         # neighbor index and its transposed slot written
         return n * (64 + 4 + 24 + 24 + 2 * 27 * 4) + nnz * (4 + 4)
+    if name == "k_pd_dt":
+        # the minimum over particles of the shortest bond (formed by the gather) over
+        # the wave speed: both read once
+        return n * (8 + 8)
     if name == "k_pd_gather":
         # pairs: bond index + influence weight; nodes: own X0, x, d, volume, row start,
         # the 18 gather sums written
@@ -153,6 +157,9 @@ def semi_step_bytes(n: int, nnz: int, mixed: bool) -> float:
     return kbar * (20 if mixed else 28) + (400 if mixed else 500)
 
 
+PD_DLT_FAC = 0.5
+
+
 def pd_plate(steps: int, warmup: int, modes=("mixed", "fp64")) -> dict:
     """Correspondence peridynamics (CONSTRUCT_FINT_PD) on the plate:
     100 x 100 x 5 mm at spacing 0.2 mm (jitter seed 3, 6.53M particles),
@@ -181,6 +188,17 @@ def pd_plate(steps: int, warmup: int, modes=("mixed", "fp64")) -> dict:
                      "max_bonds": mx, "kernels": kern}
         log(f"PD plate {name}: {1e3 * wall / steps:.3f} ms/step, {rate:.4e} node-steps/s, "
             + ", ".join(f"{k} {v['ms']:.3f} ms ({v['frac']:.2f} of HBM)" for k, v in kern.items()))
+    if "mixed" in modes:
+        # the automatic time step (IAUTO_TS = 1, DLT_FAC 0.5): the same step plus k_pd_dt
+        wall, _, kt, nnz, _, _ = timed_run(prob, "mixed", steps, warmup, dlt_fac=PD_DLT_FAC)
+        b = kernel_bytes("k_pd_dt", prob.n, nnz, True)
+        ms = kt["k_pd_dt"][0]
+        out["mixed_automatic_step"] = {"dlt_fac": PD_DLT_FAC, "rate": prob.n * steps / wall,
+                                       "ms_per_step": 1e3 * wall / steps, "k_pd_dt_ms": ms,
+                                       "k_pd_dt_frac": b / (ms * 1e-3) / PEAK_HBM,
+                                       "kernel_ms": {k: v[0] for k, v in kt.items()}}
+        log(f"PD plate mixed, automatic step: {1e3 * wall / steps:.3f} ms/step, k_pd_dt {ms:.3f} ms "
+            f"({out['mixed_automatic_step']['k_pd_dt_frac']:.2f} of HBM), k_pd_gather {kt['k_pd_gather'][0]:.3f} ms")
     return out
 
 
@@ -237,26 +255,28 @@ def make(prob, mode: str):
     return g
 
 
-def timed_run(prob, mode: str, steps: int, warmup: int, prelude: int = 0):
+def timed_run(prob, mode: str, steps: int, warmup: int, prelude: int = 0, dlt_fac: float | None = None):
     """Wall time of `steps` steps after `prelude` + `warmup` untimed steps,
     with HIP events on the force kernel only (the roofline's kernel; events on
     every kernel add a few percent), then the per-kernel breakdown from a
     further profiled run of up to 50 steps.  Returns (wall, force events of
     the timed region, breakdown, nnz, max neighbors, surrogate counters of the
-    timed region)."""
+    timed region).  dlt_fac: the automatic time step (peridynamics) with that
+    DLT_FAC instead of the fixed step."""
     g = make(prob, mode)
-    g.step(max(prelude + warmup, 1))     # includes the first step (search, shapes)
+    adv = g.step if dlt_fac is None else (lambda k: g.step_auto(dlt_fac, k))
+    adv(max(prelude + warmup, 1))        # includes the first step (search, shapes)
     if mode == "surrogate":
         g.surrogate_stats(reset=True)
     g.profile(2)
     t0 = time.perf_counter()
-    g.step(steps)                        # returns after the stream is synchronised
+    adv(steps)                           # returns after the stream is synchronised
     t1 = time.perf_counter()
     kt_timed = g.kernel_times()
     nnz, mx, _ = g.stats()
     st = g.surrogate_stats() if mode == "surrogate" else None
     g.profile(1)
-    g.step(min(steps, 50))
+    adv(min(steps, 50))
     kt = g.kernel_times()
     g.close()
     return (t1 - t0), kt_timed, kt, nnz, mx, st

@@ -205,6 +205,8 @@ struct Params {
     double grav[3];
     double dt;
     const double *dt_dev;                            // automatic time step: this step's DLT (device), else dt
+    double *char_dist;                               // automatic time step: the PD gather's shortest bond
+                                                     // per particle (GCHAR_DIST), else null
     // correspondence peridynamics
     int pd, pd_linit;
     double *Kinv;                                    // 9 per node, inverse shape tensor
@@ -314,6 +316,24 @@ __device__ __forceinline__ T group_sum(T v)
     if (W >= 8) v += dpp::mov<dpp::kHalfMirror>(v);
     if (W >= 16) v += dpp::mov<dpp::kMirror>(v);
     return v;
+}
+
+template <int W>
+__device__ __forceinline__ double group_min(double v)
+{
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16, "group of at most one DPP row");
+    if (W >= 2) v = fmin(v, dpp::mov<dpp::kXor1>(v));
+    if (W >= 4) v = fmin(v, dpp::mov<dpp::kXor2>(v));
+    if (W >= 8) v = fmin(v, dpp::mov<dpp::kHalfMirror>(v));
+    if (W >= 16) v = fmin(v, dpp::mov<dpp::kMirror>(v));
+    return v;
+}
+
+// (XJ-XI)**2 + (YJ-YI)**2 + (ZJ-ZI)**2 as FINT_PD.F90:724 evaluates it: unfused, left to right
+__device__ __forceinline__ double sq_len_unfused(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return a * a + b * b + c * c;
 }
 
 // block of kBlock threads -> thread 0 gets the sum
@@ -1527,51 +1547,24 @@ __global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
     }
 }
 
-// Automatic time step (IAUTO_TS = 1; FINT_PD.F90:690-754): per particle the
-// shortest bond at the current coordinates (neighbors other than itself; a
-// particle with none keeps its previous distance), over its wave speed; the
-// smallest over all particles, times DLT_FAC, is the step's DLT (MAIN.F90:428).
-// The distance is DSQRT((XJ-XI)**2 + (YJ-YI)**2 + (ZJ-ZI)**2) unfused and in
-// that order; the minimum is taken over squared distances, whose square root
-// is the same since a correctly rounded sqrt is monotone.  Non-negative
-// doubles order as their bit patterns, so the blocks meet in one unsigned
-// 64-bit atomicMin; a NaN quotient is no candidate.
-__global__ __launch_bounds__(kBlock) void k_pd_dt(Params p, double *char_dist, unsigned long long *dmin)
+// Automatic time step (IAUTO_TS = 1; FINT_PD.F90:690-754).  k_pd_gather
+// leaves each particle's shortest bond to another particle at the current
+// coordinates in char_dist (GCHAR_DIST; a particle with none keeps its
+// previous value); here the smallest quotient char_dist / wave speed over all
+// particles.  Non-negative doubles order as their bit patterns, so the blocks
+// (a grid-stride loop over at most kDtBlocks of them, which keeps the atomics
+// on the one address few) meet in one unsigned 64-bit atomicMin; a NaN
+// quotient is no candidate.
+constexpr int kDtBlocks = 1024;
+__global__ __launch_bounds__(kBlock) void k_pd_dt(int n, const double *char_dist, const double *wave_vel,
+                                                  unsigned long long *dmin)
 {
-#pragma clang fp contract(off)
     __shared__ unsigned long long s_min[kBlock / 64];
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(t / kGroup), lane = (int)(t % kGroup);
-    double best = INFINITY;
-    int found = 0;
-    if (i < p.n) {
-        const double xi = p.x[3 * (int64_t)i], yi = p.x[3 * (int64_t)i + 1], zi = p.x[3 * (int64_t)i + 2];
-        const int e = p.row_ptr[i + 1];
-        for (int q = p.row_ptr[i] + lane; q < e; q += kGroup) {
-            const int j = p.col[q];
-            if (j == i) continue;
-            const double dx = p.x[3 * (int64_t)j] - xi, dy = p.x[3 * (int64_t)j + 1] - yi,
-                         dz = p.x[3 * (int64_t)j + 2] - zi;
-            best = fmin(best, dx * dx + dy * dy + dz * dz);
-            found = 1;
-        }
-    }
-#pragma unroll
-    for (int m = kGroup / 2; m >= 1; m >>= 1) {
-        best = fmin(best, __shfl_xor(best, m, kGroup));
-        found |= __shfl_xor(found, m, kGroup);
-    }
     unsigned long long key = ~0ull;
-    if (i < p.n && lane == 0) {
-        double cd;
-        if (found) {
-            cd = sqrt(best);
-            char_dist[i] = cd;
-        } else {
-            cd = char_dist[i];
-        }
-        const double w = cd / p.wave_vel[i];
-        if (w == w) key = (unsigned long long)__double_as_longlong(w);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double w = char_dist[i] / wave_vel[i];
+        const unsigned long long k = (unsigned long long)__double_as_longlong(w);
+        if (w == w && k < key) key = k;
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -1621,14 +1614,23 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
     const int lane = threadIdx.x & (G - 1);
     const bool live = node < p.n;
     double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // automatic time step: the shortest squared bond to another particle at the
+    // current coordinates (FINT_PD.F90:710-730)
+    const bool cd_on = p.char_dist != nullptr;
+    double dmin = INFINITY;
     if constexpr (MIX) {
         // mixed: S - K = sum w V_j (u_j - u_i) (X_j - X_i)^T with u = x - X0 and
         // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d and the
         // packed bond records {w V_j (X_j - X_i), j's position in the block's source
         // set, or j}; k_pd_node adds K back as F = I + (S - K) K^-1.  A block with a
         // source set stages u and d of its sources in LDS.
+        // With the automatic time step the set also holds FP32 positions relative to
+        // the block's first particle, x_j - x_b: the bond x_j - x_i is then formed in
+        // FP32 from numbers of the bond's own size.
         float4 *Lu = reinterpret_cast<float4 *>(mega_dyn);
         int m = 0;
+        const int nb0 = min(blk * (kBlock / G), p.n - 1);
+        const double xb[3] = {p.x[3 * (int64_t)nb0], p.x[3 * (int64_t)nb0 + 1], p.x[3 * (int64_t)nb0 + 2]};
         if (p.gset_ptr) {
             const int s0 = p.gset_ptr[blk];
             m = p.gset_ptr[blk + 1] - s0;
@@ -1636,11 +1638,16 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                 const int j = p.gset[s0 + k];
                 Lu[k] = p.u4[j];
                 Lu[m + k] = p.d4[j];
+                if (cd_on)
+                    Lu[2 * m + k] = make_float4((float)(p.x[3 * (int64_t)j] - xb[0]), (float)(p.x[3 * (int64_t)j + 1] - xb[1]),
+                                                (float)(p.x[3 * (int64_t)j + 2] - xb[2]), 0.f);
             }
             if (m > 0) __syncthreads();
         }
         if (live) {
             const float4 ui = p.u4[node], di = p.d4[node];
+            const double xi[3] = {p.x[3 * (int64_t)node], p.x[3 * (int64_t)node + 1], p.x[3 * (int64_t)node + 2]};
+            const float pi[3] = {(float)(xi[0] - xb[0]), (float)(xi[1] - xb[1]), (float)(xi[2] - xb[2])};
             const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
             if (p.fpair) {
                 for (int base = q0 + lane; base < q1; base += G * kPdUnroll) {
@@ -1655,6 +1662,22 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                         const int j = __float_as_int(rec[e].w);
                         uj[e] = m > 0 ? Lu[j] : p.u4[j];
                         dj[e] = m > 0 ? Lu[m + j] : p.d4[j];
+                    }
+                    if (cd_on) {
+#pragma unroll
+                        for (int e = 0; e < kPdUnroll; ++e) {
+                            const int j = __float_as_int(rec[e].w);
+                            if (m > 0) {   // the particle itself is the bond with a zero record
+                                if (rec[e].x != 0.f || rec[e].y != 0.f || rec[e].z != 0.f) {
+                                    const float4 pj = Lu[2 * m + j];
+                                    const float cx = pj.x - pi[0], cy = pj.y - pi[1], cz = pj.z - pi[2];
+                                    dmin = fmin(dmin, (double)(cx * cx + cy * cy + cz * cz));
+                                }
+                            } else if (j != node) {
+                                dmin = fmin(dmin, sq_len_unfused(p.x[3 * (int64_t)j] - xi[0], p.x[3 * (int64_t)j + 1] - xi[1],
+                                                                 p.x[3 * (int64_t)j + 2] - xi[2]));
+                            }
+                        }
                     }
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
@@ -1674,6 +1697,9 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                 const float *rx = as<float>(p.dshp), *ry = rx + p.cap, *rz = ry + p.cap;
                 for (int q = q0 + lane; q < q1; q += G) {
                     const int j = p.col[q];
+                    if (cd_on && j != node)
+                        dmin = fmin(dmin, sq_len_unfused(p.x[3 * (int64_t)j] - xi[0], p.x[3 * (int64_t)j + 1] - xi[1],
+                                                         p.x[3 * (int64_t)j + 2] - xi[2]));
                     const float r[3] = {rx[q], ry[q], rz[q]};
                     const float4 u = p.u4[j], d = p.d4[j];
                     const float a[3] = {u.x - ui.x, u.y - ui.y, u.z - ui.z};
@@ -1699,6 +1725,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
             const double h[3] = {p.X0[3 * j] - X[0], p.X0[3 * j + 1] - X[1], p.X0[3 * j + 2] - X[2]};
             const double hs[3] = {p.x[3 * j] - x[0], p.x[3 * j + 1] - x[1], p.x[3 * j + 2] - x[2]};
             const double hd[3] = {p.d[3 * j] - d[0], p.d[3 * j + 1] - d[1], p.d[3 * j + 2] - d[2]};
+            if (cd_on && j != node) dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
 #pragma unroll
             for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -1710,6 +1737,12 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) { S[k] = group_sum<G>(S[k]); L[k] = group_sum<G>(L[k]); }
+    if (cd_on) {
+        // the square root of the smallest square is the smallest DSQRT (sqrt is
+        // monotone); a particle with no other neighbor keeps its distance
+        dmin = group_min<G>(dmin);
+        if (live && lane == 0 && dmin < INFINITY) p.char_dist[node] = sqrt(dmin);
+    }
     if (live && lane == 0) {
         R *red = as<R>(p.red) + (int64_t)W * node;
 #pragma unroll
@@ -2481,6 +2514,7 @@ Params params(mega_ctx *c)
     for (int k = 0; k < 3; ++k) p.grav[k] = c->c.gravity[k];
     p.dt = c->dt;
     p.dt_dev = c->auto_on ? c->dt_dev : nullptr;
+    p.char_dist = c->auto_on ? c->char_dist : nullptr;
     p.surr_tiles = c->surr_tiles; p.surr_bias = c->surr_bias; p.surr_norm = c->surr_norm;
     p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
@@ -2916,7 +2950,8 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     int nb = 0;
     const unsigned g = xcd_grid(c->n, &nb, MIX ? kPdNpb : kNodesPerBlock);
     // mixed with source sets: u and d (32 B) per gather source, T rows (40 B) per force source
-    const size_t lds_g = (MIX && p.gset_ptr) ? (size_t)c->gset_max * 32 : 0;
+    // (+ 16 B per source with the automatic time step: FP32 positions relative to the block)
+    const size_t lds_g = (MIX && p.gset_ptr) ? (size_t)c->gset_max * (p.char_dist ? 48 : 32) : 0;
     const size_t lds_f = (MIX && p.fset_ptr) ? (size_t)c->fset_max * 40 : 0;
     {
         Scope s(c, "k_pd_gather");
@@ -2929,7 +2964,8 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     }
     if (c->auto_on && force_mode >= 1) {   // this step's DLT before the corrector uses it
         Scope s(c, "k_pd_dt");
-        k_pd_dt<<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(p, c->char_dist, c->dt_min);
+        k_pd_dt<<<std::min<unsigned>(grid_for(c->n), kDtBlocks), kBlock, 0, c->stream>>>(c->n, c->char_dist,
+                                                                                           c->wave_vel, c->dt_min);
         k_pd_dt_fin<<<1, 64, 0, c->stream>>>(c->dt_min, c->dlt_fac, c->dt_dev, c->dt_hist, c->auto_step);
     }
     if (force_mode >= 0) {

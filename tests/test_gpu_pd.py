@@ -127,9 +127,11 @@ def test_pd_automatic_time_step_matches_reference_program(precision):
         assert worst["x"] < 1e-12 and worst["u"] < 1e-10
         assert worst["v"] < 1e-9 and worst["fint"] < 1e-9 and worst["stress"] < 1e-9
     else:
-        # measured: DLT 1.1e-8, TIME 2.4e-9, x 6.5e-9, v 1.2e-7, fint 1.5e-7, stress 1.8e-7
-        assert e_dlt < 1.2e-7 and e_t < 2.5e-8
-        assert worst["x"] < 7e-8 and worst["v"] < 1.2e-6 and worst["fint"] < 1.6e-6 and worst["stress"] < 1.8e-6
+        # the shortest bonds come from FP32 positions relative to each block of
+        # particles; measured: DLT 3.3e-7, TIME 5.4e-8, x 9.4e-9, v 2.7e-7, fint 3.3e-7,
+        # stress 5.4e-7
+        assert e_dlt < 3.5e-6 and e_t < 6e-7
+        assert worst["x"] < 1e-7 and worst["v"] < 3e-6 and worst["fint"] < 3.5e-6 and worst["stress"] < 6e-6
 
 
 def test_automatic_time_step_needs_peridynamics():
