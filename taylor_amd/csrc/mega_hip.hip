@@ -365,15 +365,30 @@ __device__ __forceinline__ int xcd_block()
 
 __device__ __forceinline__ double step_dt(const Params &p) { return p.dt_dev ? *p.dt_dev : p.dt; }
 
+// the predictor's increment DLT*V + DLT**2*0.5*A and half-step velocity
+// V + DLT*0.5*A, unfused as the reference writes them: the standalone
+// predictor (the first step of a call) and the one fused into the force
+// kernel then give the same bits
+__device__ __forceinline__ double predicted_d(double dt, double v, double a)
+{
+#pragma clang fp contract(off)
+    return dt * v + dt * dt * 0.5 * a;
+}
+__device__ __forceinline__ double predicted_v(double dt, double v, double a)
+{
+#pragma clang fp contract(off)
+    return v + dt * 0.5 * a;
+}
+
 __device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
 {
     const double dt = step_dt(p);
     const double v = p.v[q], a = p.a[q];
-    const double d = dt * v + dt * dt * 0.5 * a;
+    const double d = predicted_d(dt, v, a);
     const double t = p.dtot[q] + d;
     p.d[q] = d;
     p.dtot[q] = t;
-    p.v[q] = v + dt * 0.5 * a;
+    p.v[q] = predicted_v(dt, v, a);
     if (p.pd) {   // peridynamics: physical = generalized, so x moves with d
         const double u = p.uphy[q] + d;
         p.uphy[q] = u;
@@ -1067,22 +1082,17 @@ __device__ void harvest_put(const Params &p, bool h, const R *der, const R *s_in
 // for the force gather.  Lanes past the last node compute on a copy of it and
 // store nothing, so that every lane reaches the wave-wide sections.
 
+// One node's update from its gather sums red (red_stride values): kinematics,
+// rotation, trial, return map, state stores and P = V J sigma F^-T (into P,
+// when valid); returns the node's work increment.  cnt: the wave's counters
+// of the learned return map.
 template <class R, int FM, bool SURR>
-__device__ __forceinline__ void node_update(const Params &p)
+__device__ __forceinline__ double node_core(const Params &p, int i, bool valid, const R *red, R *P, unsigned *cnt)
 {
-    __shared__ double s_red[kBlock / 64];
-    __shared__ unsigned s_cnt[kBlock / 64][4];   // learned return map: counters per wave
     constexpr bool MIX = std::is_same<R, float>::value;
     static_assert(!SURR || MIX, "the learned return map runs in mixed precision");
-    constexpr bool DIR = FM == F_DIRECT;
-    constexpr int W = red_stride<R, DIR>();
     constexpr int FO = MIX ? 12 : 10;
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i0 < p.n;
-    const int i = valid ? i0 : p.n - 1;
     double work = 0.0;
-    R red[W];
-    load_vec<R, W>(as<R>(p.red) + (int64_t)W * i, red);
     double L0[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) L0[k] = (double)red[k];
@@ -1166,7 +1176,7 @@ __device__ __forceinline__ void node_update(const Params &p)
     if constexpr (SURR) {
         const bool cand = valid && mt.type == 2 && mi == p.surr_mat && mega::trial_excess(pred, ep, mt) > 0.0f;
         float s_new[6];
-        done = surr_return(p, cand, der, sr, pred, ep, s_new, mt, s_cnt[threadIdx.x >> 6]);
+        done = surr_return(p, cand, der, sr, pred, ep, s_new, mt, cnt);
         if (done)
 #pragma unroll
             for (int k = 0; k < 6; ++k) sr[k] = s_new[k];
@@ -1219,7 +1229,6 @@ __device__ __forceinline__ void node_update(const Params &p)
         // P = V J sigma IF^T  (f_j = P grad phi_ij)
         const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
         const double w = vol * det;
-        R P[pstride<R>()];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -1228,8 +1237,24 @@ __device__ __forceinline__ void node_update(const Params &p)
         P[9] = (R)(mt.rho * w);
 #pragma unroll
         for (int k = 10; k < pstride<R>(); ++k) P[k] = R(0);
-        store_vec<R, pstride<R>()>(as<R>(p.P) + pstride<R>() * (int64_t)i, P);
     }
+    return work;
+}
+
+template <class R, int FM, bool SURR>
+__device__ __forceinline__ void node_update(const Params &p)
+{
+    __shared__ double s_red[kBlock / 64];
+    __shared__ unsigned s_cnt[kBlock / 64][4];   // learned return map: counters per wave
+    constexpr int W = red_stride<R, FM == F_DIRECT>();
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < p.n;
+    const int i = valid ? i0 : p.n - 1;
+    R red[W];
+    load_vec<R, W>(as<R>(p.red) + (int64_t)W * i, red);
+    R P[pstride<R>()];
+    const double work = node_core<R, FM, SURR>(p, i, valid, red, P, s_cnt[threadIdx.x >> 6]);
+    if (valid) store_vec<R, pstride<R>()>(as<R>(p.P) + pstride<R>() * (int64_t)i, P);
     const double ws = block_sum(work, s_red);
     if (threadIdx.x == 0) p.wpart[blockIdx.x] += ws;
     if (SURR && threadIdx.x == 0) {   // the block's counters; no same-address atomics
@@ -1325,11 +1350,11 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
     if (!p.finite_strain) ie = 0.5 * f * in.dtot;
     ee = 0.5 * (fx + in.fxo) * in.d;
     if (PREDICT) {
-        const double d = dt * v + dt * dt * 0.5 * acc;
+        const double d = predicted_d(dt, v, acc);
         const double t = in.dtot + d;
         p.d[q] = d;
         p.dtot[q] = t;
-        p.v[q] = v + dt * 0.5 * acc;
+        p.v[q] = predicted_v(dt, v, acc);
         if (p.pd) {
             const double u = p.uphy[q] + d;
             p.uphy[q] = u;
