@@ -403,16 +403,30 @@ def cpu_baseline_port(sample_len: float, cores: int):
 
 
 def copy_calibration():
-    """Five device-to-device copies of 1 GiB with torch's copy kernel: under
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE this gives the counters' reading
-    for a known 1 GiB read and 1 GiB write per dispatch."""
+    """Reference kernels for the HBM byte counters (rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE): five device-to-device copies of 1 GiB with torch's copy kernel
+    (1 GiB read and 1 GiB written per dispatch), then, for windows of 128 B,
+    4 KiB and 64 KiB, five gathers of 4 B elements (torch.take) from a 256 MiB
+    array through an index that visits every element once, shuffled within
+    each window: 256 MiB of elements and 512 MiB of int64 indices read, 256 MiB
+    written per dispatch, whatever the window; the counters then show how the
+    reading for scattered loads departs from the copy's."""
     import torch
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
+    g = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+    for w in (1 << 5, 1 << 10, 1 << 14):   # elements per window: 128 B, 4 KiB, 64 KiB
+        perm = torch.randperm(w, generator=torch.Generator().manual_seed(1)).to("cuda")
+        idx = (torch.arange(1 << 26, device="cuda").view(-1, w)[:, perm]).reshape(-1).contiguous()
+        for _ in range(5):
+            torch.take(g, idx)
+        del idx
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5}))
+    print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5,
+                      "gather_read_bytes_per_dispatch": g.numel() * 4 + idx.numel() * 8,
+                      "gather_write_bytes_per_dispatch": g.numel() * 4}))
 
 
 def main():
