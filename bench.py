@@ -425,7 +425,7 @@ def copy_calibration():
         del idx
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5,
-                      "gather_read_bytes_per_dispatch": g.numel() * 4 + idx.numel() * 8,
+                      "gather_read_bytes_per_dispatch": g.numel() * (4 + 8),
                       "gather_write_bytes_per_dispatch": g.numel() * 4}))
 
 
