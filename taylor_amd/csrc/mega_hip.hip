@@ -2034,6 +2034,14 @@ constexpr int kCandPerLane = MEGA_CAND_PER_LANE;   // candidates per lane in one
 #endif
 constexpr int kMaskCap = MEGA_MASK_CAP;   // ballot masks kept per node by COUNT for FILL: its first 64 kMaskCap candidates
 
+// Transposed rows place their 27 source-bin groups with dz slowest: group g
+// holds source bin tgroup(g) of the 27 neighbor offsets (dz fastest, the order
+// of the search's counters).  The nine source bins of one z layer, which the
+// bin-ordered pair kernels visit close together in time, then write adjacent
+// groups of a row.  tgroup reverses the three base-3 digits, so it is its own
+// inverse.
+__device__ __forceinline__ int tgroup(int g) { return 9 * (g % 3) + 3 * ((g / 3) % 3) + g / 9; }
+
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *cells, const double *x,
                                                    const double *win, BinGeom g, const int32_t *cell_start,
@@ -2212,7 +2220,8 @@ __global__ void k_tbin_start(int n, const int32_t *trow, int32_t *tbin)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int run = trow[j];
-    for (int e = 0; e < 27; ++e) {
+    for (int g = 0; g < 27; ++g) {
+        const int e = tgroup(g);
         const int v = tbin[27 * (int64_t)j + e];
         tbin[27 * (int64_t)j + e] = run;
         run += v;
