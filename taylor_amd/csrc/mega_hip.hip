@@ -2042,14 +2042,23 @@ constexpr int kMaskCap = MEGA_MASK_CAP;   // ballot masks kept per node by COUNT
 // inverse.
 __device__ __forceinline__ int tgroup(int g) { return 9 * (g % 3) + 3 * ((g / 3) % 3) + g / 9; }
 
-template <bool FILL>
+// Box support, COUNT: each (node, candidate) pair is first decided in FP32
+// from positions relative to the wave's bin corner (mega::box_rel,
+// box_limits, box_axis: within 4 bins of the corner, windows above 1/50 bin,
+// margin 1e-4 of the window); pairs it leaves undecided (near the window's
+// edge, far from the corner, tiny windows, non-finite) take the exact FP64
+// test.  FILL, which tests only candidates beyond the kept masks, keeps the
+// FP64 test (the prefilter's set-up made it slower).
+
+template <bool FILL, bool SHSUP>
 __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *cells, const double *x,
                                                    const double *win, BinGeom g, const int32_t *cell_start,
-                                                   const int32_t *cell_nodes, int shsup, int32_t *counts,
+                                                   const int32_t *cell_nodes, int32_t *counts,
                                                    const int32_t *row_ptr, int32_t *col, int32_t *tbin,
                                                    int32_t *tsrc, int32_t *tpos, int32_t *tq, uint64_t *masks)
 {
     __shared__ int s_beg[kBlock / 64][28], s_off[kBlock / 64][28];
+    constexpr bool PRE = !SHSUP && !FILL;   // the FP32 prefilter
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int w = (xcd_block() * blockDim.x + threadIdx.x) >> 6;   // neighboring bins on one XCD
     if (w >= ncells) return;   // whole waves leave together
@@ -2079,13 +2088,24 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
     if (lane == 0) s_off[wv][27] = total;
     __builtin_amdgcn_wave_barrier();
     const int own0 = cell_start[m0], nown = cell_start[m0 + 1] - own0;
+    // the bin's corner and the FP32 prefilter's reach (box support)
+    const double xr[3] = {g.mn[0] + (ci - 1) * g.bin[0], g.mn[1] + (cj - 1) * g.bin[1], g.mn[2] + (ck - 1) * g.bin[2]};
+    const float reach[3] = {(float)(4.0 * g.bin[0]), (float)(4.0 * g.bin[1]), (float)(4.0 * g.bin[2])};
+    const float wmin[3] = {(float)(0.02 * g.bin[0]), (float)(0.02 * g.bin[1]), (float)(0.02 * g.bin[2])};
     for (int t0 = 0; t0 < nown; t0 += 64) {   // own nodes, 64 at a time: lane t keeps node t0+t's offset
         const int tn = min(64, nown - t0);
         const bool more = t0 + 64 < nown;
         int myi = 0, mine = 0;
+        float myq[3] = {0.f, 0.f, 0.f};   // own node relative to the corner (FP32), NaN when out of reach
         if (lane < tn) {
             myi = cell_nodes[own0 + t0 + lane];
             mine = FILL ? row_ptr[myi] : 0;
+            if (PRE) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    myq[k] = mega::box_rel(x[3 * (int64_t)myi + k], xr[k], reach[k]);
+                }
+            }
         }
         for (int base = 0; base < total; base += 64 * kCandPerLane) {
             // does this chunk test candidates here (COUNT, or beyond the kept masks)?
@@ -2093,6 +2113,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
             int cj_[kCandPerLane], tix[kCandPerLane], kc[kCandPerLane];
             double cx[kCandPerLane], cy[kCandPerLane], cz[kCandPerLane];
             double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
+            float pf[kCandPerLane][3], lo[kCandPerLane][3], hi[kCandPerLane][3];   // box prefilter
 #pragma unroll
             for (int r = 0; r < kCandPerLane; ++r) {
                 const int c = base + 64 * r + lane;
@@ -2109,16 +2130,36 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 kc[r] = (j >= 0 && (FILL || t0 > 0)) ? tbin[tix[r]] : 0;
                 cx[r] = cy[r] = cz[r] = 0.0;
                 wx[r] = wy[r] = wz[r] = 1.0;
-                if (tests) {
-                    cx[r] = x[3 * jj]; cy[r] = x[3 * jj + 1]; cz[r] = x[3 * jj + 2];
-                    wx[r] = win[3 * jj]; wy[r] = win[3 * jj + 1]; wz[r] = win[3 * jj + 2];
+                if (!PRE) {
+                    if (tests) {
+                        cx[r] = x[3 * jj]; cy[r] = x[3 * jj + 1]; cz[r] = x[3 * jj + 2];
+                        wx[r] = win[3 * jj]; wy[r] = win[3 * jj + 1]; wz[r] = win[3 * jj + 2];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        pf[r][k] = 0.f; lo[r][k] = -1.f; hi[r][k] = -1.f;   // no candidate: rejected at once
+                    }
+                    if (tests && j >= 0) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            pf[r][k] = mega::box_rel(x[3 * (int64_t)jj + k], xr[k], reach[k]);
+                            mega::box_limits(win[3 * (int64_t)jj + k], wmin[k], lo[r][k], hi[r][k]);
+                        }
+                    }
                 }
             }
             for (int t = 0; t < tn; ++t) {
                 const int i = __builtin_amdgcn_readfirstlane(__shfl(myi, t, 64));
                 int out = __shfl(mine, t, 64);
                 double xi = 0.0, yi = 0.0, zi = 0.0;
-                if (tests) { xi = x[3 * i]; yi = x[3 * i + 1]; zi = x[3 * i + 2]; }
+                if (tests && !PRE) { xi = x[3 * i]; yi = x[3 * i + 1]; zi = x[3 * i + 2]; }
+                float q[3] = {0.f, 0.f, 0.f};
+                if (PRE) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        q[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq[k]), t));
+                }
 #pragma unroll
                 for (int r = 0; r < kCandPerLane; ++r) {
                     if (base + 64 * r >= total) break;   // uniform
@@ -2130,14 +2171,30 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                         mask = *mslot;
                         add = (mask >> lane) & 1ull;
                     } else {
-                        if (cj_[r] >= 0) {
-                            const double dx = fabs(cx[r] - xi), dy = fabs(cy[r] - yi), dz = fabs(cz[r] - zi);
-                            if (shsup) {
-                                const double ax = dx / wx[r], ay = dy / wy[r], az = dz / wz[r];
-                                add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
-                            } else {   // the box test, exactly as the quotients decide it
-                                add = mega::quotient_below_one(dx, wx[r]) & mega::quotient_below_one(dy, wy[r]) &
-                                      mega::quotient_below_one(dz, wz[r]);
+                        if (!PRE) {
+                            if (cj_[r] >= 0) {
+                                const double dx = fabs(cx[r] - xi), dy = fabs(cy[r] - yi), dz = fabs(cz[r] - zi);
+                                if (SHSUP) {
+                                    const double ax = dx / wx[r], ay = dy / wy[r], az = dz / wz[r];
+                                    add = sqrt(ax * ax + ay * ay + az * az) < 1.0;
+                                } else {   // the box test, exactly as the quotients decide it
+                                    add = mega::quotient_below_one(dx, wx[r]) & mega::quotient_below_one(dy, wy[r]) &
+                                          mega::quotient_below_one(dz, wz[r]);
+                                }
+                            }
+                        } else {
+                            const int a0 = mega::box_axis(pf[r][0], q[0], lo[r][0], hi[r][0]);
+                            const int a1 = mega::box_axis(pf[r][1], q[1], lo[r][1], hi[r][1]);
+                            const int a2 = mega::box_axis(pf[r][2], q[2], lo[r][2], hi[r][2]);
+                            add = (a0 > 0) & (a1 > 0) & (a2 > 0);
+                            const bool rej = (a0 < 0) | (a1 < 0) | (a2 < 0);
+                            if (!add && !rej && cj_[r] >= 0) {   // the box test, exactly as the quotients decide it
+                                const int jj = cj_[r];
+                                const double *xj = x + 3 * (int64_t)jj, *wj = win + 3 * (int64_t)jj,
+                                             *xv = x + 3 * (int64_t)i;
+                                add = mega::quotient_below_one(fabs(xj[0] - xv[0]), wj[0]) &
+                                      mega::quotient_below_one(fabs(xj[1] - xv[1]), wj[1]) &
+                                      mega::quotient_below_one(fabs(xj[2] - xv[2]), wj[2]);
                             }
                         }
                         mask = __ballot(add);
@@ -2678,9 +2735,10 @@ int do_search(mega_ctx *c, const double *x)
     HIPCHK(hipMemsetAsync(c->tbin, 0, sizeof(int32_t) * 27 * (size_t)n, c->stream));
     {
         Scope s(c, "k_search_count");
-        k_search<false><<<search_grid(c), kBlock, 0, c->stream>>>(
-            c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, c->counts,
-            nullptr, nullptr, c->tbin, nullptr, nullptr, nullptr, c->smask);
+        auto kern = c->c.shsup ? k_search<false, true> : k_search<false, false>;
+        kern<<<search_grid(c), kBlock, 0, c->stream>>>(c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start,
+                                                      c->cell_nodes, c->counts, nullptr, nullptr, c->tbin, nullptr,
+                                                      nullptr, nullptr, c->smask);
     }
     HIPCHK(hipMemsetAsync(c->counts + n, 0, sizeof(int32_t), c->stream));
     int r = cub_scan(c, c->counts, c->row_ptr, n + 1);
@@ -2701,9 +2759,10 @@ int do_search(mega_ctx *c, const double *x)
     }
     {
         Scope s(c, "k_search_fill");
-        k_search<true><<<search_grid(c), kBlock, 0, c->stream>>>(
-            c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start, c->cell_nodes, c->c.shsup, nullptr,
-            c->row_ptr, c->col, c->tbin, c->tsrc, c->tpos, c->tq, c->smask);
+        auto kern = c->c.shsup ? k_search<true, true> : k_search<true, false>;
+        kern<<<search_grid(c), kBlock, 0, c->stream>>>(c->ncell_ne, c->cells_ne, x, c->win, c->geom, c->cell_start,
+                                                      c->cell_nodes, nullptr, c->row_ptr, c->col, c->tbin, c->tsrc,
+                                                      c->tpos, c->tq, c->smask);
     }
     if (c->tq) {
         Scope s(c, "k_tsort");

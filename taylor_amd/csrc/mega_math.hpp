@@ -49,6 +49,37 @@ MEGA_HD bool quotient_below_one(double a, double b)
     return fma(b, 0x1p-54, a - b) < 0.0;
 }
 
+// FP32 prefilter of the box test.  Positions are taken relative to a corner
+// point and kept only within `reach` of it (NaN otherwise, which decides
+// nothing); an axis then reads +1 when |dx| < w (1 - margin) for sure, -1
+// when |dx| > w (1 + margin) for sure, 0 when the exact test must decide.
+// With both points within 4 bins of the corner, the FP32 distance is within
+// 2^-24 x 16 bins of the FP64 one; for windows above 1/50 bin that is below
+// 5e-5 w, under the margin of 1e-4.
+constexpr float kBoxMargin = 1e-4f;
+MEGA_HD float box_rel(double x, double corner, float reach)
+{
+    const float p = (float)(x - corner);
+    return fabsf(p) < reach ? p : __builtin_nanf("");
+}
+// lo = hi = -1 for a window below wmin: every pair then goes to the exact test
+MEGA_HD void box_limits(double w, float wmin, float &lo, float &hi)
+{
+    const float wf = (float)w;
+    if (wf > wmin) {
+        lo = wf * (1.f - kBoxMargin);
+        hi = wf * (1.f + kBoxMargin);
+    } else {
+        lo = -1.f;
+        hi = __builtin_inff();
+    }
+}
+MEGA_HD int box_axis(float p, float q, float lo, float hi)
+{
+    const float d = fabsf(p - q);
+    return d < lo ? 1 : (d > hi ? -1 : 0);
+}
+
 // MLS_KERNEL0 cubic B-spline, value and d/d|z| (KERNEL.F90:57-71)
 template <class T>
 MEGA_HD void cubic_kernel(T x, T &phi, T &dphi)
