@@ -53,7 +53,10 @@ enum { MEGA_FP64 = 0, MEGA_MIXED = 1 };
 typedef struct mega_control {
     int32_t rk_degree;       /* RK_DEGREE, must be 1                          */
     int32_t rk_cont;         /* RK_CONT, must be 3 (cubic B-spline)           */
-    int32_t itype_int;       /* ITYPE_INT, must be 0 (direct nodal integration) */
+    int32_t itype_int;       /* ITYPE_INT: 0 direct nodal integration; 1 smoothed
+                                (stabilized non-conforming) gradients, Lagrangian RK
+                                only -- an extension: the reference's smoothing,
+                                FINT.F90:441-496, is commented out */
     int32_t ql;              /* QL switch                                      */
     double  ql_coef;         /* QL_COEF                                        */
     double  ql_len;          /* QL_LEN                                         */

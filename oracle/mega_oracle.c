@@ -441,6 +441,30 @@ int or_constitute_point(const double *lmat, double *stress, double *strain, doub
     return it;
 }
 
+void or_snni_grad(const double *x, const double *win_x, double vol_x, int n, const double *coo, const double *win,
+                  const int32_t *ebc_node, const int32_t *lstack, int ln, const or_control *c, double *shpd)
+{
+    const double thrd = 0.333333333333333;   /* THRD of PRE_MODEL_SMOOTHING_LENGTHS.F90:41 */
+    const double d1 = win_x[0] * 2.0, d2 = win_x[1] * 2.0, d3 = win_x[2] * 2.0;
+    const double ratio = pow(vol_x, thrd) / pow(d1 * d2 * d3, thrd);
+    const double l[3] = {d1 * ratio / 2.0, d2 * ratio / 2.0, d3 * ratio / 2.0};
+    const double svol = (l[0] + l[0]) * (l[1] + l[1]) * (l[2] + l[2]);
+    const double aov[3] = {(l[1] + l[1]) * (l[2] + l[2]) / svol, (l[0] + l[0]) * (l[2] + l[2]) / svol,
+                           (l[0] + l[0]) * (l[1] + l[1]) / svol};
+    double *sp = (double *)malloc(sizeof(double) * (size_t)(ln + 1));
+    double *sm = (double *)malloc(sizeof(double) * (size_t)(ln + 1));
+    double *dd = (double *)malloc(sizeof(double) * 3 * (size_t)(ln + 1));
+    for (int k = 0; k < 3; ++k) {
+        double pp[3] = {x[0], x[1], x[2]}, pm[3] = {x[0], x[1], x[2]};
+        pp[k] = x[k] + l[k];
+        pm[k] = x[k] - l[k];
+        or_rk1(pp, n, coo, win, ebc_node, lstack, ln, c, sp, dd);
+        or_rk1(pm, n, coo, win, ebc_node, lstack, ln, c, sm, dd);
+        for (int J = 0; J < ln; ++J) shpd[3 * J + k] = (sp[J] - sm[J]) * aov[k];
+    }
+    free(sp); free(sm); free(dd);
+}
+
 /* RK1: linear-basis reproducing-kernel shape functions and gradients at x
  * (RK.f90:16-453 with FILL_H :462, DERIV_H :506, MLS_KERNEL0 KERNEL.F90:3). */
 void or_rk1(const double *x, int n, const double *coo, const double *win, const int32_t *ebc_node,
@@ -707,9 +731,12 @@ int or_construct_fint(or_fint_io *io, const or_control *c)
                 }
                 push = c->finite_strain;
             } else {
-                if (c->lagrangian)
+                if (c->lagrangian) {
                     or_rk1(io->coo0 + 3 * i, n, io->coo0, io->win, io->ebc_node, lst, ln, c, shp, shpd);
-                else
+                    if (c->itype_int == 1)
+                        or_snni_grad(io->coo0 + 3 * i, io->win + 3 * i, io->vol[i], n, io->coo0, io->win,
+                                     io->ebc_node, lst, ln, c, shpd);
+                } else
                     or_rk1(io->coo_cur + 3 * i, n, io->coo_cur, io->win, io->ebc_node, lst, ln, c, shp, shpd);
                 for (int J = 0; J < ln; ++J) {
                     io->shp[p0 + J] = shp[J];

@@ -31,6 +31,8 @@ typedef struct or_control {
     int finite_strain;  /* LFINITE_STRAIN                        */
     double gravity[3];  /* IGRAVITY                              */
     int peridynamics;   /* PERIDYNAMICS: CONSTRUCT_FINT_PD        */
+    int itype_int;      /* ITYPE_INT: 0 direct nodal integration; 1 the smoothed
+                           gradients (Lagrangian RK; or_snni_grad)  */
 } or_control;
 
 /* ---- small dense math (INVERSE.F90, DETERMINANT.F90, KERNEL.F90) ---- */
@@ -58,6 +60,15 @@ int or_constitute_point(const double *lmat, double *stress, double *strain, doub
 /* ---- RK shape functions (RK.f90:16-453) ---- */
 void or_rk1(const double *x, int n, const double *coo, const double *win, const int32_t *ebc_node,
             const int32_t *lstack, int ln, const or_control *c, double *shp, double *shpd);
+
+/* Smoothed gradients at node x (ITYPE_INT = 1): the smoothing box of
+ * GET_SMOOTHING_LENGTHS (PRE_MODEL_SMOOTHING_LENGTHS.F90:49-90) from the
+ * node's window win_x[3] and volume vol_x, and
+ *   shpd[3J+k] = (phi_J(x + l_k e_k) - phi_J(x - l_k e_k)) A_k / V
+ * with RK1 at the face centres over the node's list (the smoothing of
+ * FINT.F90:441-496, commented out in the reference). */
+void or_snni_grad(const double *x, const double *win_x, double vol_x, int n, const double *coo, const double *win,
+                  const int32_t *ebc_node, const int32_t *lstack, int ln, const or_control *c, double *shpd);
 
 /* ---- bins and neighbor search (HANDELER.F90:187-252, BIN_SEARCH.F90) ---- */
 void or_bin_geometry(int n, const double *coo, const double *win, double xd, double yd, double zd,

@@ -584,6 +584,104 @@ __device__ __forceinline__ void rk_moments(double *m, const double *H, const Ker
         }
 }
 
+// Stabilized non-conforming nodal integration (ITYPE_INT = 1, Lagrangian): the
+// gradient at node L is replaced by its divergence over L's smoothing box,
+//   grad~ phi_J(x_L) = sum_k e_k (phi_J(x_L + l_k e_k) - phi_J(x_L - l_k e_k)) A_k / V,
+// with RK1 evaluated at the six face centres over L's own neighbor list, as
+// the reference's smoothing block does (FINT.F90:441-496, commented out
+// there; SM_LEN, SM_AREA, SM_VOL of PRE_MODEL_SMOOTHING_LENGTHS.F90:49-90).
+// The values keep linear fields exact (each face value reproduces x), so the
+// smoothed gradient does too.  Run once, after k_rk has written the node's own
+// phi and the lists' transposed slots: it overwrites the list-order and
+// transposed gradients.  FP64 throughout, with the reference's divisions.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_snni(Params p, const double *xs, const double *sml, const double *aov,
+                                                 int nblocks)
+{
+    constexpr int G = kRkGroup;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int node = blk * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (node >= p.n) return;   // whole groups leave together
+    const int p0 = p.row_ptr[node], p1 = p.row_ptr[node + 1];
+    const double xi[3] = {xs[3 * (int64_t)node], xs[3 * (int64_t)node + 1], xs[3 * (int64_t)node + 2]};
+    R *dshp = as<R>(p.dshp);
+    TPair<R> *tpair = as<TPair<R>>(p.tpair);
+    // quasi-linear augmentation: its sums over the six points x +- L e_k do not depend on x
+    double ms[10] = {0}, hs[4] = {0, 0, 0, 0};
+    if (p.ql) {
+        const double L = p.ql_len;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            double d[3] = {0.0, 0.0, 0.0};
+            d[s / 2] = (s & 1) ? L : -L;   // x - q_s
+            const double H[4] = {1.0, d[0], d[1], d[2]};
+            int t = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = r; c < 4; ++c, ++t) ms[t] += H[r] * H[c];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) hs[k] += H[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double l = sml[3 * (int64_t)node + k], a = aov[3 * (int64_t)node + k];
+        double B[2][4];   // e1^T M^-1 at x_L + l e_k and x_L - l e_k
+        for (int sg = 0; sg < 2; ++sg) {
+            double pt[3] = {xi[0], xi[1], xi[2]};
+            pt[k] = sg ? xi[k] - l : xi[k] + l;
+            double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = p0 + lane; q < p1; q += G) {
+                const int j = p.col[q];
+                const KernelEval e = rk_window<false>(p, pt, xs, j);
+                const double H[4] = {1.0, pt[0] - xs[3 * (int64_t)j], pt[1] - xs[3 * (int64_t)j + 1],
+                                     pt[2] - xs[3 * (int64_t)j + 2]};
+                int t = 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = r; c < 4; ++c, ++t) m[t] += H[r] * H[c] * e.phi;
+            }
+#pragma unroll
+            for (int t = 0; t < 10; ++t) m[t] = group_sum<G>(m[t]);
+            if (p.ql) {
+                const double sphi = m[0];
+#pragma unroll
+                for (int t = 0; t < 10; ++t) m[t] = m[t] + ms[t] * sphi * p.ql_coef;
+            }
+            double Mi[16];
+            mega::m44inv_sym(m, Mi);
+            B[sg][0] = Mi[0]; B[sg][1] = Mi[4]; B[sg][2] = Mi[8]; B[sg][3] = Mi[12];
+        }
+        for (int q = p0 + lane; q < p1; q += G) {
+            const int j = p.col[q];
+            double f[2];
+#pragma unroll
+            for (int sg = 0; sg < 2; ++sg) {
+                double pt[3] = {xi[0], xi[1], xi[2]};
+                pt[k] = sg ? xi[k] - l : xi[k] + l;
+                const KernelEval e = rk_window<false>(p, pt, xs, j);
+                double H[4] = {1.0, pt[0] - xs[3 * (int64_t)j], pt[1] - xs[3 * (int64_t)j + 1],
+                               pt[2] - xs[3 * (int64_t)j + 2]};
+                if (p.ql)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) H[c] = H[c] + hs[c] * p.ql_coef;
+                f[sg] = (B[sg][0] * H[0] + B[sg][1] * H[1] + B[sg][2] * H[2] + B[sg][3] * H[3]) * e.phi;
+            }
+            const R gk = (R)((f[0] - f[1]) * a);
+            dshp[(int64_t)k * p.cap + q] = gk;
+            if (p.tpos) {
+                TPair<R> *t = tpair + p.tpos[q];
+                if (k == 0) t->gx = gk;
+                else if (k == 1) t->gy = gk;
+                else t->gz = gk;
+            }
+        }
+    }
+}
+
 // Nodes are taken in bin order (order = the bins' node lists), blocks mapped to
 // XCDs in contiguous ranges, kRkGroup lanes per node.  A lane's first kRkCache
 // neighbors are loaded together, and their kernel values and H are kept in
@@ -2529,6 +2627,9 @@ struct mega_ctx {
     double dt = 0.0;
     // automatic time step (peridynamics): this step's DLT, each step's DLT of the
     // call, the characteristic distances and the running minimum
+    // smoothed gradients (ITYPE_INT = 1): per node the smoothing box's half lengths and A_k / V
+    int snni = 0;
+    double *sml = nullptr, *aov = nullptr;
     int auto_on = 0, auto_step = 0;
     double dlt_fac = 0.0;
     double *dt_dev = nullptr, *dt_hist = nullptr, *char_dist = nullptr;
@@ -2946,6 +3047,14 @@ int do_shapes(mega_ctx *c, const double *xs, bool gather)
         }
         HIPCHK(hipGetLastError());
     }
+    if (c->snni && !gather) {
+        Scope s(c, "k_snni");
+        const int nb = (int)(((int64_t)c->n * kRkGroup + kBlock - 1) / kBlock);
+        const unsigned grid = (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
+        if (c->mixed) k_snni<float><<<grid, kBlock, 0, c->stream>>>(p, xs, c->sml, c->aov, nb);
+        else k_snni<double><<<grid, kBlock, 0, c->stream>>>(p, xs, c->sml, c->aov, nb);
+        HIPCHK(hipGetLastError());
+    }
     const int r = build_sets(c);
     if (r) return r;
     c->has_shapes = 1;
@@ -3266,7 +3375,11 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
 {
     if (!ctrl || !out || n <= 0 || !X0 || !win || !vol || !mass || !ebc || !ebc_node || !mat_type || !prop)
         return MEGA_ERR_ARG;
-    if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3 || ctrl->itype_int != 0) return MEGA_ERR_ARG;
+    if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3) return MEGA_ERR_ARG;
+    // ITYPE_INT: 0 direct nodal integration; 1 the smoothed (stabilized non-conforming)
+    // gradients, an extension on Lagrangian RK steps (the reference's is commented out)
+    if (ctrl->itype_int != 0 && !(ctrl->itype_int == 1 && ctrl->lagrangian && !ctrl->peridynamics))
+        return MEGA_ERR_ARG;
     if (ctrl->precision != MEGA_FP64 && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     if (ctrl->surrogate && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     // the reference's CONSTRUCT_FINT_PD is Lagrangian only (FINT_PD.F90:408-412)
@@ -3333,6 +3446,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     // wave speeds: FP64, and peridynamics in both precisions (the automatic time step)
     if (!c->mixed || c->pd) e |= dalloc(&c->wave_vel, n);
     e |= dalloc(&c->Fst, 10 * (size_t)n);
+    c->snni = ctrl->itype_int == 1;
+    if (c->snni) { e |= dalloc(&c->sml, n3); e |= dalloc(&c->aov, n3); }
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
     e |= dalloc(&c->win, n3); e |= dalloc(&c->winv, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
@@ -3386,6 +3501,21 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     if (ok && c->fext_nmo) ok = zero(c->fext_nmo, n3 * 8);
     if (ok && c->mixed) ok = zero(c->d4, (size_t)n * 16) && zero(c->t4, (size_t)n * 16);
     if (ok && c->wave_vel) ok = zero(c->wave_vel, (size_t)n * 8);
+    if (ok && c->snni) {   // GET_SMOOTHING_LENGTHS (PRE_MODEL_SMOOTHING_LENGTHS.F90:49-90)
+        const double thrd = 0.333333333333333;   // the reference's THRD
+        std::vector<double> l(n3), ao(n3);
+        for (size_t i = 0; i < (size_t)n; ++i) {
+            const double d1 = wini[3 * i] * 2.0, d2 = wini[3 * i + 1] * 2.0, d3 = wini[3 * i + 2] * 2.0;
+            const double ratio = std::pow(voli[i], thrd) / std::pow(d1 * d2 * d3, thrd);
+            const double s1 = d1 * ratio / 2.0, s2 = d2 * ratio / 2.0, s3 = d3 * ratio / 2.0;
+            const double svol = (s1 + s1) * (s2 + s2) * (s3 + s3);
+            l[3 * i] = s1; l[3 * i + 1] = s2; l[3 * i + 2] = s3;
+            ao[3 * i] = (s2 + s2) * (s3 + s3) / svol;
+            ao[3 * i + 1] = (s1 + s1) * (s3 + s3) / svol;
+            ao[3 * i + 2] = (s1 + s1) * (s2 + s2) / svol;
+        }
+        ok = up(c->sml, l.data(), n3 * 8) && up(c->aov, ao.data(), n3 * 8);
+    }
     if (!ok) { mega_destroy(c); return MEGA_ERR_HIP; }
     *out = c;
     return MEGA_OK;
@@ -3403,7 +3533,7 @@ int mega_destroy(mega_ctx *c)
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
                     c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
                     c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair, c->fpair2, c->dt_dev,
-                    c->dt_hist, c->char_dist, c->dt_min};
+                    c->dt_hist, c->char_dist, c->dt_min, c->sml, c->aov};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }

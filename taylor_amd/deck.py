@@ -310,6 +310,7 @@ class Problem:
     gravity: np.ndarray = field(default_factory=lambda: np.zeros(3))
     elcon: np.ndarray | None = None
     peridynamics: int = 0    # CONSTRUCT_FINT_PD instead of CONSTRUCT_FINT (*PERIDYNAMICS 1)
+    itype_int: int = 0       # *INTEGRATION PARAMETERS: 0 direct nodal integration, 1 smoothed gradients
 
     @property
     def n(self) -> int:
@@ -326,7 +327,7 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
                    radius: float = 3.8e-3, length: float = 25.4e-3, time_end: float = 80e-6,
                    dt_frac: float = 0.3, lagrangian: int = 1, pdsearch_steps: int = 20,
                    material: Material | None = None, ql: int = 1, ql_coef: float = 0.001,
-                   peridynamics: int = 0, mm_units: bool = False) -> Problem:
+                   peridynamics: int = 0, mm_units: bool = False, itype_int: int = 0) -> Problem:
     """Taylor bar hitting a fixed plane: BAR gets v_z = -v0, WALL (impact face)
     gets fixity 0 0 1, dt = dt_frac * h / c_p.  peridynamics=1 selects the
     correspondence peridynamic force (Lagrangian only in the reference).
@@ -334,7 +335,8 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
     mm / tonne / s / N / MPa system instead (lengths x1e3, stresses x1e-6,
     density x1e-12), which keeps the peridynamic shape tensor O(1): in SI at
     sub-mm spacing it is ~1e-16 and the reference's INVERSE treats it as
-    singular (INVERSE.F90:41-45)."""
+    singular (INVERSE.F90:41-45).  itype_int=1 selects the smoothed
+    (stabilized non-conforming) gradients, Lagrangian only."""
     if mm_units:
         h, radius, length, v0 = h * 1e3, radius * 1e3, length * 1e3, v0 * 1e3
         base = material or copper_linear()
@@ -348,6 +350,7 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
                     norm_win=norm_win, dt=dt_frac * h / p_wave_speed(mat), time_end=time_end,
                     lagrangian=lagrangian, pdsearch=(1 if lagrangian else pdsearch_steps), ql=ql, ql_coef=ql_coef)
     prob.peridynamics = int(peridynamics)
+    prob.itype_int = int(itype_int)
     return prob
 
 
@@ -505,7 +508,7 @@ def write_control(path: str, kfile: str, prob: Problem, set_names: list, time_ou
            "*PERIDYNAMICS", "# SWITCH", "  %d" % int(prob.peridynamics), "",
            "*HPC PARAMETERS", "#HPC_SCHEME", "1", "",
            "*GHOSTING PARAMETERS", "#GHOST_BUFFER", "0", "",
-           "*INTEGRATION PARAMETERS", "#TYPE", "0", "",
+           "*INTEGRATION PARAMETERS", "#TYPE", str(int(prob.itype_int)), "",
            "*FORMULATION PARAMETERS", "# IFINITE_STRAIN, ILAGRANGIAN", "   %d   %d" % (prob.finite_strain, prob.lagrangian), ""]
     if not prob.lagrangian and prob.pdsearch > 1:
         txt += ["*PERIODIC SEARCH", "# TIME", "  %.17e" % (prob.pdsearch * prob.dt * (1 - 1e-9)), ""]

@@ -135,7 +135,7 @@ def _i32(a):
 
 def make_control(prob, precision: int = MEGA_FP64, surrogate: int = 0) -> MegaControl:
     c = MegaControl()
-    c.rk_degree, c.rk_cont, c.itype_int = 1, 3, 0
+    c.rk_degree, c.rk_cont, c.itype_int = 1, 3, int(getattr(prob, "itype_int", 0))
     c.ql, c.ql_coef, c.ql_len = int(prob.ql), float(prob.ql_coef), float(prob.ql_len)
     c.shsup, c.lagrangian, c.finite_strain = int(prob.shsup), int(prob.lagrangian), int(prob.finite_strain)
     c.pdsearch = int(prob.pdsearch)

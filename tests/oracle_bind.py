@@ -21,7 +21,8 @@ lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 class OrControl(C.Structure):
     _fields_ = [("rk_degree", C.c_int), ("rk_cont", C.c_int), ("ql", C.c_int), ("ql_coef", C.c_double),
                 ("ql_len", C.c_double), ("shsup", C.c_int), ("lagrangian", C.c_int),
-                ("finite_strain", C.c_int), ("gravity", C.c_double * 3), ("peridynamics", C.c_int)]
+                ("finite_strain", C.c_int), ("gravity", C.c_double * 3), ("peridynamics", C.c_int),
+                ("itype_int", C.c_int)]
 
 
 class OrFintIO(C.Structure):
@@ -89,6 +90,7 @@ def load_oracle():
     lib.or_interp.argtypes = [C.c_int, lp, ip, dp, dp, dp]
     lib.or_udfm.argtypes = [dp, C.c_int, dp, dp, dp, ip, C.c_int, dp, dp]
     lib.or_dfm.argtypes = [dp, dp, dp, dp, ip, C.c_int, dp, dp, dp]
+    lib.or_snni_grad.argtypes = [dp, dp, C.c_double, C.c_int, dp, dp, ip, ip, C.c_int, C.POINTER(OrControl), dp]
     lib.or_construct_fint_pd.argtypes = [C.POINTER(OrFintIO), C.POINTER(OrControl)]
     lib.or_construct_fint_pd.restype = C.c_int
     lib.or_model_init.argtypes = [C.POINTER(OrModel)]
@@ -141,6 +143,7 @@ def control(prob) -> OrControl:
     for k in range(3):
         c.gravity[k] = float(prob.gravity[k])
     c.peridynamics = int(getattr(prob, "peridynamics", 0))
+    c.itype_int = int(getattr(prob, "itype_int", 0))
     return c
 
 

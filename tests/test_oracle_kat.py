@@ -178,6 +178,49 @@ def test_rk_partition_of_unity_with_ql_and_singular_kernel(oracle):
     assert np.abs(sums - 1).max() < 1e-10
 
 
+def snni_grads(oracle, prob, rp, col):
+    c = control(prob)
+    coo, win = np.ascontiguousarray(prob.coo), np.ascontiguousarray(prob.win)
+    ebc = np.ascontiguousarray(prob.ebc_node, dtype=np.int32)
+    out = []
+    for i in range(prob.n):
+        lst = np.ascontiguousarray(col[rp[i]:rp[i + 1]])
+        g = np.zeros(3 * len(lst))
+        oracle.or_snni_grad(coo[i].copy(), win[i].copy(), float(prob.vol[i]), prob.n, coo, win, ebc, lst, len(lst),
+                            C.byref(c), g)
+        out.append(g.reshape(-1, 3))
+    return out
+
+
+def test_snni_gradients_reproduce_linear_fields(oracle):
+    """Smoothed gradients (ITYPE_INT = 1): with QL off every face value
+    reproduces constants and linear fields, so sum grad~ phi = 0 and
+    sum grad~ phi (x) x = I exactly (EBC nodes and their singular kernels
+    included: the face points are off the nodes).  A wrong A/V (1/l instead of
+    1/(2l)), sign or axis breaks the second identity by O(1)."""
+    prob = deck.taylor_problem(0.7e-3, length=6e-3, seed=0, ql=0, itype_int=1)
+    rp, col, _ = oracle_search(oracle, prob)
+    x = prob.coo
+    for i, g in enumerate(snni_grads(oracle, prob, rp, col)):
+        xj = x[col[rp[i]:rp[i + 1]]]
+        assert np.abs(g.sum(0)).max() * 0.7e-3 < 1e-12
+        assert np.abs(g.T @ xj - np.eye(3)).max() < 1e-12
+
+
+def test_snni_gradients_with_ql_stay_close_to_linear(oracle):
+    """With the reference's QL augmentation on (default), RK1 keeps the
+    partition of unity but only approximately the linear conditions: the
+    node-point gradients of the small bar miss sum grad phi (x) x = I by up to
+    7 % at interior nodes (100 % at the singular-kernel nodes); the smoothed
+    gradients, sampled off the nodes, by at most 4 %."""
+    prob = deck.taylor_problem(0.7e-3, length=6e-3, seed=0, itype_int=1)
+    rp, col, _ = oracle_search(oracle, prob)
+    x = prob.coo
+    worst = max(float(np.abs(g.T @ x[col[rp[i]:rp[i + 1]]] - np.eye(3)).max())
+                for i, g in enumerate(snni_grads(oracle, prob, rp, col)))
+    assert worst < 0.04
+
+
 def test_force_balance_without_ql(oracle):
     """With sum grad phi = 0 the assembled internal forces sum to zero
     (momentum conservation of the nodal integration)."""
