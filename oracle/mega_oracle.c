@@ -736,8 +736,12 @@ int or_construct_fint(or_fint_io *io, const or_control *c)
                     if (c->itype_int == 1)
                         or_snni_grad(io->coo0 + 3 * i, io->win + 3 * i, io->vol[i], n, io->coo0, io->win,
                                      io->ebc_node, lst, ln, c, shpd);
-                } else
+                } else {
                     or_rk1(io->coo_cur + 3 * i, n, io->coo_cur, io->win, io->ebc_node, lst, ln, c, shp, shpd);
+                    if (c->itype_int == 1)
+                        or_snni_grad(io->coo_cur + 3 * i, io->win + 3 * i, io->vol[i], n, io->coo_cur, io->win,
+                                     io->ebc_node, lst, ln, c, shpd);
+                }
                 for (int J = 0; J < ln; ++J) {
                     io->shp[p0 + J] = shp[J];
                     for (int k = 0; k < 3; ++k) io->dshp[3 * (p0 + J) + k] = shpd[3 * J + k];

@@ -32,7 +32,7 @@ typedef struct or_control {
     double gravity[3];  /* IGRAVITY                              */
     int peridynamics;   /* PERIDYNAMICS: CONSTRUCT_FINT_PD        */
     int itype_int;      /* ITYPE_INT: 0 direct nodal integration; 1 the smoothed
-                           gradients (Lagrangian RK; or_snni_grad)  */
+                           gradients (or_snni_grad)                 */
 } or_control;
 
 /* ---- small dense math (INVERSE.F90, DETERMINANT.F90, KERNEL.F90) ---- */

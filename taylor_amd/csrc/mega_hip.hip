@@ -3377,9 +3377,8 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
         return MEGA_ERR_ARG;
     if (ctrl->rk_degree != 1 || ctrl->rk_cont != 3) return MEGA_ERR_ARG;
     // ITYPE_INT: 0 direct nodal integration; 1 the smoothed (stabilized non-conforming)
-    // gradients, an extension on Lagrangian RK steps (the reference's is commented out)
-    if (ctrl->itype_int != 0 && !(ctrl->itype_int == 1 && ctrl->lagrangian && !ctrl->peridynamics))
-        return MEGA_ERR_ARG;
+    // gradients, an extension on RK steps (the reference's is commented out)
+    if (ctrl->itype_int != 0 && !(ctrl->itype_int == 1 && !ctrl->peridynamics)) return MEGA_ERR_ARG;
     if (ctrl->precision != MEGA_FP64 && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     if (ctrl->surrogate && ctrl->precision != MEGA_MIXED) return MEGA_ERR_ARG;
     // the reference's CONSTRUCT_FINT_PD is Lagrangian only (FINT_PD.F90:408-412)
@@ -3869,8 +3868,13 @@ static int run_steps(mega_ctx *c, int32_t nsteps)
                 else k_interp<double, true><<<grid_for((int64_t)n * kGroup), kBlock, 0, c->stream>>>(p, nullptr, nullptr);
             }
             r = search_step(c, c->x, first);
-            if (!r) r = do_shapes(c, c->x, true);   // with the gather sums
-            if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2, true);
+            if (c->snni) {   // the smoothed gradients replace the node-point ones before the gather
+                if (!r) r = do_shapes(c, c->x, false);
+                if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2, false);
+            } else {
+                if (!r) r = do_shapes(c, c->x, true);   // with the gather sums
+                if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2, true);
+            }
         }
         if (r) return r;
         c->predicted = last ? 0 : 1;

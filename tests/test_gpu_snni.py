@@ -36,10 +36,11 @@ def test_snni_gradients_match_oracle(oracle, precision):
     assert e < (1e-12 if precision == "fp64" else 2e-7)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_snni_steps_match_oracle(oracle, precision):
-    """40 Lagrangian steps with the smoothed gradients against the oracle's run."""
-    prob = deck.taylor_problem(H, length=LEN, seed=0, itype_int=1)
+@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
+def test_snni_steps_match_oracle(oracle, lagrangian, precision):
+    """40 steps with the smoothed gradients against the oracle's run, Lagrangian
+    and semi-Lagrangian (bins every 3 steps)."""
+    prob = deck.taylor_problem(H, length=LEN, seed=0, itype_int=1, lagrangian=lagrangian, pdsearch_steps=3)
     g = mega.Mega(prob, precision=mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
     g.step(40)
     run = OracleRun(oracle, prob).step(40)
@@ -48,7 +49,7 @@ def test_snni_steps_match_oracle(oracle, precision):
             "fint": rel_err(g.download("fint"), A["fint"]), "stress": rel_err(g.download("stress"), A["stress"]),
             "eps": rel_err(g.download("eps"), A["state"][:, 2])}
     g.close()
-    print(f"{precision}, 40 steps with smoothed gradients vs oracle: {errs}", flush=True)
+    print(f"L{lagrangian} {precision}, 40 steps with smoothed gradients vs oracle: {errs}", flush=True)
     if precision == "fp64":
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
     else:

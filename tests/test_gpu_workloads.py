@@ -21,6 +21,9 @@
   radius and the volume-weighted L2 norm of the plastic strain are compared;
   the length must be within 0.5 % (the benchmark's gate) and within the
   tighter bars below, about ten times the errors measured.
+* The same bar at 220 m/s with the smoothed gradients (ITYPE_INT = 1, an
+  extension): FP64, mixed and the learned return map in lockstep over a
+  window of ~3,750 steps (21.6 us) instead of ~850.
 """
 import os
 from collections import deque
@@ -193,3 +196,37 @@ def test_175M_surrogate_vs_fp64_at_220(lib, bar175):
     for k, bar in SURROGATE_BARS.items():
         assert e[k] < bar, (k, e[k])
     assert sstat["evaluated"] > 0 and answered > 0.2
+
+
+# ---------------------------------------------------------------------------
+# 1.75M nodes with the smoothed gradients (ITYPE_INT = 1, an extension): a
+# finite window four to six times longer than direct nodal integration's
+
+@pytest.fixture(scope="module")
+def bar175_snni():
+    prob = deck.taylor_problem(0.088e-3, seed=2, v0=220.0, itype_int=1)
+    return prob
+
+
+def test_175M_smoothed_gradients_mixed_and_surrogate_vs_fp64(lib, bar175_snni):
+    prob = bar175_snni
+    v0 = 220.0
+    step, snap, reason, sstat = lockstep(prob, v0, ("fp64", "mixed", "surrogate"), chunk=250, max_steps=8000)
+    out = {m: compare(prob, snap, m) for m in ("mixed", "surrogate")}
+    answered = 1.0 - sstat["rejected"] / max(1, sstat["evaluated"])
+    for m, e in out.items():
+        print(f"1.75M smoothed gradients, {m} vs FP64 at {v0:.0f} m/s: window {step} steps "
+              f"({step * prob.dt * 1e6:.2f} us; end: {reason}); length {e['length']:.2e} radius {e['radius']:.2e} "
+              f"eqps L2 {e['eqps_l2']:.2e} (length {e['length_fp64_m'] * 1e3:.4f} mm, radius "
+              f"{e['radius_fp64_m'] * 1e3:.4f} mm)", flush=True)
+    print(f"network answered {answered:.1%} of {sstat['plastic']} plastic return maps", flush=True)
+    assert step >= 3000
+    # bars about ten times the errors measured over the 3,750-step window (21.6 us):
+    # mixed length 3.2e-9, radius 1.4e-8, eqps L2 2.0e-4; learned return map 3.2e-5,
+    # 1.5e-4, 2.4e-3
+    bars = {"mixed": {"length": 3e-8, "radius": 1.5e-7, "eqps_l2": 2e-3},
+            "surrogate": {"length": 3e-4, "radius": 1.5e-3, "eqps_l2": 2.5e-2}}
+    for m, e in out.items():
+        assert e["length"] <= 0.005 and e["radius"] <= 0.005
+        for k, bar in bars[m].items():
+            assert e[k] < bar, (m, k, e[k])
