@@ -245,7 +245,7 @@ def read_traffic(kernel: str, mixed: bool):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_dispatch"):
+        if line.startswith("traffic_bytes_per_dispatch"):
             return float(line.split()[1])
     return None
 
