@@ -193,6 +193,8 @@ struct Params {
     const float *surr_bias, *surr_norm;
     int surr_mat;                                    // material slot it was trained for
     float surr_tol;                                  // yield residual / sigma_y accepted
+    float surr_env[2 * mlp::kIn];                    // training envelope: per-input min, then max
+                                                     // (kernel arguments: scalar loads)
     unsigned long long *sstat;                       // per k_node block: [plastic, outside envelope,
                                                      // evaluated, rejected], summed on the host
     // tuple harvest (k_node): 20 floats per plastic point
@@ -1053,12 +1055,13 @@ __device__ __forceinline__ int lanes_below(uint64_t m)
 }
 
 // the packed weights (staged once per block that has a candidate) and, per
-// wave, the normalised inputs of its candidate lanes, then the outputs
+// wave, the normalised inputs of its candidate lanes, then the outputs,
+// feature-major ([feature][slot]: consecutive slots on consecutive banks)
 struct SurrLds {
     mlp::bf16x8 tiles[mlp::kTilesFwd * 64];
     float bias[mlp::kBiasPack];
     float norm[mlp::kNorm];
-    float stage[kBlock / 64][64][16];
+    float stage[kBlock / 64][16][64];
 };
 
 // cand: this lane's J2 trial state is plastic and belongs to the trained
@@ -1082,7 +1085,7 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
     if (cand)
 #pragma unroll
         for (int k = 0; k < mlp::kIn; ++k)
-            inenv = inenv && x[k] >= p.surr_norm[mlp::kEnvLo + k] && x[k] <= p.surr_norm[mlp::kEnvHi + k];
+            inenv = inenv && x[k] >= p.surr_env[k] && x[k] <= p.surr_env[mlp::kIn + k];
     const uint64_t mc = __ballot(cand), mu = __ballot(inenv);
     if (lane == 0) {
         cnt[0] = (unsigned)__popcll(mc);
@@ -1102,26 +1105,26 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
     const float *norm = L.norm;
     const int nc = __popcll(mu);
     const int slot = lanes_below(mu);
-    float(*st)[16] = L.stage[wv];
+    float(*st)[64] = L.stage[wv];
     if (inenv) {
 #pragma unroll
-        for (int k = 0; k < mlp::kIn; ++k) st[slot][k] = (x[k] - norm[mlp::kInMean + k]) * norm[mlp::kInScale + k];
-        st[slot][13] = st[slot][14] = st[slot][15] = 0.0f;
+        for (int k = 0; k < mlp::kIn; ++k) st[k][slot] = (x[k] - norm[mlp::kInMean + k]) * norm[mlp::kInScale + k];
+        st[13][slot] = st[14][slot] = st[15][slot] = 0.0f;
     }
     wave_sync();
     const int g = lane >> 4;
     for (int c0 = 0; c0 < nc; c0 += 16) {
         const int s = c0 + (lane & 15);
         mlp::bf16x8 x0 = {};
-        if (g < 2 && s < nc) {
-            const float4 a = *reinterpret_cast<const float4 *>(&st[s][8 * g]);
-            const float4 b = *reinterpret_cast<const float4 *>(&st[s][8 * g + 4]);
-            x0 = mlp::pack8(mlp::f32x4{a.x, a.y, a.z, a.w}, mlp::f32x4{b.x, b.y, b.z, b.w});
-        }
+        if (g < 2 && s < nc)
+            x0 = mlp::pack8(mlp::f32x4{st[8 * g][s], st[8 * g + 1][s], st[8 * g + 2][s], st[8 * g + 3][s]},
+                            mlp::f32x4{st[8 * g + 4][s], st[8 * g + 5][s], st[8 * g + 6][s], st[8 * g + 7][s]});
         mlp::f32x4 z1[4], z2[4], z3[4], y;
         mlp::forward16(L.tiles, L.bias, x0, lane, z1, z2, z3, y);
         wave_sync();
-        if (g < 2 && s < nc) *reinterpret_cast<float4 *>(&st[s][4 * g]) = float4{y[0], y[1], y[2], y[3]};
+        if (g < 2 && s < nc)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) st[4 * g + t][s] = y[t];
     }
     wave_sync();
     bool ok = false;
@@ -1129,8 +1132,8 @@ __device__ bool surr_return(const Params &p, bool cand, const float *der, const 
         const float *os = norm + mlp::kOutScale;
         float s[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s[k] = pred[k] - os[k] * st[slot][k];
-        const float e = ep + os[6] * st[slot][6];
+        for (int k = 0; k < 6; ++k) s[k] = pred[k] - os[k] * st[k][slot];
+        const float e = ep + os[6] * st[6][slot];
         ok = e >= ep && mega::return_residual(pred, s, ep, e, mt) <= p.surr_tol;
         if (ok) {
 #pragma unroll
@@ -2640,6 +2643,7 @@ struct mega_ctx {
     // learned return map
     int surr_on = 0, surr_mat = -1;
     float surr_tol = 1e-3f;
+    float surr_env[2 * mlp::kIn] = {};   // envelope min / max per input (from the normalisation block)
     uint16_t *surr_tiles = nullptr;
     float *surr_bias = nullptr, *surr_norm = nullptr;
     unsigned long long *sstat = nullptr;
@@ -2709,6 +2713,7 @@ Params params(mega_ctx *c)
     p.char_dist = c->auto_on ? c->char_dist : nullptr;
     p.surr_tiles = c->surr_tiles; p.surr_bias = c->surr_bias; p.surr_norm = c->surr_norm;
     p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
+    for (int k = 0; k < 2 * mlp::kIn; ++k) p.surr_env[k] = c->surr_env[k];
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
     p.pd = c->pd; p.pd_linit = 0; p.Kinv = c->Kinv; p.gvec = c->gvec;
     return p;
@@ -4121,6 +4126,10 @@ int mega_set_surrogate(mega_ctx *c, const float *params, const float *norm, cons
     HIPCHK(hipMemcpyAsync(c->surr_bias, bias.data(), bias.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->surr_norm, norm, mlp::kNorm * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < mlp::kIn; ++k) {
+        c->surr_env[k] = norm[mlp::kEnvLo + k];
+        c->surr_env[mlp::kIn + k] = norm[mlp::kEnvHi + k];
+    }
     c->surr_mat = slot;
     c->surr_tol = (float)tol;
     c->surr_on = 1;
