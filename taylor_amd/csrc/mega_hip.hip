@@ -171,6 +171,10 @@ struct Params {
     const int32_t *gset_ptr, *gset, *fset_ptr, *fset;
     const uint16_t *gloc;                            // forward slot -> position in its block's set
     const float4 *fpair;                             // mixed, with gloc: {grad phi, phi} per forward slot
+    const float2 *pd8;                               // peridynamics, mixed: 8 B bond records, gather
+                                                     // [0, cap) and force [cap, 2 cap)
+    const double *pd_qb;                             // per block of kPdNpb particles: the fixed-point
+                                                     // quantum of its bond vectors
     const double2 *fpair2;                           // FP64, with gloc: {gx, gy}, {gz, phi} per forward slot
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
@@ -351,6 +355,15 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     __syncthreads();
     return s;
 }
+
+// Peridynamic bond vectors in mixed precision: reference positions relative
+// to a block's first particle X_b in fixed point, integer units of the
+// block's quantum q_b (pd_qb, chosen so that every offset within the block's
+// source sets stays below 2^29 units); the difference of two is exact, and
+// the bond vector X_j - X_i = q_b (n_j - n_i) is rounded once, converting it
+// to FP32.  Each bond record belongs to one block (that of its row's
+// particle), and q_b is in its weight.
+__device__ __forceinline__ int pd_fix(double x, double xb, double iq) { return (int)rint((x - xb) * iq); }
 
 // Blocks are handed to the XCDs round-robin; this maps them so that XCD x
 // processes the contiguous range [x*per, (x+1)*per) of logical blocks.  The
@@ -1642,15 +1655,12 @@ __global__ __launch_bounds__(kBlock) void k_pd_shape(Params p)
             w[q] = (R)om;
             tw[p.tpos[q]] = (R)om;
             if constexpr (std::is_same<R, float>::value) {
-                // mixed: the bond records of the gather and the force, w V (X_j - X_i) in list
-                // order (x, y, z blocks of dshp) and w V_i (X_i - X_j) with the source in the
-                // transposed slot
-                float *r = as<float>(p.dshp);
-                const double c = om * vj;
-                r[q] = (float)(c * h0); r[p.cap + q] = (float)(c * h1); r[2 * p.cap + q] = (float)(c * h2);
-                const double ci = -om * p.vol[node];
-                as<TPair<float>>(p.tpair)[p.tpos[q]] = TPair<float>{(float)(ci * h0), (float)(ci * h1),
-                                                                    (float)(ci * h2), (int32_t)node};
+                // mixed: the bond weights of the gather and the force, w V_j in list order
+                // (the x block of dshp) and w V_i with the source in the transposed slot;
+                // build_pd_sets packs them into 8 B records
+                as<float>(p.dshp)[q] = (float)(om * vj);
+                as<TPair<float>>(p.tpair)[p.tpos[q]] = TPair<float>{(float)(om * p.vol[node]), 0.f, 0.f,
+                                                                    (int32_t)node};
             }
             K[0] += h0 * h0 * om * vj; K[1] += h0 * h1 * om * vj; K[2] += h0 * h2 * om * vj;
             K[3] += h1 * h1 * om * vj; K[4] += h1 * h2 * om * vj; K[5] += h2 * h2 * om * vj;
@@ -1746,27 +1756,38 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
     double dmin = INFINITY;
     if constexpr (MIX) {
         // mixed: S - K = sum w V_j (u_j - u_i) (X_j - X_i)^T with u = x - X0 and
-        // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d and the
-        // packed bond records {w V_j (X_j - X_i), j's position in the block's source
-        // set, or j}; k_pd_node adds K back as F = I + (S - K) K^-1.  A block with a
-        // source set stages u and d of its sources in LDS.
-        // With the automatic time step the set also holds FP32 positions relative to
-        // the block's first particle, x_j - x_b: the bond x_j - x_i is then formed in
-        // FP32 from numbers of the bond's own size.
+        // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d.  One
+        // 8 B record per bond, {w V_j, j's position in the block's source set (or j,
+        // for a block without a set)}; the bond X_j - X_i is formed in FP32 from
+        // reference positions relative to the block's first particle X_b, staged
+        // with u and d (in their fourth lanes, z apart).  k_pd_node adds K back as
+        // F = I + (S - K) K^-1.
+        // With the automatic time step the set also holds FP32 current positions
+        // relative to the block's first particle, x_j - x_b: the bond x_j - x_i is then
+        // formed in FP32 from numbers of the bond's own size.
         float4 *Lu = reinterpret_cast<float4 *>(mega_dyn);
         int m = 0;
         const int nb0 = min(blk * (kBlock / G), p.n - 1);
         const double xb[3] = {p.x[3 * (int64_t)nb0], p.x[3 * (int64_t)nb0 + 1], p.x[3 * (int64_t)nb0 + 2]};
-        if (p.gset_ptr) {
+        const double Xb[3] = {p.X0[3 * (int64_t)nb0], p.X0[3 * (int64_t)nb0 + 1], p.X0[3 * (int64_t)nb0 + 2]};
+        const double iq = p.pd_qb ? 1.0 / p.pd_qb[blk] : 1.0;   // 1 / the block's quantum
+        float4 *Ld = Lu, *Lx = Lu;
+        float *Lz = reinterpret_cast<float *>(Lu);
+        if (p.gset_ptr && p.pd8) {
             const int s0 = p.gset_ptr[blk];
             m = p.gset_ptr[blk + 1] - s0;
+            Ld = Lu + m;
+            Lz = reinterpret_cast<float *>(Lu + 2 * m);
+            Lx = Lu + 2 * m + (m + 3) / 4;
             for (int k = threadIdx.x; k < m; k += kBlock) {
                 const int j = p.gset[s0 + k];
-                Lu[k] = p.u4[j];
-                Lu[m + k] = p.d4[j];
+                const float4 u = p.u4[j], d = p.d4[j];
+                Lu[k] = make_float4(u.x, u.y, u.z, __int_as_float(pd_fix(p.X0[3 * (int64_t)j], Xb[0], iq)));
+                Ld[k] = make_float4(d.x, d.y, d.z, __int_as_float(pd_fix(p.X0[3 * (int64_t)j + 1], Xb[1], iq)));
+                Lz[k] = __int_as_float(pd_fix(p.X0[3 * (int64_t)j + 2], Xb[2], iq));
                 if (cd_on)
-                    Lu[2 * m + k] = make_float4((float)(p.x[3 * (int64_t)j] - xb[0]), (float)(p.x[3 * (int64_t)j + 1] - xb[1]),
-                                                (float)(p.x[3 * (int64_t)j + 2] - xb[2]), 0.f);
+                    Lx[k] = make_float4((float)(p.x[3 * (int64_t)j] - xb[0]), (float)(p.x[3 * (int64_t)j + 1] - xb[1]),
+                                        (float)(p.x[3 * (int64_t)j + 2] - xb[2]), 0.f);
             }
             if (m > 0) __syncthreads();
         }
@@ -1774,32 +1795,60 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
             const float4 ui = p.u4[node], di = p.d4[node];
             const double xi[3] = {p.x[3 * (int64_t)node], p.x[3 * (int64_t)node + 1], p.x[3 * (int64_t)node + 2]};
             const float pi[3] = {(float)(xi[0] - xb[0]), (float)(xi[1] - xb[1]), (float)(xi[2] - xb[2])};
+            const int Xi[3] = {pd_fix(p.X0[3 * (int64_t)node], Xb[0], iq),
+                               pd_fix(p.X0[3 * (int64_t)node + 1], Xb[1], iq),
+                               pd_fix(p.X0[3 * (int64_t)node + 2], Xb[2], iq)};
+            const double *Xn = p.X0 + 3 * (int64_t)node;
             const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
-            if (p.fpair) {
+            const float2 *rec8 = p.pd8;
+            const float *rw = as<float>(p.dshp);   // w V_j in list order, before the records are packed
+            // LDS: the block's source set (j is a set position), else global memory (j is
+            // the particle); one instantiation each, so that no bond chooses between them
+            auto sweep = [&](auto lds_tag) {
+                constexpr bool LDS = decltype(lds_tag)::value;
                 for (int base = q0 + lane; base < q1; base += G * kPdUnroll) {
-                    float4 rec[kPdUnroll], uj[kPdUnroll], dj[kPdUnroll];
+                    float2 rec[kPdUnroll];
+                    float4 uj[kPdUnroll], dj[kPdUnroll];
+                    float r[kPdUnroll][3];
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
                         const int q = base + G * e;
-                        rec[e] = q < q1 ? p.fpair[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(m > 0 ? 0 : node));
+                        const int pad = LDS ? 0 : node;   // a zero weight on a valid source
+                        if (LDS || rec8) rec[e] = q < q1 ? rec8[q] : make_float2(0.f, __int_as_float(pad));
+                        else rec[e] = q < q1 ? make_float2(rw[q], __int_as_float(p.col[q])) : make_float2(0.f, __int_as_float(pad));
                     }
+                    // bond j: (w V_j (X_j - X_i), u_j, d_j)
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
-                        const int j = __float_as_int(rec[e].w);
-                        uj[e] = m > 0 ? Lu[j] : p.u4[j];
-                        dj[e] = m > 0 ? Lu[m + j] : p.d4[j];
+                        const int j = __float_as_int(rec[e].y);
+                        float xr[3];
+                        if constexpr (LDS) {
+                            uj[e] = Lu[j];
+                            dj[e] = Ld[j];
+                            xr[0] = (float)(__float_as_int(uj[e].w) - Xi[0]);
+                            xr[1] = (float)(__float_as_int(dj[e].w) - Xi[1]);
+                            xr[2] = (float)(__float_as_int(Lz[j]) - Xi[2]);
+                        } else {
+                            uj[e] = p.u4[j];
+                            dj[e] = p.d4[j];
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) xr[k] = (float)((p.X0[3 * (int64_t)j + k] - Xn[k]) * iq);
+                        }
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) r[e][k] = rec[e].x * xr[k];
                     }
                     if (cd_on) {
 #pragma unroll
                         for (int e = 0; e < kPdUnroll; ++e) {
-                            const int j = __float_as_int(rec[e].w);
-                            if (m > 0) {   // the particle itself is the bond with a zero record
-                                if (rec[e].x != 0.f || rec[e].y != 0.f || rec[e].z != 0.f) {
-                                    const float4 pj = Lu[2 * m + j];
+                            const int j = __float_as_int(rec[e].y);
+                            if constexpr (LDS) {
+                                // the particle itself is the bond of zero length; padding has zero weight
+                                if (rec[e].x != 0.f && (r[e][0] != 0.f || r[e][1] != 0.f || r[e][2] != 0.f)) {
+                                    const float4 pj = Lx[j];
                                     const float cx = pj.x - pi[0], cy = pj.y - pi[1], cz = pj.z - pi[2];
                                     dmin = fmin(dmin, (double)(cx * cx + cy * cy + cz * cz));
                                 }
-                            } else if (j != node) {
+                            } else if (j != node && rec[e].x != 0.f) {
                                 dmin = fmin(dmin, sq_len_unfused(p.x[3 * (int64_t)j] - xi[0], p.x[3 * (int64_t)j + 1] - xi[1],
                                                                  p.x[3 * (int64_t)j + 2] - xi[2]));
                             }
@@ -1807,38 +1856,20 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                     }
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
-                        const float r[3] = {rec[e].x, rec[e].y, rec[e].z};
                         const float a[3] = {uj[e].x - ui.x, uj[e].y - ui.y, uj[e].z - ui.z};
                         const float b[3] = {dj[e].x - di.x, dj[e].y - di.y, dj[e].z - di.z};
 #pragma unroll
                         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
                             for (int aa = 0; aa < 3; ++aa) {
-                                S[aa + 3 * bb] += (double)a[aa] * r[bb];
-                                L[aa + 3 * bb] += (double)b[aa] * r[bb];
+                                S[aa + 3 * bb] += (double)a[aa] * r[e][bb];
+                                L[aa + 3 * bb] += (double)b[aa] * r[e][bb];
                             }
                     }
                 }
-            } else {   // before the records are packed (first call): SoA records and indices
-                const float *rx = as<float>(p.dshp), *ry = rx + p.cap, *rz = ry + p.cap;
-                for (int q = q0 + lane; q < q1; q += G) {
-                    const int j = p.col[q];
-                    if (cd_on && j != node)
-                        dmin = fmin(dmin, sq_len_unfused(p.x[3 * (int64_t)j] - xi[0], p.x[3 * (int64_t)j + 1] - xi[1],
-                                                         p.x[3 * (int64_t)j + 2] - xi[2]));
-                    const float r[3] = {rx[q], ry[q], rz[q]};
-                    const float4 u = p.u4[j], d = p.d4[j];
-                    const float a[3] = {u.x - ui.x, u.y - ui.y, u.z - ui.z};
-                    const float b[3] = {d.x - di.x, d.y - di.y, d.z - di.z};
-#pragma unroll
-                    for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-                        for (int aa = 0; aa < 3; ++aa) {
-                            S[aa + 3 * bb] += (double)a[aa] * r[bb];
-                            L[aa + 3 * bb] += (double)b[aa] * r[bb];
-                        }
-                }
-            }
+            };
+            if (m > 0) sweep(std::true_type{});
+            else sweep(std::false_type{});
         }
     } else if (live) {
         const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
@@ -1926,12 +1957,24 @@ __global__ __launch_bounds__(kBlock) void k_pd_node(Params p)
     }
     as<R>(p.eps)[i] = ep;
     double si[6], ei[6];
+    // mixed: the stress increment of the wave-speed estimate without the FP32
+    // roundings of the whole stress (rotated stress to FP32, trial sum): the
+    // rotation's change and C:de in FP64 plus the return map's correction,
+    // equal in exact arithmetic to new - old.  ESTIMATE_MODULI's secant moduli
+    // switch at increments of 1e-8, where those roundings alone would decide.
+    double cde[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (std::is_same<R, float>::value) {
+        const double z6[6] = {0, 0, 0, 0, 0, 0};
+        mega::elastic_trial(z6, de, mt, cde);
+    }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         stress[k] = sr[k];
         strain[k] = (R)e6[k];
+        if constexpr (std::is_same<R, float>::value)
+            si[k] = (sig[k] - old_s[k]) + cde[k] + ((double)sr[k] - (double)pred[k]);
         sig[k] = (double)sr[k];
-        si[k] = sig[k] - old_s[k];
+        if constexpr (!std::is_same<R, float>::value) si[k] = sig[k] - old_s[k];
         ei[k] = e6[k] - old_s[k];   // the reference's STRAIN_INC = LSTRAIN - GSTRESS (FINT_PD.F90:509)
     }
     if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
@@ -1967,47 +2010,67 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const R *Pb = as<R>(p.P);
     if constexpr (MIX) {
-        // mixed: one 16 B record per transposed slot holds w V_k (X_k - X_i) and the
-        // source k, or k's position in the block's source set, whose T rows (the 9
-        // used values, 40 B) a block with a set stages in LDS
-        float2 *Lt = reinterpret_cast<float2 *>(mega_dyn);
+        // mixed: one 8 B record per transposed slot, {w V_k, k's position in the
+        // block's source set (or k, for a block without a set)}; a block with a set
+        // stages per source the 9 used values of its T row and its reference
+        // position relative to the block's first particle X_b (FP32, 48 B), and the
+        // bond X_k - X_i is formed from those
+        float4 *Lt = reinterpret_cast<float4 *>(mega_dyn);
         int m = 0;
-        if (p.fset_ptr) {
+        const int nb0 = min(blk * (kBlock / G), p.n - 1);
+        const double Xb[3] = {p.X0[3 * (int64_t)nb0], p.X0[3 * (int64_t)nb0 + 1], p.X0[3 * (int64_t)nb0 + 2]};
+        const double iq = p.pd_qb ? 1.0 / p.pd_qb[blk] : 1.0;   // 1 / the block's quantum
+        if (p.fset_ptr && p.pd8) {
             const int s0 = p.fset_ptr[blk];
             m = p.fset_ptr[blk + 1] - s0;
             for (int r = threadIdx.x; r < m; r += kBlock) {
-                const float4 *q = reinterpret_cast<const float4 *>(Pb) + 3 * (int64_t)p.fset[s0 + r];
+                const int k = p.fset[s0 + r];
+                const float4 *q = reinterpret_cast<const float4 *>(Pb) + 3 * (int64_t)k;
                 const float4 a = q[0], b = q[1], c = q[2];
-                Lt[5 * r] = make_float2(a.x, a.y); Lt[5 * r + 1] = make_float2(a.z, a.w);
-                Lt[5 * r + 2] = make_float2(b.x, b.y); Lt[5 * r + 3] = make_float2(b.z, b.w);
-                Lt[5 * r + 4] = make_float2(c.x, c.y);
+                Lt[3 * r] = a;
+                Lt[3 * r + 1] = b;
+                Lt[3 * r + 2] = make_float4(c.x, __int_as_float(pd_fix(p.X0[3 * (int64_t)k], Xb[0], iq)),
+                                            __int_as_float(pd_fix(p.X0[3 * (int64_t)k + 1], Xb[1], iq)),
+                                            __int_as_float(pd_fix(p.X0[3 * (int64_t)k + 2], Xb[2], iq)));
             }
             if (m > 0) __syncthreads();
         }
         if (live) {
-            const TPair<float> *tp = as<const TPair<float>>(p.tpair);
+            const int Xi[3] = {pd_fix(p.X0[3 * (int64_t)node], Xb[0], iq),
+                               pd_fix(p.X0[3 * (int64_t)node + 1], Xb[1], iq),
+                               pd_fix(p.X0[3 * (int64_t)node + 2], Xb[2], iq)};
+            const double *Xn = p.X0 + 3 * (int64_t)node;
+            const float2 *rec8 = p.pd8 ? p.pd8 + p.cap : nullptr;
+            const TPair<float> *tp = as<const TPair<float>>(p.tpair);   // {w V_k, -, -, k} before packing
             const int t0 = p.trow[node], t1 = p.trow[node + 1];
             for (int base = t0 + lane; base < t1; base += G * kPdUnroll) {
-                TPair<float> e[kPdUnroll];
+                float2 e[kPdUnroll];
 #pragma unroll
                 for (int u = 0; u < kPdUnroll; ++u) {
                     const int t = base + G * u;
-                    e[u] = t < t1 ? tp[t] : TPair<float>{0.f, 0.f, 0.f, 0};
+                    if (rec8) e[u] = t < t1 ? rec8[t] : make_float2(0.f, 0.f);
+                    else e[u] = t < t1 ? make_float2(tp[t].gx, __int_as_float(tp[t].src)) : make_float2(0.f, 0.f);
                 }
 #pragma unroll
                 for (int u = 0; u < kPdUnroll; ++u) {
-                    float T[10];
+                    const int k = __float_as_int(e[u].y);
+                    float T[10], xr[3];
                     if (m > 0) {
-                        const float2 *r = Lt + 5 * e[u].src;
-                        const float2 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3], v4 = r[4];
-                        T[0] = v0.x; T[1] = v0.y; T[2] = v1.x; T[3] = v1.y; T[4] = v2.x; T[5] = v2.y;
-                        T[6] = v3.x; T[7] = v3.y; T[8] = v4.x;
+                        const float4 v0 = Lt[3 * k], v1 = Lt[3 * k + 1], v2 = Lt[3 * k + 2];
+                        T[0] = v0.x; T[1] = v0.y; T[2] = v0.z; T[3] = v0.w; T[4] = v1.x; T[5] = v1.y;
+                        T[6] = v1.z; T[7] = v1.w; T[8] = v2.x;
+                        xr[0] = (float)(__float_as_int(v2.y) - Xi[0]);
+                        xr[1] = (float)(__float_as_int(v2.z) - Xi[1]);
+                        xr[2] = (float)(__float_as_int(v2.w) - Xi[2]);
                     } else {
-                        load_P<float>(Pb, e[u].src, T);
+                        load_P<float>(Pb, k, T);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) xr[c] = (float)((p.X0[3 * (int64_t)k + c] - Xn[c]) * iq);
                     }
-                    a0 += (double)(T[0] * e[u].gx + T[3] * e[u].gy + T[6] * e[u].gz);
-                    a1 += (double)(T[1] * e[u].gx + T[4] * e[u].gy + T[7] * e[u].gz);
-                    a2 += (double)(T[2] * e[u].gx + T[5] * e[u].gy + T[8] * e[u].gz);
+                    const float gx = e[u].x * xr[0], gy = e[u].x * xr[1], gz = e[u].x * xr[2];
+                    a0 += (double)(T[0] * gx + T[3] * gy + T[6] * gz);
+                    a1 += (double)(T[1] * gx + T[4] * gy + T[7] * gz);
+                    a2 += (double)(T[2] * gx + T[5] * gy + T[8] * gz);
                 }
             }
         }
@@ -2477,18 +2540,52 @@ __global__ void k_fpair(int64_t nnz, int64_t cap, const float *shp, const float 
     out[q] = make_float4(dshp[q], dshp[cap + q], dshp[2 * cap + q], shp[q]);
 }
 
-// peridynamics (mixed): the gather's bond record {w V_j (X_j - X_i), j's position
-// in the block's source set (or j, for a block without a set)}, kPdG lanes per particle
-__global__ void k_pd_fpair(int n, const int32_t *row_ptr, const int32_t *col, int64_t cap, const float *rec,
-                           const uint16_t *loc, const int32_t *gset_ptr, float4 *out)
+// peridynamics (mixed): the 8 B bond records, out[q] = {w V_j, j's position in the
+// block's source set (or j, for a block without a set)} for the gather in list
+// order and out[cap + t] = {w V_k, k's set position (or k)} for the force in
+// transposed order (the set builder has written the position into the
+// transposed slot's src); kPdG lanes per particle
+__global__ void k_pd_pack8(int n, const int32_t *row_ptr, const int32_t *col, const int32_t *trow, int64_t cap,
+                           const float *wv, const uint16_t *loc, const int32_t *gset_ptr, const TPair<float> *tp,
+                           const double *qb, float2 *out)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = (int)(t / kPdG), lane = (int)(t % kPdG);
     if (i >= n) return;
     const int b = i / kPdNpb;
     const bool set = gset_ptr[b + 1] > gset_ptr[b];
+    const double quantum = qb[b];
     for (int q = row_ptr[i] + lane; q < row_ptr[i + 1]; q += kPdG)
-        out[q] = make_float4(rec[q], rec[cap + q], rec[2 * cap + q], __int_as_float(set ? (int)loc[q] : col[q]));
+        out[q] = make_float2((float)((double)wv[q] * quantum), __int_as_float(set ? (int)loc[q] : col[q]));
+    for (int s = trow[i] + lane; s < trow[i + 1]; s += kPdG)
+        out[cap + s] = make_float2((float)((double)tp[s].gx * quantum), __int_as_float(tp[s].src));
+}
+
+// per block b, the quantum q_b = R_b / 2^29 with R_b the largest offset
+// |X_j - X_b|_inf of a source-set member or a particle of the block from the
+// block's first particle (1 for a block of coincident particles); one thread
+// block per set
+__global__ __launch_bounds__(kBlock) void k_pd_extent(int n, const double *X0, const int32_t *gptr, const int32_t *gset,
+                                                      const int32_t *fptr, const int32_t *fset, double *qb)
+{
+    __shared__ double s_m[kBlock / 64];
+    const int b = blockIdx.x, i0 = b * kPdNpb;
+    const double xb[3] = {X0[3 * (int64_t)i0], X0[3 * (int64_t)i0 + 1], X0[3 * (int64_t)i0 + 2]};
+    double r = 0.0;
+    auto take = [&](int j) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r = fmax(r, fabs(X0[3 * (int64_t)j + k] - xb[k]));
+    };
+    for (int k = gptr[b] + threadIdx.x; k < gptr[b + 1]; k += kBlock) take(gset[k]);
+    for (int k = fptr[b] + threadIdx.x; k < fptr[b + 1]; k += kBlock) take(fset[k]);
+    if (threadIdx.x < kPdNpb && i0 + (int)threadIdx.x < n) take(i0 + threadIdx.x);
+    for (int m = 32; m >= 1; m >>= 1) r = fmax(r, __shfl_xor(r, m, 64));
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) r = fmax(r, s_m[w]);
+        qb[b] = r > 0.0 ? r * 0x1p-29 : 1.0;
+    }
 }
 
 __global__ void k_fpair2(int64_t nnz, int64_t cap, const double *shp, const double *dshp, double2 *out)
@@ -2654,6 +2751,7 @@ struct mega_ctx {
     int hv_every = 0, hv_active = 0;
     // correspondence peridynamics
     int pd = 0, pd_shapes = 0;
+    double *pd_qb = nullptr;   // mixed peridynamics: per block, the fixed-point quantum of the bond vectors
     double *Kinv = nullptr, *gvec = nullptr;
     // node numbering: internal node i is the caller's node perm[i]
     std::vector<int32_t> perm, iperm;
@@ -2694,8 +2792,9 @@ Params params(mega_ctx *c)
     if (c->sets_on) {
         p.fset_ptr = c->fset_ptr; p.fset = c->fset;
         p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc;
-        if (c->mixed) p.fpair = c->fpair;
-        else p.fpair2 = c->fpair2;
+        if (!c->mixed) p.fpair2 = c->fpair2;
+        else if (c->pd) { p.pd8 = reinterpret_cast<const float2 *>(c->fpair); p.pd_qb = c->pd_qb; }
+        else p.fpair = c->fpair;
     }
     p.shp = c->shp; p.dshp = c->dshp; p.tpair = c->tpair; p.tshp = c->tshp;
     p.X0 = c->X0; p.win = c->win; p.winv = c->winv; p.vol = c->vol; p.mass = c->mass;
@@ -2715,7 +2814,8 @@ Params params(mega_ctx *c)
     p.surr_mat = c->surr_mat; p.surr_tol = c->surr_tol; p.sstat = c->sstat;
     for (int k = 0; k < 2 * mlp::kIn; ++k) p.surr_env[k] = c->surr_env[k];
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
-    p.pd = c->pd; p.pd_linit = 0; p.Kinv = c->Kinv; p.gvec = c->gvec;
+    p.pd = c->pd; p.pd_linit = 0;
+    p.Kinv = c->Kinv; p.gvec = c->gvec;
     return p;
 }
 
@@ -3004,6 +3104,7 @@ int build_pd_sets(mega_ctx *c)
     if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nb + 1);
     if (!c->set_max) e |= dalloc(&c->set_max, 1);
     if (!c->gset_ptr) e |= dalloc(&c->gset_ptr, nb + 1);
+    if (!c->pd_qb) e |= dalloc(&c->pd_qb, nb);
     if (c->gloc_cap < c->nnz) {
         (void)hipFree(c->gloc);
         c->gloc = nullptr;
@@ -3026,8 +3127,12 @@ int build_pd_sets(mega_ctx *c)
         r = build_set<float>(c, kPdNpb, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc, nullptr,
                              &c->gset_max, kPdSetCap);
     if (r) return r;
-    k_pd_fpair<<<grid_for((int64_t)c->n * kPdG), kBlock, 0, c->stream>>>(
-        c->n, c->row_ptr, c->col, c->nnz_cap, (const float *)c->dshp, c->gloc, c->gset_ptr, c->fpair);
+    // the fixed-point quantum of each block's bond vectors: offsets below 2^29 units
+    k_pd_extent<<<nb, kBlock, 0, c->stream>>>(c->n, c->X0, c->gset_ptr, c->gset, c->fset_ptr, c->fset, c->pd_qb);
+    HIPCHK(hipGetLastError());
+    k_pd_pack8<<<grid_for((int64_t)c->n * kPdG), kBlock, 0, c->stream>>>(
+        c->n, c->row_ptr, c->col, c->trow, c->nnz_cap, (const float *)c->dshp, c->gloc, c->gset_ptr,
+        static_cast<const TPair<float> *>(c->tpair), c->pd_qb, reinterpret_cast<float2 *>(c->fpair));
     HIPCHK(hipGetLastError());
     c->sets_on = 1;
     return MEGA_OK;
@@ -3156,10 +3261,12 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     p.pd_linit = linit;
     int nb = 0;
     const unsigned g = xcd_grid(c->n, &nb, MIX ? kPdNpb : kNodesPerBlock);
-    // mixed with source sets: u and d (32 B) per gather source, T rows (40 B) per force source
-    // (+ 16 B per source with the automatic time step: FP32 positions relative to the block)
-    const size_t lds_g = (MIX && p.gset_ptr) ? (size_t)c->gset_max * (p.char_dist ? 48 : 32) : 0;
-    const size_t lds_f = (MIX && p.fset_ptr) ? (size_t)c->fset_max * 40 : 0;
+    // mixed with source sets: u, d and the relative reference position (36 B) per gather
+    // source, T rows and the relative reference position (48 B) per force source (+ 16 B
+    // per gather source with the automatic time step: FP32 current positions)
+    const size_t mg = (size_t)c->gset_max;
+    const size_t lds_g = (MIX && p.gset_ptr && p.pd8) ? 32 * mg + 16 * ((mg + 3) / 4) + (p.char_dist ? 16 * mg : 0) : 0;
+    const size_t lds_f = (MIX && p.fset_ptr && p.pd8) ? (size_t)c->fset_max * 48 : 0;
     {
         Scope s(c, "k_pd_gather");
         if (c->mixed) k_pd_u4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->x, c->X0, c->u4);
@@ -3537,7 +3644,7 @@ int mega_destroy(mega_ctx *c)
                     c->cub_tmp, c->surr_tiles, c->surr_bias, c->surr_norm, c->sstat, c->hv, c->hv_count,
                     c->Kinv, c->gvec, c->orig, c->cells_ne, c->cell_flag, c->tbin, c->smask, c->gset_ptr,
                     c->gset, c->fset_ptr, c->fset, c->set_max, c->gloc, c->fpair, c->fpair2, c->dt_dev,
-                    c->dt_hist, c->char_dist, c->dt_min, c->sml, c->aov};
+                    c->dt_hist, c->char_dist, c->dt_min, c->sml, c->aov, c->pd_qb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }

@@ -128,8 +128,8 @@ def test_pd_automatic_time_step_matches_reference_program(precision):
         assert worst["v"] < 1e-9 and worst["fint"] < 1e-9 and worst["stress"] < 1e-9
     else:
         # the shortest bonds come from FP32 positions relative to each block of
-        # particles; measured: DLT 3.3e-7, TIME 5.4e-8, x 9.4e-9, v 2.7e-7, fint 3.3e-7,
-        # stress 5.4e-7
+        # particles; measured: DLT 3.3e-7, TIME 4.8e-8, x 1.1e-8, v 2.9e-7, fint 2.9e-7,
+        # stress 4.6e-7
         assert e_dlt < 3.5e-6 and e_t < 6e-7
         assert worst["x"] < 1e-7 and worst["v"] < 3e-6 and worst["fint"] < 3.5e-6 and worst["stress"] < 6e-6
 
@@ -168,5 +168,5 @@ def test_pd_mixed_precision_steps():
         m.close()
     errs = {k: rel_err(out[mega.MEGA_MIXED][k], out[mega.MEGA_FP64][k]) for k in out[mega.MEGA_FP64]}
     print("mixed vs FP64 after 40 steps:", {k: f"{v:.2e}" for k, v in errs.items()})
-    # measured: x 7.0e-9, uphy 7.3e-8, v 1.3e-7, stress 2.5e-7
+    # measured: x 1.2e-8, uphy 1.2e-7, v 1.5e-7, stress 2.1e-7
     assert errs["x"] < 7e-8 and errs["uphy"] < 7e-7 and errs["v"] < 1.3e-6 and errs["stress"] < 2.5e-6

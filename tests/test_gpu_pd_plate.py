@@ -55,7 +55,7 @@ def test_reduced_plate_matches_oracle(lib, oracle, precision):
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
         assert fsum < 1e-12
     else:
-        # measured: x 3.6e-10, v 1.3e-7, fint 2.1e-7, stress 2.8e-7, force sum 2.2e-10
+        # measured: x 3.4e-10, v 1.2e-7, fint 2.5e-7, stress 3.1e-7, force sum 3.4e-10
         assert errs["x"] < 4e-9 and errs["v"] < 1.5e-6 and errs["fint"] < 2e-6 and errs["stress"] < 3e-6
         assert fsum < 3e-9
 
@@ -82,7 +82,7 @@ def test_reduced_plate_automatic_time_step_matches_oracle(lib, oracle, precision
     if precision == "fp64":
         assert e_dlt < 1e-14
         assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
-    else:   # measured: DLT 2.0e-7, x 5.7e-10, v 1.9e-7, fint 5.7e-7, stress 4.5e-7
+    else:   # measured: DLT 2.0e-7, x 7.1e-10, v 1.7e-7, fint 7.3e-7, stress 4.9e-7
         assert e_dlt < 2e-6
         assert errs["x"] < 6e-9 and errs["v"] < 2e-6 and errs["fint"] < 6e-6 and errs["stress"] < 5e-6
 
@@ -134,8 +134,8 @@ def test_full_plate_impact_mixed_vs_fp64(lib, plate):
           f"fp64 {out['fp64']['fsum']:.1e} mixed {out['mixed']['fsum']:.1e}", flush=True)
     for r in out.values():
         assert r["status"] == 0 and np.isfinite(r["x"]).all() and np.isfinite(r["fint"]).all()
-    # measured: force sums 8.2e-16 (FP64), 2.4e-10 (mixed); mixed vs FP64 x 7.4e-11, v 1.6e-7,
-    # fint 2.2e-7, stress 3.7e-7
+    # measured: force sums 8.3e-16 (FP64), 1.8e-10 (mixed); mixed vs FP64 x 7.9e-11, v 1.7e-7,
+    # fint 2.2e-7, stress 4.0e-7
     assert out["fp64"]["fsum"] < 1e-14 and out["mixed"]["fsum"] < 3e-9
     assert np.abs(out["fp64"]["v"][:, 2]).max() > 100e3   # the plug still moves (mm/s)
     assert errs["x"] < 1e-9 and errs["v"] < 2e-6 and errs["fint"] < 2e-6 and errs["stress"] < 4e-6
