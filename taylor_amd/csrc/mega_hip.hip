@@ -1756,7 +1756,9 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
     double dmin = INFINITY;
     if constexpr (MIX) {
         // mixed: S - K = sum w V_j (u_j - u_i) (X_j - X_i)^T with u = x - X0 and
-        // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d.  One
+        // L0 = sum w V_j (d_j - d_i) (X_j - X_i)^T from FP32 copies of u and d, each
+        // lane's sums in FP32 (terms of one sign and size: differences, not
+        // positions), the lanes' partials in FP64.  One
         // 8 B record per bond, {w V_j, j's position in the block's source set (or j,
         // for a block without a set)}; the bond X_j - X_i is formed in FP32 from
         // reference positions relative to the block's first particle X_b, staged
@@ -1801,6 +1803,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
             const double *Xn = p.X0 + 3 * (int64_t)node;
             const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
             const float2 *rec8 = p.pd8;
+            float Sf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Lf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             const float *rw = as<float>(p.dshp);   // w V_j in list order, before the records are packed
             // LDS: the block's source set (j is a set position), else global memory (j is
             // the particle); one instantiation each, so that no bond chooses between them
@@ -1862,14 +1865,16 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
                             for (int aa = 0; aa < 3; ++aa) {
-                                S[aa + 3 * bb] += (double)a[aa] * r[e][bb];
-                                L[aa + 3 * bb] += (double)b[aa] * r[e][bb];
+                                Sf[aa + 3 * bb] += a[aa] * r[e][bb];
+                                Lf[aa + 3 * bb] += b[aa] * r[e][bb];
                             }
                     }
                 }
             };
             if (m > 0) sweep(std::true_type{});
             else sweep(std::false_type{});
+#pragma unroll
+            for (int k = 0; k < 9; ++k) { S[k] = (double)Sf[k]; L[k] = (double)Lf[k]; }
         }
     } else if (live) {
         const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
