@@ -140,6 +140,17 @@ constexpr int kSetCap = MEGA_SET_CAP;
 #define MEGA_PD_UNROLL 4
 #endif
 constexpr int kPdSetCap = MEGA_PD_SET_CAP;
+// peridynamics in FP64: the kernels' blocks of kNodesPerBlock particles (16 lanes
+// each) and source sets: up to kPdSetCap64 80 B rows (X0, x, d) for the gather,
+// up to kPdFSetCap64 96 B rows (T, X0) for the force (four force blocks' sets
+// fill a CU's 160 KB; a larger set reads global memory)
+#ifndef MEGA_PD_SET_CAP64
+#define MEGA_PD_SET_CAP64 640
+#endif
+#ifndef MEGA_PD_FSET_CAP64
+#define MEGA_PD_FSET_CAP64 416
+#endif
+constexpr int kPdSetCap64 = MEGA_PD_SET_CAP64, kPdFSetCap64 = MEGA_PD_FSET_CAP64;
 constexpr int kPdG = MEGA_PD_GROUP;       // lanes per particle
 constexpr int kPdNpb = kBlock / kPdG;     // particles per block (and per source set)
 constexpr int kPdUnroll = MEGA_PD_UNROLL; // bonds per lane in flight
@@ -175,6 +186,8 @@ struct Params {
                                                      // [0, cap) and force [cap, 2 cap)
     const double *pd_qb;                             // per block of kPdNpb particles: the fixed-point
                                                      // quantum of its bond vectors
+    const double2 *pd16;                             // peridynamics, FP64: 16 B bond records {w V, set
+                                                     // position}, gather [0, cap) and force [cap, 2 cap)
     const double2 *fpair2;                           // FP64, with gloc: {gx, gy}, {gz, phi} per forward slot
     void *shp, *dshp, *tpair, *tshp;                 // R; grad phi as x, y, z blocks at +0, +cap, +2 cap;
                                                      // tpair: TPair<R> per transposed slot
@@ -712,7 +725,10 @@ __global__ __launch_bounds__(kBlock) void k_snni(Params p, const double *xs, con
 // difference).  Two waves per SIMD are kept by bound: the fused sums' few
 // extra registers would otherwise move the kernel to one (4-8 VGPRs spill).
 template <class R, bool GATHER>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
+#ifndef MEGA_RK_WAVES
+#define MEGA_RK_WAVES 2
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MEGA_RK_WAVES, MEGA_RK_WAVES))) void k_rk(Params p, const double *xs, const int32_t *order, int nblocks)
 {
     static_assert(!GATHER || kRkGroup == kGroup, "the fused gather visits entries as k_gather does");
     constexpr bool MIX = std::is_same<R, float>::value;
@@ -1876,25 +1892,64 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
 #pragma unroll
             for (int k = 0; k < 9; ++k) { S[k] = (double)Sf[k]; L[k] = (double)Lf[k]; }
         }
-    } else if (live) {
-        const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
-        const double x[3] = {p.x[3 * node], p.x[3 * node + 1], p.x[3 * node + 2]};
-        const double d[3] = {p.d[3 * node], p.d[3 * node + 1], p.d[3 * node + 2]};
-        const R *w = as<R>(p.shp);
-        for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
-            const int j = p.col[q];
-            const double wv = (double)w[q] * p.vol[j];
-            const double h[3] = {p.X0[3 * j] - X[0], p.X0[3 * j + 1] - X[1], p.X0[3 * j + 2] - X[2]};
-            const double hs[3] = {p.x[3 * j] - x[0], p.x[3 * j + 1] - x[1], p.x[3 * j + 2] - x[2]};
-            const double hd[3] = {p.d[3 * j] - d[0], p.d[3 * j + 1] - d[1], p.d[3 * j + 2] - d[2]};
-            if (cd_on && j != node) dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
+    } else {
+        // FP64: with the packed records {w V_j, j's set position} a block with a source
+        // set stages X0, x and d of its sources in LDS (80 B rows); the same products
+        // in the same lanes and order as without them
+        double2 *Ls = reinterpret_cast<double2 *>(mega_dyn);
+        int m = 0;
+        if (p.pd16 && p.gset_ptr) {
+            const int s0 = p.gset_ptr[blk];
+            m = p.gset_ptr[blk + 1] - s0;
+            for (int k = threadIdx.x; k < m; k += kBlock) {
+                const int64_t j = p.gset[s0 + k];
+                Ls[5 * k] = make_double2(p.X0[3 * j], p.X0[3 * j + 1]);
+                Ls[5 * k + 1] = make_double2(p.X0[3 * j + 2], p.x[3 * j]);
+                Ls[5 * k + 2] = make_double2(p.x[3 * j + 1], p.x[3 * j + 2]);
+                Ls[5 * k + 3] = make_double2(p.d[3 * j], p.d[3 * j + 1]);
+                Ls[5 * k + 4] = make_double2(p.d[3 * j + 2], 0.0);
+            }
+            if (m > 0) __syncthreads();
+        }
+        if (live) {
+            const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
+            const double x[3] = {p.x[3 * node], p.x[3 * node + 1], p.x[3 * node + 2]};
+            const double d[3] = {p.d[3 * node], p.d[3 * node + 1], p.d[3 * node + 2]};
+            auto term = [&](const double *h, const double *hs, const double *hd, double wv) {
 #pragma unroll
-            for (int b = 0; b < 3; ++b)
+                for (int b = 0; b < 3; ++b)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    S[a + 3 * b] += hs[a] * h[b] * wv;
-                    L[a + 3 * b] += hd[a] * h[b] * wv;
+                    for (int a = 0; a < 3; ++a) {
+                        S[a + 3 * b] += hs[a] * h[b] * wv;
+                        L[a + 3 * b] += hd[a] * h[b] * wv;
+                    }
+            };
+            if (m > 0) {
+                for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
+                    const double2 rec = p.pd16[q];
+                    const int k = (int)__double_as_longlong(rec.y);
+                    const double2 v0 = Ls[5 * k], v1 = Ls[5 * k + 1], v2 = Ls[5 * k + 2], v3 = Ls[5 * k + 3],
+                                  v4 = Ls[5 * k + 4];
+                    const double h[3] = {v0.x - X[0], v0.y - X[1], v1.x - X[2]};
+                    const double hs[3] = {v1.y - x[0], v2.x - x[1], v2.y - x[2]};
+                    const double hd[3] = {v3.x - d[0], v3.y - d[1], v4.x - d[2]};
+                    // the particle itself is the bond of zero reference length
+                    if (cd_on && (h[0] != 0.0 || h[1] != 0.0 || h[2] != 0.0))
+                        dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
+                    term(h, hs, hd, rec.x);
                 }
+            } else {
+                const R *w = as<R>(p.shp);
+                for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
+                    const int j = p.col[q];
+                    const double wv = (double)w[q] * p.vol[j];
+                    const double h[3] = {p.X0[3 * j] - X[0], p.X0[3 * j + 1] - X[1], p.X0[3 * j + 2] - X[2]};
+                    const double hs[3] = {p.x[3 * j] - x[0], p.x[3 * j + 1] - x[1], p.x[3 * j + 2] - x[2]};
+                    const double hd[3] = {p.d[3 * j] - d[0], p.d[3 * j + 1] - d[1], p.d[3 * j + 2] - d[2]};
+                    if (cd_on && j != node) dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
+                    term(h, hs, hd, wv);
+                }
+            }
         }
     }
 #pragma unroll
@@ -2079,18 +2134,55 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
                 }
             }
         }
-    } else if (live) {
-        const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
-        const R *tw = as<R>(p.tshp);
-        for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
-            const int k = p.tsrc[t];
-            R T[10];
-            load_P<R>(Pb, k, T);
-            const double c = (double)tw[t] * (double)T[9];
-            const double dx[3] = {p.X0[3 * k] - X[0], p.X0[3 * k + 1] - X[1], p.X0[3 * k + 2] - X[2]};
-            a0 += c * ((double)T[0] * dx[0] + (double)T[3] * dx[1] + (double)T[6] * dx[2]);
-            a1 += c * ((double)T[1] * dx[0] + (double)T[4] * dx[1] + (double)T[7] * dx[2]);
-            a2 += c * ((double)T[2] * dx[0] + (double)T[5] * dx[1] + (double)T[8] * dx[2]);
+    } else {
+        // FP64: with the packed records {w V_k, k's set position} a block with a source
+        // set stages the T rows (9 values) and X0 of its sources in LDS (96 B rows);
+        // the same products in the same lanes and order as without them
+        double2 *Ls = reinterpret_cast<double2 *>(mega_dyn);
+        int m = 0;
+        if (p.pd16 && p.fset_ptr) {
+            const int s0 = p.fset_ptr[blk];
+            m = p.fset_ptr[blk + 1] - s0;
+            for (int r = threadIdx.x; r < m; r += kBlock) {
+                const int64_t k = p.fset[s0 + r];
+                const double2 *q = reinterpret_cast<const double2 *>(Pb + 10 * k);
+                const double2 t0 = q[0], t1 = q[1], t2 = q[2], t3 = q[3], t4 = q[4];
+                Ls[6 * r] = t0; Ls[6 * r + 1] = t1; Ls[6 * r + 2] = t2; Ls[6 * r + 3] = t3;
+                Ls[6 * r + 4] = make_double2(t4.x, p.X0[3 * k]);
+                Ls[6 * r + 5] = make_double2(p.X0[3 * k + 1], p.X0[3 * k + 2]);
+            }
+            if (m > 0) __syncthreads();
+        }
+        if (live) {
+            const double X[3] = {p.X0[3 * node], p.X0[3 * node + 1], p.X0[3 * node + 2]};
+            auto term = [&](const double *T, const double *dx, double c) {
+                a0 += c * (T[0] * dx[0] + T[3] * dx[1] + T[6] * dx[2]);
+                a1 += c * (T[1] * dx[0] + T[4] * dx[1] + T[7] * dx[2]);
+                a2 += c * (T[2] * dx[0] + T[5] * dx[1] + T[8] * dx[2]);
+            };
+            if (m > 0) {
+                for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
+                    const double2 rec = p.pd16[p.cap + t];
+                    const int k = (int)__double_as_longlong(rec.y);
+                    const double2 v0 = Ls[6 * k], v1 = Ls[6 * k + 1], v2 = Ls[6 * k + 2], v3 = Ls[6 * k + 3],
+                                  v4 = Ls[6 * k + 4], v5 = Ls[6 * k + 5];
+                    const double T[9] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y, v4.x};
+                    const double dx[3] = {v4.y - X[0], v5.x - X[1], v5.y - X[2]};
+                    term(T, dx, rec.x);
+                }
+            } else {
+                const R *tw = as<R>(p.tshp);
+                for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
+                    const int k = p.tsrc[t];
+                    R T[10];
+                    load_P<R>(Pb, k, T);
+                    const double c = (double)tw[t] * (double)T[9];
+                    const double dx[3] = {p.X0[3 * k] - X[0], p.X0[3 * k + 1] - X[1], p.X0[3 * k + 2] - X[2]};
+                    const double Td[9] = {(double)T[0], (double)T[1], (double)T[2], (double)T[3], (double)T[4],
+                                          (double)T[5], (double)T[6], (double)T[7], (double)T[8]};
+                    term(Td, dx, c);
+                }
+            }
         }
     }
     a0 = group_sum<G>(a0); a1 = group_sum<G>(a1); a2 = group_sum<G>(a2);
@@ -2566,6 +2658,31 @@ __global__ void k_pd_pack8(int n, const int32_t *row_ptr, const int32_t *col, co
         out[cap + s] = make_float2((float)((double)tp[s].gx * quantum), __int_as_float(tp[s].src));
 }
 
+// peridynamics (FP64): the 16 B bond records, out[q] = {w_q V_j, j's position in
+// the block's source set (or j, for a block without a set)} in list order and
+// out[cap + t] = {w_t V_k, k's set position (or k)} in transposed order; the
+// products are the ones the gather and the force form without records.  kGroup
+// lanes per particle, blocks of npb particles
+__global__ void k_pd_pack16(int n, int npb, const int32_t *row_ptr, const int32_t *col, const int32_t *trow,
+                            const int32_t *tsrc, int64_t cap, const double *w, const double *tw, const double *vol,
+                            const uint16_t *loc, const int32_t *gset_ptr, const int32_t *fset_ptr,
+                            const TPair<double> *tp, double2 *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kGroup), lane = (int)(t % kGroup);
+    if (i >= n) return;
+    const int b = i / npb;
+    const bool gs = gset_ptr[b + 1] > gset_ptr[b], fs = fset_ptr[b + 1] > fset_ptr[b];
+    for (int q = row_ptr[i] + lane; q < row_ptr[i + 1]; q += kGroup) {
+        const int j = col[q];
+        out[q] = make_double2(w[q] * vol[j], __longlong_as_double((long long)(gs ? (int)loc[q] : j)));
+    }
+    for (int s = trow[i] + lane; s < trow[i + 1]; s += kGroup) {
+        const int k = tsrc[s];
+        out[cap + s] = make_double2(tw[s] * vol[k], __longlong_as_double((long long)(fs ? tp[s].src : k)));
+    }
+}
+
 // per block b, the quantum q_b = R_b / 2^29 with R_b the largest offset
 // |X_j - X_b|_inf of a source-set member or a particle of the block from the
 // block's first particle (1 for a block of coincident particles); one thread
@@ -2797,7 +2914,8 @@ Params params(mega_ctx *c)
     if (c->sets_on) {
         p.fset_ptr = c->fset_ptr; p.fset = c->fset;
         p.gset_ptr = c->gset_ptr; p.gset = c->gset; p.gloc = c->gloc;
-        if (!c->mixed) p.fpair2 = c->fpair2;
+        if (!c->mixed && c->pd) p.pd16 = reinterpret_cast<const double2 *>(c->fpair2);
+        else if (!c->mixed) p.fpair2 = c->fpair2;
         else if (c->pd) { p.pd8 = reinterpret_cast<const float2 *>(c->fpair); p.pd_qb = c->pd_qb; }
         else p.fpair = c->fpair;
     }
@@ -3103,13 +3221,14 @@ int build_sets(mega_ctx *c)
 int build_pd_sets(mega_ctx *c)
 {
     c->sets_on = 0;
-    if (!c->pd || !c->mixed || !c->tsrc || c->nnz == 0) return MEGA_OK;
-    const int nb = (c->n + kPdNpb - 1) / kPdNpb;
+    if (!c->pd || !c->tsrc || c->nnz == 0) return MEGA_OK;
+    const int npb = c->mixed ? kPdNpb : kNodesPerBlock;
+    const int nb = (c->n + npb - 1) / npb;
     int e = 0;
     if (!c->fset_ptr) e |= dalloc(&c->fset_ptr, nb + 1);
     if (!c->set_max) e |= dalloc(&c->set_max, 1);
     if (!c->gset_ptr) e |= dalloc(&c->gset_ptr, nb + 1);
-    if (!c->pd_qb) e |= dalloc(&c->pd_qb, nb);
+    if (c->mixed && !c->pd_qb) e |= dalloc(&c->pd_qb, nb);
     if (c->gloc_cap < c->nnz) {
         (void)hipFree(c->gloc);
         c->gloc = nullptr;
@@ -3117,19 +3236,41 @@ int build_pd_sets(mega_ctx *c)
         e |= dalloc(&c->gloc, c->nnz_cap);
         if (!e) c->gloc_cap = c->nnz_cap;
     }
-    if (c->fpair_cap < c->nnz) {
+    if (c->mixed && c->fpair_cap < c->nnz) {
         (void)hipFree(c->fpair);
         c->fpair = nullptr;
         c->fpair_cap = 0;
         e |= dalloc(&c->fpair, c->nnz_cap);
         if (!e) c->fpair_cap = c->nnz_cap;
     }
+    if (!c->mixed && c->fpair2_cap < c->nnz) {
+        (void)hipFree(c->fpair2);
+        c->fpair2 = nullptr;
+        c->fpair2_cap = 0;
+        e |= dalloc(&c->fpair2, 2 * (size_t)c->nnz_cap);
+        if (!e) c->fpair2_cap = c->nnz_cap;
+    }
     if (e) return MEGA_ERR_NOMEM;
     Scope s(c, "k_blk_set");
-    int r = build_set<float>(c, kPdNpb, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
+    if (!c->mixed) {
+        int r = build_set<double>(c, npb, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
+                                  static_cast<TPair<double> *>(c->tpair), &c->fset_max, kPdFSetCap64);
+        if (!r)
+            r = build_set<double>(c, npb, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc, nullptr,
+                                  &c->gset_max, kPdSetCap64);
+        if (r) return r;
+        k_pd_pack16<<<grid_for((int64_t)c->n * kGroup), kBlock, 0, c->stream>>>(
+            c->n, npb, c->row_ptr, c->col, c->trow, c->tsrc, c->nnz_cap, (const double *)c->shp,
+            (const double *)c->tshp, c->vol, c->gloc, c->gset_ptr, c->fset_ptr,
+            static_cast<const TPair<double> *>(c->tpair), reinterpret_cast<double2 *>(c->fpair2));
+        HIPCHK(hipGetLastError());
+        c->sets_on = 1;
+        return MEGA_OK;
+    }
+    int r = build_set<float>(c, npb, c->trow, c->tsrc, c->fset_ptr, &c->fset, &c->fset_cap, nullptr,
                              static_cast<TPair<float> *>(c->tpair), &c->fset_max, kPdSetCap);
     if (!r)
-        r = build_set<float>(c, kPdNpb, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc, nullptr,
+        r = build_set<float>(c, npb, c->row_ptr, c->col, c->gset_ptr, &c->gset, &c->gset_cap, c->gloc, nullptr,
                              &c->gset_max, kPdSetCap);
     if (r) return r;
     // the fixed-point quantum of each block's bond vectors: offsets below 2^29 units
@@ -3270,8 +3411,12 @@ int pd_kernels(mega_ctx *c, int linit, int force_mode)
     // source, T rows and the relative reference position (48 B) per force source (+ 16 B
     // per gather source with the automatic time step: FP32 current positions)
     const size_t mg = (size_t)c->gset_max;
-    const size_t lds_g = (MIX && p.gset_ptr && p.pd8) ? 32 * mg + 16 * ((mg + 3) / 4) + (p.char_dist ? 16 * mg : 0) : 0;
-    const size_t lds_f = (MIX && p.fset_ptr && p.pd8) ? (size_t)c->fset_max * 48 : 0;
+    size_t lds_g = (MIX && p.gset_ptr && p.pd8) ? 32 * mg + 16 * ((mg + 3) / 4) + (p.char_dist ? 16 * mg : 0) : 0;
+    size_t lds_f = (MIX && p.fset_ptr && p.pd8) ? (size_t)c->fset_max * 48 : 0;
+    if (!MIX && p.pd16) {   // FP64: 80 B gather rows, 96 B force rows
+        lds_g = 80 * mg;
+        lds_f = 96 * (size_t)c->fset_max;
+    }
     {
         Scope s(c, "k_pd_gather");
         if (c->mixed) k_pd_u4<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->x, c->X0, c->u4);
