@@ -1825,10 +1825,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
             // the particle); one instantiation each, so that no bond chooses between them
             auto sweep = [&](auto lds_tag) {
                 constexpr bool LDS = decltype(lds_tag)::value;
-                for (int base = q0 + lane; base < q1; base += G * kPdUnroll) {
-                    float2 rec[kPdUnroll];
-                    float4 uj[kPdUnroll], dj[kPdUnroll];
-                    float r[kPdUnroll][3];
+                auto load_r = [&](int base, float2 *rec) {
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
                         const int q = base + G * e;
@@ -1836,6 +1833,16 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                         if (LDS || rec8) rec[e] = q < q1 ? rec8[q] : make_float2(0.f, __int_as_float(pad));
                         else rec[e] = q < q1 ? make_float2(rw[q], __int_as_float(p.col[q])) : make_float2(0.f, __int_as_float(pad));
                     }
+                };
+                float2 recn[kPdUnroll];
+                if (q0 + lane < q1) load_r(q0 + lane, recn);
+                for (int base = q0 + lane; base < q1; base += G * kPdUnroll) {
+                    float2 rec[kPdUnroll];
+                    float4 uj[kPdUnroll], dj[kPdUnroll];
+                    float r[kPdUnroll][3];
+#pragma unroll
+                    for (int e = 0; e < kPdUnroll; ++e) rec[e] = recn[e];
+                    if (base + G * kPdUnroll < q1) load_r(base + G * kPdUnroll, recn);
                     // bond j: (w V_j (X_j - X_i), u_j, d_j)
 #pragma unroll
                     for (int e = 0; e < kPdUnroll; ++e) {
@@ -2103,14 +2110,21 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
             const float2 *rec8 = p.pd8 ? p.pd8 + p.cap : nullptr;
             const TPair<float> *tp = as<const TPair<float>>(p.tpair);   // {w V_k, -, -, k} before packing
             const int t0 = p.trow[node], t1 = p.trow[node + 1];
-            for (int base = t0 + lane; base < t1; base += G * kPdUnroll) {
-                float2 e[kPdUnroll];
+            auto load_e = [&](int base, float2 *e) {
 #pragma unroll
                 for (int u = 0; u < kPdUnroll; ++u) {
                     const int t = base + G * u;
                     if (rec8) e[u] = t < t1 ? rec8[t] : make_float2(0.f, 0.f);
                     else e[u] = t < t1 ? make_float2(tp[t].gx, __int_as_float(tp[t].src)) : make_float2(0.f, 0.f);
                 }
+            };
+            float2 en[kPdUnroll];
+            if (t0 + lane < t1) load_e(t0 + lane, en);
+            for (int base = t0 + lane; base < t1; base += G * kPdUnroll) {
+                float2 e[kPdUnroll];
+#pragma unroll
+                for (int u = 0; u < kPdUnroll; ++u) e[u] = en[u];
+                if (base + G * kPdUnroll < t1) load_e(base + G * kPdUnroll, en);
 #pragma unroll
                 for (int u = 0; u < kPdUnroll; ++u) {
                     const int k = __float_as_int(e[u].y);
