@@ -1932,18 +1932,36 @@ __global__ __launch_bounds__(kBlock) void k_pd_gather(Params p, int nblocks)
                     }
             };
             if (m > 0) {
-                for (int q = p.row_ptr[node] + lane; q < p.row_ptr[node + 1]; q += kGroup) {
-                    const double2 rec = p.pd16[q];
-                    const int k = (int)__double_as_longlong(rec.y);
-                    const double2 v0 = Ls[5 * k], v1 = Ls[5 * k + 1], v2 = Ls[5 * k + 2], v3 = Ls[5 * k + 3],
-                                  v4 = Ls[5 * k + 4];
-                    const double h[3] = {v0.x - X[0], v0.y - X[1], v1.x - X[2]};
-                    const double hs[3] = {v1.y - x[0], v2.x - x[1], v2.y - x[2]};
-                    const double hd[3] = {v3.x - d[0], v3.y - d[1], v4.x - d[2]};
-                    // the particle itself is the bond of zero reference length
-                    if (cd_on && (h[0] != 0.0 || h[1] != 0.0 || h[2] != 0.0))
-                        dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
-                    term(h, hs, hd, rec.x);
+                constexpr int U = 4;
+                const int q0 = p.row_ptr[node], q1 = p.row_ptr[node + 1];
+                auto load_r = [&](int base, double2 *rec) {
+#pragma unroll
+                    for (int e = 0; e < U; ++e) {
+                        const int q = base + kGroup * e;
+                        rec[e] = q < q1 ? p.pd16[q] : make_double2(0.0, 0.0);
+                    }
+                };
+                double2 recn[U];
+                if (q0 + lane < q1) load_r(q0 + lane, recn);
+                for (int base = q0 + lane; base < q1; base += kGroup * U) {
+                    double2 rec[U];
+#pragma unroll
+                    for (int e = 0; e < U; ++e) rec[e] = recn[e];
+                    if (base + kGroup * U < q1) load_r(base + kGroup * U, recn);
+#pragma unroll
+                    for (int e = 0; e < U; ++e) {
+                        if (base + kGroup * e >= q1) break;
+                        const int k = (int)__double_as_longlong(rec[e].y);
+                        const double2 v0 = Ls[5 * k], v1 = Ls[5 * k + 1], v2 = Ls[5 * k + 2], v3 = Ls[5 * k + 3],
+                                      v4 = Ls[5 * k + 4];
+                        const double h[3] = {v0.x - X[0], v0.y - X[1], v1.x - X[2]};
+                        const double hs[3] = {v1.y - x[0], v2.x - x[1], v2.y - x[2]};
+                        const double hd[3] = {v3.x - d[0], v3.y - d[1], v4.x - d[2]};
+                        // the particle itself is the bond of zero reference length
+                        if (cd_on && (h[0] != 0.0 || h[1] != 0.0 || h[2] != 0.0))
+                            dmin = fmin(dmin, sq_len_unfused(hs[0], hs[1], hs[2]));
+                        term(h, hs, hd, rec[e].x);
+                    }
                 }
             } else {
                 const R *w = as<R>(p.shp);
@@ -2175,14 +2193,32 @@ __global__ __launch_bounds__(kBlock) void k_pd_force(Params p, int nblocks)
                 a2 += c * (T[2] * dx[0] + T[5] * dx[1] + T[8] * dx[2]);
             };
             if (m > 0) {
-                for (int t = p.trow[node] + lane; t < p.trow[node + 1]; t += kGroup) {
-                    const double2 rec = p.pd16[p.cap + t];
-                    const int k = (int)__double_as_longlong(rec.y);
-                    const double2 v0 = Ls[6 * k], v1 = Ls[6 * k + 1], v2 = Ls[6 * k + 2], v3 = Ls[6 * k + 3],
-                                  v4 = Ls[6 * k + 4], v5 = Ls[6 * k + 5];
-                    const double T[9] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y, v4.x};
-                    const double dx[3] = {v4.y - X[0], v5.x - X[1], v5.y - X[2]};
-                    term(T, dx, rec.x);
+                constexpr int U = 4;
+                const int s0 = p.trow[node], s1 = p.trow[node + 1];
+                auto load_r = [&](int base, double2 *rec) {
+#pragma unroll
+                    for (int e = 0; e < U; ++e) {
+                        const int t = base + kGroup * e;
+                        rec[e] = t < s1 ? p.pd16[p.cap + t] : make_double2(0.0, 0.0);
+                    }
+                };
+                double2 recn[U];
+                if (s0 + lane < s1) load_r(s0 + lane, recn);
+                for (int base = s0 + lane; base < s1; base += kGroup * U) {
+                    double2 rec[U];
+#pragma unroll
+                    for (int e = 0; e < U; ++e) rec[e] = recn[e];
+                    if (base + kGroup * U < s1) load_r(base + kGroup * U, recn);
+#pragma unroll
+                    for (int e = 0; e < U; ++e) {
+                        if (base + kGroup * e >= s1) break;
+                        const int k = (int)__double_as_longlong(rec[e].y);
+                        const double2 v0 = Ls[6 * k], v1 = Ls[6 * k + 1], v2 = Ls[6 * k + 2], v3 = Ls[6 * k + 3],
+                                      v4 = Ls[6 * k + 4], v5 = Ls[6 * k + 5];
+                        const double T[9] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y, v4.x};
+                        const double dx[3] = {v4.y - X[0], v5.x - X[1], v5.y - X[2]};
+                        term(T, dx, rec[e].x);
+                    }
                 }
             } else {
                 const R *tw = as<R>(p.tshp);
