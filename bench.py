@@ -82,6 +82,7 @@ V0 = 220.0            # the learned return map's test speed (its training runs w
 V0_IMPACT = 190.0     # the speed of the impact-start window beside the headline
 T_END = 80e-6
 PEAK_HBM = 8.0e12
+PEAK_FP64 = 78.6e12   # MI355X vector FP64 (flop/s)
 # steps over which the 1.75M-node bar stays finite at V0 in every mode (direct
 # nodal integration has no live stabilisation; at 220 m/s the FP64 step moves
 # a node faster than 5 v0 by step 950): the prelude is cut so that the timed
@@ -223,9 +224,18 @@ def preprocessing() -> dict:
             "windows_volumes_equal": bool(np.array_equal(win, ref.win0) and np.array_equal(vol, ref.vol))}
 
 
+def rk_flops(n: int, nnz: int) -> float:
+    """Minimal FP64 flops of RK1 for all nodes (symmetric moment matrix M and
+    its three derivatives, the 4x4 inverse, the shape functions and their
+    gradients): about 250 per neighbor and 1,500 per node."""
+    return 250.0 * nnz + 1500.0 * n
+
+
 def semi_lagrangian(steps: int, warmup: int) -> dict:
     """The same bar in the semi-Lagrangian formulation (SOFT_SEARCH and RK1
-    every step, GET_BINS every 20 steps): FP64 and mixed rates."""
+    every step, GET_BINS every 20 steps): FP64 and mixed rates, and the
+    shape-function kernel against both its rooflines (HBM bytes and FP64
+    flops; its moment matrix and inverse are FP64 in both modes)."""
     prob = deck.taylor_problem(H, v0=V0_IMPACT, seed=SEED, lagrangian=0, pdsearch_steps=20)
     out = {"workload": "same bar, semi-Lagrangian, bins every 20 steps", "steps": steps}
     for name in ("fp64", "mixed"):
@@ -234,7 +244,15 @@ def semi_lagrangian(steps: int, warmup: int) -> dict:
         out[name] = {"rate": rate, "ms_per_step": 1e3 * wall / steps,
                      "kernel_ms": {k: v[0] for k, v in kt.items()},
                      "step_roofline_frac": rate * semi_step_bytes(prob.n, nnz, name != "fp64") / PEAK_HBM}
-        log(f"semi-Lagrangian {name}: {1e3 * wall / steps:.3f} ms/step, {rate:.4e} node-steps/s")
+        if "k_rk" in kt:
+            s = kt["k_rk"][0] * 1e-3
+            out[name]["k_rk"] = {"ms": kt["k_rk"][0],
+                                 "hbm_frac": kernel_bytes("k_rk", prob.n, nnz, name != "fp64") / s / PEAK_HBM,
+                                 "fp64_flops": rk_flops(prob.n, nnz),
+                                 "fp64_frac": rk_flops(prob.n, nnz) / s / PEAK_FP64}
+        log(f"semi-Lagrangian {name}: {1e3 * wall / steps:.3f} ms/step, {rate:.4e} node-steps/s"
+            + (f"; k_rk {out[name]['k_rk']['hbm_frac']:.3f} of HBM, {out[name]['k_rk']['fp64_frac']:.3f} of FP64"
+               if "k_rk" in out[name] else ""))
     return out
 
 
