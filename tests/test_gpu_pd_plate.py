@@ -8,6 +8,10 @@ lateral faces clamped, a 5 mm-radius plug under the plate's centre given
   MAIN loop over 20 steps: bond lists equal in the reference's order, FP64
   positions within 1e-12, velocity, force and stress within 1e-9; mixed
   precision (FP32 bond forces, FP64 integration) within the stated bars.
+* A small plate with a horizon of five spacings (~500 bonds per particle):
+  the blocks' lists exceed what the set builder sorts (4,096 entries), so
+  the gather and the force read their sources from global memory instead of
+  a staged set; against the oracle over 10 steps with the same bars.
 * The full plate, FP64: one CONSTRUCT_FINT_PD call under a uniform affine
   deformation x = A X.  The deformation gradient S K^-1 reproduces A at
   every particle, whatever its bond family, so the stress is one uniform
@@ -58,6 +62,32 @@ def test_reduced_plate_matches_oracle(lib, oracle, precision):
         # measured: x 3.4e-10, v 1.2e-7, fint 2.5e-7, stress 3.1e-7, force sum 3.4e-10
         assert errs["x"] < 4e-9 and errs["v"] < 1.5e-6 and errs["fint"] < 2e-6 and errs["stress"] < 3e-6
         assert fsum < 3e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_horizon_reads_global_memory(lib, oracle, precision):
+    prob = deck.pd_plate_problem(lx=4e-3, ly=4e-3, lz=1.6e-3, impact_radius=1e-3, horizon=5.0)
+    nsteps = 10
+    g = mega.Mega(prob, precision=mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
+    g.step(nsteps)
+    nnz, _, _ = g.stats()
+    kbar = nnz / prob.n
+    assert 16 * kbar > 4096   # the blocks of both precisions beyond the set builder's sort
+    run = OracleRun(oracle, prob, nthreads=8).step(nsteps)
+    A = run.arr
+    errs = {k: rel_err(g.download(k), A[r]) for k, r in
+            (("x", "x"), ("v", "v"), ("fint", "fint"), ("stress", "stress"))}
+    fsum = float(np.abs(g.download("fint").sum(0)).max() / np.abs(g.download("fint")).sum(0).max())
+    g.close()
+    print(f"wide horizon: {prob.n} particles, {kbar:.0f} bonds each, {precision}, {nsteps} steps vs oracle: {errs}; "
+          f"|sum fint| / sum |fint| {fsum:.1e}", flush=True)
+    if precision == "fp64":
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert fsum < 1e-12
+    else:
+        # measured: x 1.7e-10, v 6.0e-8, fint 1.2e-7, stress 1.5e-7, force sum 2.1e-9
+        assert errs["x"] < 2e-9 and errs["v"] < 6e-7 and errs["fint"] < 1.2e-6 and errs["stress"] < 1.5e-6
+        assert fsum < 2e-8
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
