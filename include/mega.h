@@ -118,7 +118,9 @@ int mega_interp(mega_ctx *ctx, const double *coef, double *phys);
  * peridynamics one CONSTRUCT_FINT_PD call (FINT_PD.F90:14-759; fext is then
  * zero and int_work unchanged, as the reference passes neither), constitution
  * included: inputs dinc[3n] (GDINC), dinc_tot[3n] (GDINC_TOT), x_cur[3n]
- * (GCOO_CUURENT); in/out stress[6n], strain[6n], state[20n]; out fint[3n],
+ * (GCOO_CUURENT); in/out stress[6n], strain[6n], state[20n] (GSTATE: of its
+ * 20 columns the step reads and writes STATE(3) = EPS and the damage models'
+ * STATE(1), (2), (4), (5); the others are left as they are); out fint[3n],
  * fext[3n]; int_work accumulates (GINT_WORK).  linit=1 computes and stores
  * shapes as the first call does. */
 int mega_fint(mega_ctx *ctx, int32_t linit, const double *dinc, const double *dinc_tot,

@@ -2037,6 +2037,19 @@ __global__ __launch_bounds__(kBlock) void k_pd_node(Params p)
     } else if (mt.type == 1) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) sr[k] = pred[k];
+    } else if (mt.type == 6 || mt.type == 3) {
+        // CONSTITUTION (FINT_PD.F90:493) as on the RK path: the damage models take
+        // the rotated stress, the total strain LSTRAIN and the increment
+        R *st4 = as<R>(p.st4) + 4 * (int64_t)i;
+        R es = st4[0], dam = st4[2], dsv = st4[3], tot[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tot[k] = (R)e6[k];
+        if (mt.type == 6) {
+            if (mega::von_mises_dam(sr, tot, der, ep, dam, dsv, mt) < 0) atomicAdd(&p.flags[0], 1);
+        } else {
+            mega::druck_prag(sr, tot, der, es, ep, dam, dsv, mt);
+        }
+        st4[0] = es; st4[2] = dam; st4[3] = dsv;
     } else {
         atomicAdd(&p.flags[1], 1);
     }
@@ -2062,7 +2075,11 @@ __global__ __launch_bounds__(kBlock) void k_pd_node(Params p)
         if constexpr (!std::is_same<R, float>::value) si[k] = sig[k] - old_s[k];
         ei[k] = e6[k] - old_s[k];   // the reference's STRAIN_INC = LSTRAIN - GSTRESS (FINT_PD.F90:509)
     }
-    if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
+    if (p.wave_vel) {
+        double wv = mega::wave_speed(si, ei, mt);
+        if (mt.type == 3) wv = wv / 0.15;   // the reference's Drucker-Prager factor (FINT_PD.F90:539-542)
+        p.wave_vel[i] = wv;
+    }
     // T = det sigma IF^T K^-1 (FINT_PD.F90:620-624)
     const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
     double IFt[9], PK[9], T[9];
@@ -4027,11 +4044,17 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
         return MEGA_ERR_ARG;
     const int n = c->n;
     const size_t b3 = sizeof(double) * 3 * n;
-    std::vector<double> eps0(n);
-    for (int i = 0; i < n; ++i) eps0[i] = state[20 * (int64_t)i + 2];
+    // STATE(3) = EPS, and the damage models' STATE(1), (2), (4), (5) (st4)
+    constexpr int kDam[4] = {0, 1, 3, 4};
+    std::vector<double> eps0(n), dam0(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        eps0[i] = state[20 * (int64_t)i + 2];
+        for (int k = 0; k < 4; ++k) dam0[4 * (size_t)i + k] = state[20 * (int64_t)i + kDam[k]];
+    }
     const auto di = to_internal(c, dinc, 3), ti = to_internal(c, dinc_tot, 3);
     const auto si = to_internal(c, stress, 6), ei = to_internal(c, strain, 6);
     auto eps = to_internal(c, eps0.data(), 1);
+    auto dam = to_internal(c, dam0.data(), 4);
     std::vector<double> xi;
     HIPCHK(hipMemcpyAsync(c->d, di.data(), b3, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->dtot, ti.data(), b3, hipMemcpyHostToDevice, c->stream));
@@ -4045,6 +4068,8 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     if (!r) r = state_in(c, c->strain, 6 * (int64_t)n, ei.data());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_in(c, c->eps, n, eps.data());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_in(c, c->st4, 4 * (int64_t)n, dam.data());
     if (r) return r;
     HIPCHK(hipMemsetAsync(c->flags, 0, 16, c->stream));
     int err = 0;
@@ -4068,6 +4093,8 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     if (!r) r = state_out(c, c->strain, 6 * (int64_t)n, strain);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!r) r = state_out(c, c->eps, n, eps.data());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!r) r = state_out(c, c->st4, 4 * (int64_t)n, dam.data());
     if (r) return r;
     HIPCHK(hipMemcpyAsync(fint, c->fint, b3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(fext, c->fext, b3, hipMemcpyDeviceToHost, c->stream));
@@ -4075,9 +4102,13 @@ int mega_fint(mega_ctx *c, int32_t linit, const double *dinc, const double *dinc
     to_caller(c, stress, 6);
     to_caller(c, strain, 6);
     to_caller(c, eps.data(), 1);
+    to_caller(c, dam.data(), 4);
     to_caller(c, fint, 3);
     to_caller(c, fext, 3);
-    for (int i = 0; i < n; ++i) state[20 * (int64_t)i + 2] = eps[i];
+    for (int i = 0; i < n; ++i) {
+        state[20 * (int64_t)i + 2] = eps[i];
+        for (int k = 0; k < 4; ++k) state[20 * (int64_t)i + kDam[k]] = dam[4 * (size_t)i + k];
+    }
     if (int_work) *int_work += w;
     c->predicted = 0;
     return st;

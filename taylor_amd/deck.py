@@ -92,6 +92,22 @@ class Material:
         p[0], p[1], p[2] = nu, E, rho
         return Material(1, fill_lame(p))
 
+    def in_mm(self) -> "Material":
+        """The same material in mm / tonne / s / MPa: moduli and stresses x1e-6,
+        density x1e-12; strains, angles, exponents and damping unchanged."""
+        p = self.props
+        E, nu, rho, damping = p[1] * 1e-6, p[0], p[2] * 1e-12, p[20]
+        if self.mat_type == 1:
+            return Material.elastic(E, nu, rho)
+        if self.mat_type == 2:
+            return Material.j2(E, nu, rho, p[3] * 1e-6, p[4], p[5], damping)
+        if self.mat_type == 6:
+            return Material.j2_damage(E, nu, rho, p[3] * 1e-6, p[4], p[5], p[8], p[9], p[10], damping)
+        if self.mat_type == 3:
+            return Material.drucker_prager(E, nu, rho, int(p[3]), p[4], p[5] * 1e-6, p[6], p[7] * 1e-6, p[8], p[9],
+                                           p[10], damping)
+        raise ValueError(f"material type {self.mat_type}: no unit conversion")
+
 
 def drucker_prager_prep(p: np.ndarray) -> np.ndarray:
     """Props 25-27 = Q_PHI, K_PHI, Q_PSI of the Drucker-Prager cone from
@@ -339,11 +355,7 @@ def taylor_problem(h: float, v0: float = 190.0, seed: int = 0, norm_win: float =
     (stabilized non-conforming) gradients, Lagrangian only."""
     if mm_units:
         h, radius, length, v0 = h * 1e3, radius * 1e3, length * 1e3, v0 * 1e3
-        base = material or copper_linear()
-        p0 = base.props
-        material = Material.j2(E=p0[1] * 1e-6, nu=p0[0], rho=p0[2] * 1e-12, sig0=p0[3] * 1e-6, B=p0[4], CE=p0[5],
-                               damping=p0[20]) if base.mat_type == 2 else Material.elastic(p0[1] * 1e-6, p0[0],
-                                                                                           p0[2] * 1e-12)
+        material = (material or copper_linear()).in_mm()
     mesh = taylor_bar_mesh(h, radius=radius, length=length, seed=seed)
     mat = material or copper_linear()
     prob = assemble(mesh, {"BAR": (0, 0, 0)} | {"WALL": (0, 0, 1)}, {"BAR": (0.0, 0.0, -v0)}, mat,

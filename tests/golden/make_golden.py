@@ -11,6 +11,7 @@ compressed .npz files.  The fixtures are data only.
     python tests/golden/make_golden.py --pd-auto  # peridynamics with the automatic time step
     python tests/golden/make_golden.py --final    # the whole 80 us bar at 0.5 mm
     python tests/golden/make_golden.py --damage   # material types 6 and 3
+    python tests/golden/make_golden.py --pd-damage   # peridynamics with material types 6 and 3
     python tests/golden/make_golden.py --vtk      # the reference's VTK frames of a small deck
     python tests/golden/make_golden.py --trajectory [--semi]   # 181k-node bar, 1,400 / 500 steps, 8 threads
 """
@@ -465,6 +466,22 @@ def pd_main():
         np.savez_compressed(os.path.join(HERE, "run_pd.npz"), **mega_run(prob, mesh, 40, (1, 10, 40), wd))
 
 
+def pd_damage_main():
+    """The small peridynamic bar (mm deck) with the damage materials, 40 steps
+    of the reference program, frames 1, 10 and 40 with damage and shear strain."""
+    mesh = deck.taylor_bar_mesh(H * 1e3, radius=3.8e-3 * 1e3, length=LEN * 1e3, seed=0)
+    for mt, name in ((6, "run_pd_dam6.npz"), (3, "run_pd_dp3.npz")):
+        prob = small(peridynamics=1, mm_units=True, material=damage_material(mt))
+        assert np.array_equal(mesh.coo, prob.coo) and prob.mat_types()[0] == mt
+        with tempfile.TemporaryDirectory() as wd:
+            np.savez_compressed(os.path.join(HERE, name),
+                                **mega_run(prob, mesh, 40, (1, 10, 40), wd, extra=("dam", "eps_shear")))
+    # the automatic step with the Drucker-Prager wave-speed factor (FINT_PD.F90:539-542)
+    prob = small(peridynamics=1, mm_units=True, material=damage_material(3))
+    np.savez_compressed(os.path.join(HERE, "run_pd_dp3_auto.npz"),
+                        **pd_auto_run(prob, PD_DLT_FAC, extra=("dam", "eps_shear")))
+
+
 PD_DLT_FAC = 0.3
 
 
@@ -475,7 +492,6 @@ def pd_auto_main():
     every step's TIME (TIME.OUT, MAIN.F90:653) and frames 1, 10 and 40.  The
     deck's TIME_END lies midway between the oracle's times after steps 39 and
     40; the program's own TIME.OUT says how many steps it took."""
-    from oracle_bind import OracleRun, load_oracle
     ref = load_ref()
     g = np.load(os.path.join(HERE, "pd.npz"))
     out = {}
@@ -510,23 +526,32 @@ def pd_auto_main():
                         "call_wave": wv, "call_f": f})
         finally:
             os.chdir(cwd)
-    prob = small(peridynamics=1, mm_units=True)
-    run = OracleRun(load_oracle(), prob, dlt_fac=PD_DLT_FAC).step(40)
+    out.update(pd_auto_run(small(peridynamics=1, mm_units=True), PD_DLT_FAC))
+    np.savez_compressed(os.path.join(HERE, "pd_auto.npz"), **out)
+
+
+def pd_auto_run(prob, dlt_fac, extra=()):
+    """40 steps of the reference program on the small peridynamic bar (mm deck)
+    with IAUTO_TS = 1: TIME_END midway between the oracle's times after steps
+    39 and 40, every step's TIME from TIME.OUT, frames 1, 10 and 40."""
+    from oracle_bind import OracleRun, load_oracle
+    run = OracleRun(load_oracle(), prob, dlt_fac=dlt_fac).step(40)
     t = np.cumsum(run.dlt)
     mesh = deck.taylor_bar_mesh(H * 1e3, radius=3.8e-3 * 1e3, length=LEN * 1e3, seed=0)
+    out = {}
     with tempfile.TemporaryDirectory() as wd:
         kfile = os.path.join(wd, "bar.k")
         deck.write_k(kfile, mesh)
         prob.time_end = 0.5 * (t[38] + t[39])
         deck.write_control(os.path.join(wd, "Control.dat"), kfile, prob, ["BAR", "WALL"], time_output=-1.0,
-                           out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps",
-                           dlt_fac=PD_DLT_FAC)
+                           out_vars="unformatted\ndisplacement\nvelocity\nfint\nstress\nstrain\neps" +
+                           "".join("\n" + v for v in extra), dlt_fac=dlt_fac)
         with open(os.path.join(wd, "mega.out"), "w") as fo:
             subprocess.run([REF_MEGA, "-j", "1"], cwd=wd, stdin=subprocess.DEVNULL, stdout=fo, stderr=fo,
                            check=True, timeout=3600)
         with open(os.path.join(wd, "TIME.OUT")) as f:
             times = [float(ln.split()[0].replace("D", "E")) for ln in f if ln.strip()]
-        out.update({"run_dlt_fac": np.array([PD_DLT_FAC]), "run_time_end": np.array([prob.time_end]),
+        out.update({"run_dlt_fac": np.array([dlt_fac]), "run_time_end": np.array([prob.time_end]),
                     "run_time": np.array(times)})
         for k in (1, 10, 40):
             fr = read_unformatted_vtk(os.path.join(wd, "MEGAOUT_%04d.vtk" % k))
@@ -536,7 +561,9 @@ def pd_auto_main():
             out[f"step{k}_fint"] = np.stack([fr["fint_x"], fr["fint_y"], fr["fint_z"]], 1)
             out[f"step{k}_stress"] = np.stack([fr[f"stress_{c}"] for c in ("xx", "yy", "zz", "yz", "xz", "xy")], 1)
             out[f"step{k}_eps"] = fr["eps"]
-    np.savez_compressed(os.path.join(HERE, "pd_auto.npz"), **out)
+            for v in extra:
+                out[f"step{k}_{v}"] = fr[v]
+    return out
 
 
 def main():
@@ -546,6 +573,9 @@ def main():
     if "--final" in sys.argv:
         with tempfile.TemporaryDirectory() as wd:
             np.savez_compressed(os.path.join(HERE, "taylor_h05_final.npz"), **final_state_vectors(wd))
+        return
+    if "--pd-damage" in sys.argv:
+        pd_damage_main()
         return
     if "--pd" in sys.argv:
         pd_main()

@@ -99,3 +99,65 @@ def test_damage_steps_mixed_precision(mega, mat_type):
     ed = float(np.abs(d32[:, 2] - d64[:, 2]).max())
     print("mixed vs FP64: x", ex, "damage", ed)
     assert ex < 1e-6 and ed < 2e-2
+
+
+@pytest.mark.parametrize("mat_type", [6, 3])
+@pytest.mark.parametrize("pd", [0, 1])
+def test_construct_fint_carries_the_damage_state(mega, oracle, mat_type, pd):
+    """One CONSTRUCT_FINT (or CONSTRUCT_FINT_PD) call through mega_fint on a
+    deformed state with damage under way: the caller's STATE(1), (2), (4), (5)
+    go in and come back updated, as GSTATE does in the reference
+    (FINT.F90:809-821, FINT_PD.F90:497-503), against the oracle."""
+    from oracle_bind import oracle_fint, oracle_fint_pd, oracle_search
+    mm = bool(pd)   # the peridynamic deck in mm / tonne / s / MPa
+    prob = deck.taylor_problem(0.7e-3, length=6e-3, seed=0, pdsearch_steps=3, peridynamics=pd, mm_units=mm,
+                               material=damage_material(mat_type))
+    n = prob.n
+    L = 1e3 if mm else 1.0
+    S = (1e-6 if mm else 1.0) * (3e8 if mat_type == 6 else 2e7)
+    rng = np.random.default_rng(40 + mat_type + 2 * pd)
+    m = mega.Mega(prob)
+    m.search()
+    rp, col, _ = oracle_search(oracle, prob)
+    nnz = int(rp[-1])
+    z = np.zeros((n, 3))
+    zs = lambda w: np.zeros((n, w))   # noqa: E731
+    if pd:
+        omega, invk = np.zeros(nnz), np.zeros((n, 9))
+        oracle_fint_pd(oracle, prob, rp, col, omega, invk, 1, z, z, prob.coo, zs(6), zs(6), zs(20))
+    else:
+        shp, dshp = np.zeros(nnz), np.zeros((nnz, 3))
+        oracle_fint(oracle, prob, rp, col, shp, dshp, 1, z, z, prob.coo, zs(6), zs(6), zs(20))
+    m.construct_fint(1, z, z, prob.coo, zs(6), zs(6), zs(20))
+    dtot = rng.normal(size=(n, 3)) * 2e-6 * L
+    dinc = (rng.normal(size=(n, 3)) * 4e-7 + np.array([0, 0, -1e-6])) * L
+    stress = rng.normal(size=(n, 6)) * S
+    strain = rng.normal(size=(n, 6)) * (0.01 if mat_type == 6 else 0.004)   # total strain norms within (K_I, K_C)
+    state = np.zeros((n, 20))
+    state[:, 0] = np.abs(rng.normal(size=n)) * 0.01    # EPS_SHEAR
+    state[:, 1] = rng.normal(size=n) * 1e-3             # EPS_VOL (carried unchanged)
+    state[:, 2] = np.abs(rng.normal(size=n)) * 0.02     # EPS
+    state[:, 3] = rng.uniform(0.0, 0.5, size=n)         # DAM
+    state[:, 4] = np.abs(rng.normal(size=n)) * 0.01     # DAM_SV
+    x_cur = prob.coo + dtot
+    s_r, e_r, st_r = stress.copy(), strain.copy(), state.copy()
+    if pd:
+        f_r, _, _ = oracle_fint_pd(oracle, prob, rp, col, omega, invk, 0, dinc, dtot, x_cur, s_r, e_r, st_r)
+    else:
+        f_r, _, _, _ = oracle_fint(oracle, prob, rp, col, shp, dshp, 0, dinc, dtot, x_cur, s_r, e_r, st_r)
+    s_g, e_g, st_g = stress.copy(), strain.copy(), state.copy()
+    f_g, _, _, status = m.construct_fint(0, dinc, dtot, x_cur, s_g, e_g, st_g)
+    m.close()
+    errs = {"fint": rel_err(f_g, f_r), "stress": rel_err(s_g, s_r)}
+    for k, name in ((0, "eps_shear"), (1, "eps_vol"), (2, "eps"), (3, "dam"), (4, "dam_sv")):
+        errs[name] = rel_err(st_g[:, k], st_r[:, k])
+    changed = {name: int((st_r[:, k] != state[:, k]).sum()) for k, name in ((0, "eps_shear"), (3, "dam"), (4, "dam_sv"))}
+    print(f"type {mat_type}, {'PD' if pd else 'RK'}: status {status}, changed {changed}, "
+          + " ".join(f"{a} {b:.1e}" for a, b in errs.items()), flush=True)
+    assert status == 0
+    assert changed["dam"] > n // 10 and changed["dam_sv"] > n // 10
+    if mat_type == 3:
+        assert changed["eps_shear"] > n // 10
+    assert np.array_equal(st_g[:, 1], state[:, 1])
+    for k, v in errs.items():
+        assert v < 1e-10, (k, v)

@@ -170,3 +170,88 @@ def test_pd_mixed_precision_steps():
     print("mixed vs FP64 after 40 steps:", {k: f"{v:.2e}" for k, v in errs.items()})
     # measured: x 1.2e-8, uphy 1.2e-7, v 1.5e-7, stress 2.1e-7
     assert errs["x"] < 7e-8 and errs["uphy"] < 7e-7 and errs["v"] < 1.3e-6 and errs["stress"] < 2.5e-6
+
+
+@pytest.mark.parametrize("mat_type,name", [(6, "run_pd_dam6.npz"), (3, "run_pd_dp3.npz")])
+def test_pd_damage_steps_match_reference_program(mat_type, name):
+    """The damage material types on the peridynamic path (CONSTITUTION called
+    from FINT_PD.F90:493): 40 FP64 steps against the reference program's frames
+    1, 10, 40, damage and shear plastic strain included."""
+    from damage_util import damage_material
+    g = load(name)
+    prob = small(mm_units=True, material=damage_material(mat_type))
+    m = mega.Mega(prob)
+    done = 0
+    for k in (1, 10, 40):
+        assert m.step(k - done) == 0
+        done = k
+        ds = m.download("damage_state")
+        errs = {"x": rel_err(m.download("x"), g[f"step{k}_x"]),
+                "v": rel_err(m.download("v") - 0.5 * prob.dt * m.download("a"), g[f"step{k}_vphy"]),
+                "fint": rel_err(m.download("fint"), g[f"step{k}_fint"]),
+                "stress": rel_err(m.download("stress"), g[f"step{k}_stress"]),
+                "eps": float(np.abs(m.download("eps") - g[f"step{k}_eps"]).max()),
+                "dam": float(np.abs(ds[:, 2] - g[f"step{k}_dam"]).max()),
+                "eps_shear": float(np.abs(ds[:, 0] - g[f"step{k}_eps_shear"]).max())}
+        print(f"type {mat_type} step {k}: " + " ".join(f"{a} {b:.2e}" for a, b in errs.items()), flush=True)
+        assert errs["x"] < 1e-12 and errs["v"] < 1e-9 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9
+        assert errs["eps"] < 1e-9 and errs["dam"] < 1e-9 and errs["eps_shear"] < 1e-9
+    m.close()
+
+
+@pytest.mark.parametrize("mat_type", [6, 3])
+def test_pd_damage_mixed_precision_steps(mat_type):
+    """Mixed precision (FP32 stress, return maps and damage state) against FP64
+    over 40 peridynamic steps with the damage materials."""
+    from damage_util import damage_material
+    prob = small(mm_units=True, material=damage_material(mat_type))
+    out = {}
+    for prec in (mega.MEGA_FP64, mega.MEGA_MIXED):
+        m = mega.Mega(prob, precision=prec)
+        assert m.step(40) == 0
+        out[prec] = {k: m.download(k) for k in ("x", "v", "stress")}
+        out[prec]["dam"] = m.download("damage_state")[:, 2]
+        m.close()
+    a, b = out[mega.MEGA_MIXED], out[mega.MEGA_FP64]
+    errs = {k: rel_err(a[k], b[k]) for k in ("x", "v", "stress")}
+    errs["dam"] = float(np.abs(a["dam"] - b["dam"]).max())
+    print(f"type {mat_type}, mixed vs FP64 after 40 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    # about ten times the measured errors: type 6 x 1.2e-8, v 1.8e-7, stress 4.1e-7, damage 1.7e-6;
+    # type 3 x 9.7e-9, v 6.3e-7, stress 2.0e-6, damage 6.9e-6
+    bars = {6: {"x": 1.2e-7, "v": 2e-6, "stress": 5e-6, "dam": 2e-5},
+            3: {"x": 1e-7, "v": 7e-6, "stress": 2.5e-5, "dam": 7e-5}}[mat_type]
+    for k, bar in bars.items():
+        assert errs[k] < bar, (k, errs[k], bar)
+
+
+def test_pd_drucker_prager_automatic_time_step_matches_reference_program():
+    """IAUTO_TS = 1 with material type 3, whose wave speed the reference divides
+    by 0.15 (FINT_PD.F90:539-542): every step's DLT and TIME and frames 1, 10,
+    40 against the reference program (tests/golden/run_pd_dp3_auto.npz)."""
+    from damage_util import damage_material
+    a = load("run_pd_dp3_auto.npz")
+    fac = float(a["run_dlt_fac"][0])
+    ref_dlt = np.diff(np.concatenate([[0.0], a["run_time"]]))
+    prob = small(mm_units=True, material=damage_material(3))
+    m = mega.Mega(prob)
+    done, dlt = 0, []
+    for k in (1, 10, 40):
+        st, d = m.step_auto(fac, k - done)
+        assert st == 0
+        dlt.extend(d.tolist())
+        done = k
+        errs = {"x": rel_err(m.download("x"), a[f"step{k}_x"]),
+                "fint": rel_err(m.download("fint"), a[f"step{k}_fint"]),
+                "stress": rel_err(m.download("stress"), a[f"step{k}_stress"]),
+                "dam": float(np.abs(m.download("damage_state")[:, 2] - a[f"step{k}_dam"]).max())}
+        print(f"step {k}: " + " ".join(f"{q} {e:.2e}" for q, e in errs.items()), flush=True)
+        assert errs["x"] < 1e-12 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9 and errs["dam"] < 1e-9
+    m.close()
+    t, tt = [], 0.0
+    for d in dlt:
+        tt = tt + d
+        t.append(tt)
+    e_dlt = float(np.abs(np.array(dlt) - ref_dlt).max() / ref_dlt.max())
+    e_t = float(np.abs(np.array(t) - a["run_time"]).max() / a["run_time"].max())
+    print(f"DLT {e_dlt:.2e}, TIME {e_t:.2e}", flush=True)
+    assert e_dlt < 1e-14 and e_t < 1e-14

@@ -276,3 +276,51 @@ def test_whole_run_damage_bit_exact(oracle, mat_type, name):
         assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
         assert np.array_equal(A["state"][:, 3], g[f"step{k}_dam"])
         assert np.array_equal(A["state"][:, 0], g[f"step{k}_eps_shear"])
+
+
+@pytest.mark.parametrize("mat_type,name", [(6, "run_pd_dam6.npz"), (3, "run_pd_dp3.npz")])
+def test_whole_run_pd_damage_bit_exact(oracle, mat_type, name):
+    """40 steps of the reference program with *PERIDYNAMICS 1 (mm deck) and
+    material type 6 / 3: CONSTRUCT_FINT_PD passes the damage models the same
+    arguments as CONSTRUCT_FINT (FINT_PD.F90:493); frames 1, 10, 40 including
+    the damage and the shear plastic strain."""
+    from damage_util import damage_material
+    g = load(name)
+    run = OracleRun(oracle, small(peridynamics=1, mm_units=True, material=damage_material(mat_type)))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        assert np.array_equal(A["x"], g[f"step{k}_x"])
+        assert np.array_equal(A["vphy"], g[f"step{k}_vphy"])
+        assert np.array_equal(A["fint"], g[f"step{k}_fint"])
+        assert np.array_equal(A["stress"], g[f"step{k}_stress"])
+        assert np.array_equal(A["state"][:, 2], g[f"step{k}_eps"])
+        assert np.array_equal(A["state"][:, 3], g[f"step{k}_dam"])
+        assert np.array_equal(A["state"][:, 0], g[f"step{k}_eps_shear"])
+    assert (g["step40_dam"] > 0).sum() > 100   # damage under way in many particles
+
+
+def test_whole_run_pd_drucker_prager_automatic_time_step_bit_exact(oracle):
+    """40 steps of the reference program with *PERIDYNAMICS 1, material type 3
+    and IAUTO_TS = 1: the Drucker-Prager wave speed is divided by 0.15
+    (FINT_PD.F90:539-542) before the step is formed; every step's TIME and
+    frames 1, 10, 40 bit-exact."""
+    from damage_util import damage_material
+    a = load("run_pd_dp3_auto.npz")
+    run = OracleRun(oracle, small(peridynamics=1, mm_units=True, material=damage_material(3)),
+                    dlt_fac=float(a["run_dlt_fac"][0]))
+    done = 0
+    for k in (1, 10, 40):
+        run.step(k - done)
+        done = k
+        A = run.arr
+        for key, ref in (("x", "x"), ("vphy", "vphy"), ("fint", "fint"), ("stress", "stress")):
+            assert np.array_equal(A[key], a[f"step{k}_{ref}"])
+        assert np.array_equal(A["state"][:, 3], a[f"step{k}_dam"])
+    t, tt = [], 0.0
+    for d in run.dlt:
+        tt = tt + d
+        t.append(tt)
+    assert len(a["run_time"]) == 40 and np.array_equal(np.array(t), a["run_time"])
