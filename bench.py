@@ -102,16 +102,19 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
     r: bytes of the shape/stress type; node masters are FP64."""
     r = 4 if mixed else 8
     if name == "k_gather":
-        # pairs: col + phi + grad phi; nodes: own d (FP64) read, uphy r/w, X0 read,
-        # x write, gather sums write (18 or 21 values), plus dtot own for F
-        return nnz * (4 + r + 3 * r) + n * (24 + 48 + 24 + 24 + (21 if mixed else 18) * r + (12 if mixed else 0))
+        # pairs: col + phi + grad phi; nodes: own d (FP64) read, uphy r/w, gather sums
+        # write (18 or 21 values), plus dtot own for F.  x = X0 + u is stored only on
+        # a call's last step (nothing in the step reads it)
+        return nnz * (4 + r + 3 * r) + n * (24 + 48 + (21 if mixed else 18) * r + (12 if mixed else 0))
     if name == "k_node":
         # sums read, stress/strain r/w, eps r/w, vol, mat id, P write, (d, dtot own in mixed)
         return n * ((21 if mixed else 18) * r + 4 * 6 * r + 2 * r + 8 + 4 + 10 * r + (24 if mixed else 8))
     if name == "k_force":
-        # pairs: source + grad phi (transposed); nodes: P read once, fint write, mass, ebc,
-        # mat id, v r/w, a write, d write, dtot r/w (+FP32 copies of d, dtot in mixed)
-        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 + (24 if mixed else 0))
+        # pairs: source + grad phi (transposed); nodes: P read once, mass, ebc, mat id,
+        # v r/w, d write, dtot r/w (+FP32 copies of d, dtot in mixed).  fint, fext and a
+        # are stored only on a call's last steps (the next predictor is fused, so
+        # nothing in the step reads them)
+        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 12 + 4 + 48 + 24 + 48 + (24 if mixed else 0))
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
     if name == "k_search_count":

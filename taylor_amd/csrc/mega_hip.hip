@@ -226,6 +226,11 @@ struct Params {
     const double *dt_dev;                            // automatic time step: this step's DLT (device), else dt
     double *char_dist;                               // automatic time step: the PD gather's shortest bond
                                                      // per particle (GCHAR_DIST), else null
+    // stores of values that no kernel of the step reads: out_dof, fint / a / fext
+    // in the corrector; out_x, x = X0 + u in the Lagrangian gather.  Within
+    // mega_step they are made on the steps whose values are read (a by the
+    // last step's DO_INTERP, the rest by the caller after the call)
+    int out_dof, out_x;
     // correspondence peridynamics
     int pd, pd_linit;
     double *Kinv;                                    // 9 per node, inverse shape tensor
@@ -1063,7 +1068,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Params p, int nblocks)
         const double du = MIX ? p.d[qd] + (double)uk : (double)uk;
         const double t = p.uphy[qd] + du;
         p.uphy[qd] = t;
-        p.x[qd] = p.X0[qd] + t;
+        if (p.out_x) p.x[qd] = p.X0[qd] + t;
     }
 }
 
@@ -1458,11 +1463,12 @@ template <bool CORRECT, bool PREDICT>
 __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn &in, double f, double fx, double &ke,
                                         double &ie, double &ee)
 {
-    p.fint[q] = f;
     if (!CORRECT) {
+        p.fint[q] = f;
         p.fext[q] = fx;
         return;
     }
+    if (p.out_dof) p.fint[q] = f;
     const double dt = step_dt(p);
     double v = in.v, acc = 0.0;
     if (in.ebc == 1) {
@@ -1473,8 +1479,10 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
         acc = (fx - f - c * v) / (in.mass + 0.5 * dt * c);
         v = v + 0.5 * dt * acc;
     }
-    p.a[q] = acc;
-    p.fext[q] = fx;
+    if (p.out_dof) {
+        p.a[q] = acc;
+        p.fext[q] = fx;
+    }
     if (p.has_gravity) p.fext_nmo[q] = fx;
     ke = 0.5 * in.mass * v * v;
     if (!p.finite_strain) ie = 0.5 * f * in.dtot;
@@ -2925,6 +2933,7 @@ struct mega_ctx {
     unsigned long long *dt_min = nullptr;
     int64_t dt_hist_cap = 0;
     int has_gravity = 0;
+    int out_dof = 1, out_x = 1;   // see Params
     std::vector<Material> host_mats;
     // learned return map
     int surr_on = 0, surr_mat = -1;
@@ -3006,6 +3015,7 @@ Params params(mega_ctx *c)
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
     p.pd = c->pd; p.pd_linit = 0;
     p.Kinv = c->Kinv; p.gvec = c->gvec;
+    p.out_dof = c->out_dof; p.out_x = c->out_x;
     return p;
 }
 
@@ -4198,6 +4208,10 @@ static int run_steps(mega_ctx *c, int32_t nsteps)
                 }
             }
         }
+        // a is read by the last step's DO_INTERP above, fint, fext and the
+        // Lagrangian x by the caller after the call
+        c->out_dof = s >= nsteps - 2;
+        c->out_x = last || c->pd || !lag;
         if (c->pd) {
             c->auto_step = s;
             r = do_pd_fint(c, 0, last ? 1 : 2);   // DO_INTERP is the identity (HANDELER.F90:134-137)
@@ -4219,6 +4233,7 @@ static int run_steps(mega_ctx *c, int32_t nsteps)
                 if (!r) r = do_fint(c, F_NONE, false, last ? 1 : 2, true);
             }
         }
+        c->out_dof = c->out_x = 1;
         if (r) return r;
         c->predicted = last ? 0 : 1;
         HIPCHK(hipGetLastError());

@@ -316,20 +316,24 @@ def test_steps_without_ql_and_with_elastic_material(mega, oracle, case, lagrangi
     g.close()
 
 
-@pytest.mark.parametrize("lagrangian,precision", [(1, "fp64"), (1, "mixed"), (0, "fp64"), (0, "mixed")])
-def test_steps_split_over_calls(mega, lagrangian, precision):
+@pytest.mark.parametrize("path,precision", [(p, q) for p in ("lagrangian", "semi", "peridynamic") for q in ("fp64", "mixed")])
+def test_steps_split_over_calls(mega, path, precision):
     """Within one mega_step call the next predictor is fused into the force
-    kernel; the last step of a call leaves it to the next call's own predictor
-    launch.  12 steps as 12, as 5 + 7 and as 1 + 4 + 7 give the same state bit
-    for bit, and the same energies."""
-    prob = small_problem(lagrangian=lagrangian)
+    kernel, and values no kernel of the step reads (fint, fext, a, the
+    Lagrangian x) are stored on the call's last steps only; the last step of a
+    call leaves the predictor to the next call's own launch.  12 steps as 12,
+    as 5 + 7, as 1 + 4 + 7 and as twelve single steps (every value stored
+    every step) give the same state bit for bit, and the same energies."""
+    pd = path == "peridynamic"
+    prob = small_problem(lagrangian=int(path != "semi"), peridynamics=int(pd), mm_units=pd)
     prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
+    fields = ("x", "v", "a", "d", "dtot", "uphy", "fint", "fext", "stress", "strain", "eps", "aphy", "vphy_step")
     out = []
-    for split in ((12,), (5, 7), (1, 4, 7)):
+    for split in ((12,), (5, 7), (1, 4, 7), (1,) * 12):
         g = mega.Mega(prob, precision=prec)
         for k in split:
             g.step(k)
-        out.append(({k: g.download(k) for k in ("x", "v", "uphy", "stress", "eps", "fint")}, g.energies()))
+        out.append(({k: g.download(k) for k in fields}, g.energies()))
         g.close()
     for st, en in out[1:]:
         assert all(np.array_equal(st[k], out[0][0][k]) for k in st), [k for k in st if not np.array_equal(st[k], out[0][0][k])]
