@@ -111,10 +111,10 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         return n * ((21 if mixed else 18) * r + 4 * 6 * r + 2 * r + 8 + 4 + 10 * r + (24 if mixed else 8))
     if name == "k_force":
         # pairs: source + grad phi (transposed); nodes: P read once, mass, ebc, mat id,
-        # v r/w, d write, dtot r/w (+FP32 copies of d, dtot in mixed).  fint, fext and a
+        # v r/w, d write, dtot r/w (+ the FP32 copy of d in mixed).  fint, fext and a
         # are stored only on a call's last steps (the next predictor is fused, so
         # nothing in the step reads them)
-        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 12 + 4 + 48 + 24 + 48 + (24 if mixed else 0))
+        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 12 + 4 + 48 + 24 + 48 + (12 if mixed else 0))
     if name == "k_predict":
         return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
     if name == "k_search_count":

@@ -427,10 +427,9 @@ __device__ __forceinline__ void predict_dof(const Params &p, int64_t q)
         p.uphy[q] = u;
         p.x[q] = p.X0[q] + u;
     }
-    if (p.d4) {
+    if (p.d4) {   // t4 is only read by F_DIRECT launches, which sync_d4 precedes
         const int64_t i = q / 3, k = q - 3 * i;
         reinterpret_cast<float *>(p.d4)[4 * i + k] = (float)d;
-        reinterpret_cast<float *>(p.t4)[4 * i + k] = (float)t;
     }
 }
 
@@ -1360,7 +1359,9 @@ __device__ __forceinline__ double node_core(const Params &p, int i, bool valid, 
             if (p.finite_strain) work += 0.5 * dd[k] * (2.0 * sig[k] - si[k]) * vol * det;
             else work += 0.5 * ei[k] * si[k] * vol * det;
         }
-        if (p.wave_vel) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
+        // GMAX_WVEL: output only on the RK path (its time-step estimate is commented
+        // out, FINT.F90:1250-1326), so stored with the step's other outputs
+        if (p.wave_vel && p.out_dof) p.wave_vel[i] = mega::wave_speed(si, ei, mt);
         // P = V J sigma IF^T  (f_j = P grad phi_ij)
         const double Sg[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
         const double w = vol * det;
@@ -1498,10 +1499,9 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
             p.uphy[q] = u;
             p.x[q] = p.X0[q] + u;
         }
-        if (p.d4) {
+        if (p.d4) {   // the FP32 copy of d for the next gather (t4: see predict_dof)
             const int64_t i = q / 3, k = q - 3 * i;
             reinterpret_cast<float *>(p.d4)[4 * i + k] = (float)d;
-            reinterpret_cast<float *>(p.t4)[4 * i + k] = (float)t;
         }
     } else {
         p.v[q] = v;
@@ -2879,7 +2879,9 @@ struct mega_ctx {
     double *fint = nullptr, *fext = nullptr, *fext_nmo = nullptr, *win = nullptr, *vol = nullptr, *mass = nullptr;
     double *winv = nullptr;   // 1 / win, for the box-support kernel
     double *wave_vel = nullptr, *scratch = nullptr;   // scratch: 6n doubles
-    float4 *d4 = nullptr, *t4 = nullptr;             // FP32 gather copies of d, dtot (mixed)
+    float4 *d4 = nullptr, *t4 = nullptr;             // FP32 gather copies of d, dtot (mixed); d4 kept
+                                                     // by every predictor, t4 by sync_d4 before the
+                                                     // F_DIRECT launches, its only readers
     float4 *u4 = nullptr;                            // FP32 x - X0 (peridynamics, mixed)
     double *Fst = nullptr;                           // deformation gradient per node
     int F_valid = 0;                                 // Fst consistent with dtot

@@ -327,7 +327,8 @@ def test_steps_split_over_calls(mega, path, precision):
     pd = path == "peridynamic"
     prob = small_problem(lagrangian=int(path != "semi"), peridynamics=int(pd), mm_units=pd)
     prec = mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED
-    fields = ("x", "v", "a", "d", "dtot", "uphy", "fint", "fext", "stress", "strain", "eps", "aphy", "vphy_step")
+    fields = ("x", "v", "a", "d", "dtot", "uphy", "fint", "fext", "stress", "strain", "eps", "aphy", "vphy_step") + \
+        (("wave_vel",) if precision == "fp64" or pd else ())
     out = []
     for split in ((12,), (5, 7), (1, 4, 7), (1,) * 12):
         g = mega.Mega(prob, precision=prec)
