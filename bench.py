@@ -102,21 +102,26 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
     r: bytes of the shape/stress type; node masters are FP64."""
     r = 4 if mixed else 8
     if name == "k_gather":
-        # pairs: col + phi + grad phi; nodes: own d (FP64) read, uphy r/w, gather sums
-        # write (18 or 21 values), plus dtot own for F.  x = X0 + u is stored only on
-        # a call's last step (nothing in the step reads it)
-        return nnz * (4 + r + 3 * r) + n * (24 + 48 + (21 if mixed else 18) * r + (12 if mixed else 0))
+        # pairs: col + phi + grad phi; nodes: own d (FP64) read, uphy r/w, the gather
+        # sums written as the steps carry F (L0 and, mixed, S: 12 floats; FP64 L0
+        # padded to 10 doubles).  x = X0 + u is stored only on a call's last step
+        # (nothing in the step reads it)
+        return nnz * (4 + r + 3 * r) + n * (24 + 48 + (12 if mixed else 10) * r)
     if name == "k_node":
-        # sums read, stress/strain r/w, eps r/w, vol, mat id, P write, (d, dtot own in mixed)
-        return n * ((21 if mixed else 18) * r + 4 * 6 * r + 2 * r + 8 + 4 + 10 * r + (24 if mixed else 8))
+        # the gather sums read (12 floats / 10 doubles), stress/strain r/w, eps r/w, vol,
+        # mat id, the force record P written whole (12 floats / 10 doubles), own d
+        # (mixed: S x d_i), the carried deformation gradient r/w (10 doubles)
+        return n * ((12 if mixed else 10) * r + 4 * 6 * r + 2 * r + 8 + 4 + (12 if mixed else 10) * r +
+                    (24 if mixed else 0) + 2 * 80)
     if name == "k_force":
-        # pairs: source + grad phi (transposed); nodes: P read once, mass, ebc, mat id,
-        # v r/w, d write, dtot r/w (+ the FP32 copy of d in mixed).  fint, fext and a
-        # are stored only on a call's last steps (the next predictor is fused, so
-        # nothing in the step reads them)
-        return nnz * (4 + 3 * r) + n * (10 * r + 24 + 12 + 4 + 48 + 24 + 48 + (12 if mixed else 0))
+        # pairs: source + grad phi (transposed); nodes: P read once, mass (one per node),
+        # ebc, mat id, v r/w, d write, dtot r/w (+ the FP32 copy of d in mixed).  fint,
+        # fext and a are stored only on a call's last steps (the next predictor is
+        # fused, so nothing in the step reads them)
+        return nnz * (4 + 3 * r) + n * (10 * r + 8 + 12 + 4 + 48 + 24 + 48 + (12 if mixed else 0))
     if name == "k_predict":
-        return n * (24 + 24 + 24 + 48 + 24 + (24 if mixed else 0))
+        # v and a read, d written, dtot r/w, v written (+ the FP32 copy of d in mixed)
+        return n * (24 + 24 + 24 + 48 + 24 + (12 if mixed else 0))
     if name == "k_search_count":
         # one wave per bin tests the 27 bins' candidates; per node: own x and window
         # (a candidate's are re-reads), its slot in the bin list, the count, the kept
@@ -140,9 +145,11 @@ def kernel_bytes(name: str, n: int, nnz: int, mixed: bool) -> float:
         return n * (18 * r + 72 + 24 * r + 2 * r + 8 + 4 + 10 * r)
     if name == "k_pd_force":
         # pairs: source + influence weight (transposed); nodes: row start, own X0, own T,
-        # g vector, then the corrector's node arrays as k_force
-        return nnz * (4 + r) + n * (4 + 24 + 10 * r + 24 + 24 + 24 + 12 + 4 + 48 + 24 + 24 + 48 +
-                                    (24 if mixed else 0))
+        # g vector, then the corrector's node arrays as k_force (mass, ebc, mat id, v r/w,
+        # d write, dtot r/w, the FP32 copy of d in mixed) and, since the particles move
+        # with d, uphy r/w and x written
+        return nnz * (4 + r) + n * (4 + 24 + 10 * r + 24 + 8 + 12 + 4 + 48 + 24 + 48 + 48 + 24 +
+                                    (12 if mixed else 0))
     if name == "k_rk":
         # the semi-Lagrangian launch, gather sums fused: pairs: col + tpos read,
         # phi written in list order and {grad phi, source} in transposed order;

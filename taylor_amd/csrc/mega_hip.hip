@@ -231,6 +231,7 @@ struct Params {
     // mega_step they are made on the steps whose values are read (a by the
     // last step's DO_INTERP, the rest by the caller after the call)
     int out_dof, out_x;
+    int mass_node;                                   // mass[n]: one value per node (all three equal)
     // correspondence peridynamics
     int pd, pd_linit;
     double *Kinv;                                    // 9 per node, inverse shape tensor
@@ -1443,7 +1444,7 @@ __device__ __forceinline__ void load_P(const R *Pb, int i, R *P)
 // energy terms (MAIN.F90:524-553) and, fused, the next step's predictor
 // (the arithmetic of predict_dof, without reading back v, a and dtot).
 struct DofIn {
-    double v, mass, damp, fxo, d, dtot;
+    double v, mass, damp, fxo, dtot;
     int ebc;
 };
 
@@ -1452,10 +1453,9 @@ __device__ __forceinline__ DofIn dof_in(const Params &p, int node, int64_t q)
     DofIn r;
     r.v = p.v[q];
     r.ebc = p.ebc[q];
-    r.mass = p.mass[q];
+    r.mass = p.mass[p.mass_node ? node : q];
     r.damp = p.mats[p.mat[node]].damping;
     r.fxo = p.has_gravity ? p.fext_nmo[q] : 0.0;
-    r.d = p.d[q];
     r.dtot = p.dtot[q];
     return r;
 }
@@ -1487,7 +1487,10 @@ __device__ __forceinline__ void dof_out(const Params &p, int64_t q, const DofIn 
     if (p.has_gravity) p.fext_nmo[q] = fx;
     ke = 0.5 * in.mass * v * v;
     if (!p.finite_strain) ie = 0.5 * f * in.dtot;
-    ee = 0.5 * (fx + in.fxo) * in.d;
+    // external work of the increment: d is read only where a force acts (a
+    // zero force contributes +-0, which leaves the block's sum unchanged)
+    const double fe = fx + in.fxo;
+    ee = fe != 0.0 ? 0.5 * fe * p.d[q] : 0.0;
     if (PREDICT) {
         const double d = predicted_d(dt, v, acc);
         const double t = in.dtot + d;
@@ -2936,6 +2939,7 @@ struct mega_ctx {
     int64_t dt_hist_cap = 0;
     int has_gravity = 0;
     int out_dof = 1, out_x = 1;   // see Params
+    int mass_node = 0;            // mass held per node (the caller's three per node are equal)
     std::vector<Material> host_mats;
     // learned return map
     int surr_on = 0, surr_mat = -1;
@@ -3017,7 +3021,7 @@ Params params(mega_ctx *c)
     p.hv = c->hv_active ? c->hv : nullptr; p.hv_count = c->hv_count; p.hv_cap = c->hv_cap;
     p.pd = c->pd; p.pd_linit = 0;
     p.Kinv = c->Kinv; p.gvec = c->gvec;
-    p.out_dof = c->out_dof; p.out_x = c->out_x;
+    p.out_dof = c->out_dof; p.out_x = c->out_x; p.mass_node = c->mass_node;
     return p;
 }
 
@@ -3789,7 +3793,11 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     c->snni = ctrl->itype_int == 1;
     if (c->snni) { e |= dalloc(&c->sml, n3); e |= dalloc(&c->aov, n3); }
     if (c->pd) { e |= dalloc(&c->Kinv, 9 * (size_t)n); e |= dalloc(&c->gvec, 3 * (size_t)n); }
-    e |= dalloc(&c->win, n3); e |= dalloc(&c->winv, n3); e |= dalloc(&c->vol, n); e |= dalloc(&c->mass, n3);
+    c->mass_node = 1;   // LOCAL_MASS repeats each node's mass over its three dofs (STATE_INIT.f90:111-113)
+    for (int i = 0; i < n && c->mass_node; ++i)
+        c->mass_node = mass[3 * (int64_t)i] == mass[3 * (int64_t)i + 1] && mass[3 * (int64_t)i] == mass[3 * (int64_t)i + 2];
+    e |= dalloc(&c->win, n3); e |= dalloc(&c->winv, n3); e |= dalloc(&c->vol, n);
+    e |= dalloc(&c->mass, c->mass_node ? (size_t)n : n3);
     e |= dalloc(&c->ebc, n3); e |= dalloc(&c->ebc_node, n); e |= dalloc(&c->mat, n);
     e |= dalloc(&c->mats, c->host_mats.size());
     e |= dalloc(&c->stress, n6, r); e |= dalloc(&c->strain, n6, r); e |= dalloc(&c->eps, n, r);
@@ -3821,13 +3829,15 @@ int mega_create(const mega_control *ctrl, int32_t n, const double *X0, const dou
     };
     auto zero = [&](void *dst, size_t bytes) { return hipMemset(dst, 0, bytes) == hipSuccess; };
     const auto X0i = to_internal(c, X0, 3), wini = to_internal(c, win, 3), voli = to_internal(c, vol, 1);
-    const auto massi = to_internal(c, mass, 3);
+    auto massi = to_internal(c, mass, 3);
+    if (c->mass_node)
+        for (int i = 0; i < n; ++i) massi[i] = massi[3 * (size_t)i];
     const auto ebci = to_internal(c, ebc, 3), ebni = to_internal(c, ebc_node, 1), midi = to_internal(c, mid.data(), 1);
     std::vector<double> winvi(n3);
     for (size_t k = 0; k < n3; ++k) winvi[k] = 1.0 / wini[k];
     bool ok = up(c->X0, X0i.data(), n3 * 8) && up(c->x, X0i.data(), n3 * 8) && up(c->win, wini.data(), n3 * 8) &&
               up(c->winv, winvi.data(), n3 * 8) &&
-              up(c->vol, voli.data(), (size_t)n * 8) && up(c->mass, massi.data(), n3 * 8) &&
+              up(c->vol, voli.data(), (size_t)n * 8) && up(c->mass, massi.data(), (c->mass_node ? (size_t)n : n3) * 8) &&
               up(c->ebc, ebci.data(), n3 * 4) && up(c->ebc_node, ebni.data(), (size_t)n * 4) &&
               up(c->mat, midi.data(), (size_t)n * 4) && up(c->orig, c->perm.data(), (size_t)n * 4) &&
               up(c->mats, c->host_mats.data(), c->host_mats.size() * sizeof(Material));
