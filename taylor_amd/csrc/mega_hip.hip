@@ -2453,6 +2453,14 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
             double cx[kCandPerLane], cy[kCandPerLane], cz[kCandPerLane];
             double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
             float pf[kCandPerLane][3], lo[kCandPerLane][3], hi[kCandPerLane][3];   // box prefilter
+            // FILL: this pass's kept masks of own node t0 + lane, loaded together before
+            // the walk over the own nodes reads them lane by lane
+            uint64_t mym[kCandPerLane];
+#pragma unroll
+            for (int r = 0; r < kCandPerLane; ++r) {
+                const int c64 = (base >> 6) + r;
+                mym[r] = (FILL && lane < tn && c64 < kMaskCap) ? masks[(int64_t)myi * kMaskCap + c64] : 0ull;
+            }
 #pragma unroll
             for (int r = 0; r < kCandPerLane; ++r) {
                 const int c = base + 64 * r + lane;
@@ -2507,7 +2515,8 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                     bool add = false;
                     unsigned long long mask;
                     if (FILL && c64 < kMaskCap) {
-                        mask = *mslot;
+                        mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mym[r] >> 32), t) << 32) |
+                               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mym[r], t);
                         add = (mask >> lane) & 1ull;
                     } else {
                         if (!PRE) {
