@@ -2446,6 +2446,9 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 }
             }
         }
+        // FILL: the lane's candidates only move forward, so their bins' scan resumes
+        // where it stopped (COUNT keeps independent scans, which overlap better there)
+        int cell_at = 0;
         for (int base = 0; base < total; base += 64 * kCandPerLane) {
             // does this chunk test candidates here (COUNT, or beyond the kept masks)?
             const bool tests = !FILL || (base >> 6) + kCandPerLane > kMaskCap;
@@ -2464,10 +2467,11 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
 #pragma unroll
             for (int r = 0; r < kCandPerLane; ++r) {
                 const int c = base + 64 * r + lane;
-                int j = -1, cell = 0;
+                int j = -1, cell = FILL ? cell_at : 0;
                 if (c < total) {
                     while (cell < 26 && s_off[wv][cell + 1] <= c) ++cell;
                     j = cell_nodes[s_beg[wv][cell] + (c - s_off[wv][cell])];
+                    if (FILL) cell_at = cell;
                 }
                 cj_[r] = j;
                 const int jj = j < 0 ? 0 : j;
