@@ -2452,7 +2452,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
         for (int base = 0; base < total; base += 64 * kCandPerLane) {
             // does this chunk test candidates here (COUNT, or beyond the kept masks)?
             const bool tests = !FILL || (base >> 6) + kCandPerLane > kMaskCap;
-            int cj_[kCandPerLane], tix[kCandPerLane], kc[kCandPerLane];
+            int cj_[kCandPerLane], tix[kCandPerLane], kc[kCandPerLane], kc0[kCandPerLane];
             double cx[kCandPerLane], cy[kCandPerLane], cz[kCandPerLane];
             double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
             float pf[kCandPerLane][3], lo[kCandPerLane][3], hi[kCandPerLane][3];   // box prefilter
@@ -2479,6 +2479,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 // this bin's running count (COUNT) or next slot (FILL) in j's transposed row;
                 // an earlier chunk of own nodes left it in tbin, written by this same lane
                 kc[r] = (j >= 0 && (FILL || t0 > 0)) ? tbin[tix[r]] : 0;
+                kc0[r] = kc[r];
                 cx[r] = cy[r] = cz[r] = 0.0;
                 wx[r] = wy[r] = wz[r] = 1.0;
                 if (!PRE) {
@@ -2564,9 +2565,12 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 }
                 if (lane == t) mine = out;
             }
+            // COUNT stores a counter only where this bin's nodes kept the candidate (tbin
+            // was cleared, and an earlier chunk's value is already there); FILL stores
+            // its cursors for a later chunk of own nodes
 #pragma unroll
             for (int r = 0; r < kCandPerLane; ++r)
-                if (cj_[r] >= 0 && (!FILL || more)) tbin[tix[r]] = kc[r];
+                if (cj_[r] >= 0 && (FILL ? more : kc[r] != kc0[r])) tbin[tix[r]] = kc[r];
         }
         if (!FILL && lane < tn) counts[myi] = mine;
     }
