@@ -2457,7 +2457,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
             double wx[kCandPerLane], wy[kCandPerLane], wz[kCandPerLane];
             float pf[kCandPerLane][3], lo[kCandPerLane][3], hi[kCandPerLane][3];   // box prefilter
             // FILL: this pass's kept masks of own node t0 + lane, loaded together before
-            // the walk over the own nodes reads them lane by lane
+            // the walk over the own nodes reads them lane by lane; COUNT: collected there
             uint64_t mym[kCandPerLane];
 #pragma unroll
             for (int r = 0; r < kCandPerLane; ++r) {
@@ -2516,7 +2516,6 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                 for (int r = 0; r < kCandPerLane; ++r) {
                     if (base + 64 * r >= total) break;   // uniform
                     const int c64 = (base >> 6) + r;
-                    uint64_t *mslot = masks + (int64_t)i * kMaskCap + c64;
                     bool add = false;
                     unsigned long long mask;
                     if (FILL && c64 < kMaskCap) {
@@ -2551,7 +2550,7 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                             }
                         }
                         mask = __ballot(add);
-                        if (!FILL && c64 < kMaskCap && lane == 0) *mslot = mask;
+                        if (!FILL && lane == t) mym[r] = mask;   // own node t's lane keeps it
                     }
                     if (FILL && add) {
                         const int q = out + __popcll(mask & ((1ull << lane) - 1ull));
@@ -2564,6 +2563,14 @@ __global__ __launch_bounds__(kBlock) void k_search(int ncells, const int32_t *ce
                     out += __popcll(mask);
                 }
                 if (lane == t) mine = out;
+            }
+            // COUNT: each lane stores its own node's masks of this pass together
+            if (!FILL && lane < tn) {
+#pragma unroll
+                for (int r = 0; r < kCandPerLane; ++r) {
+                    const int c64 = (base >> 6) + r;
+                    if (c64 < kMaskCap && base + 64 * r < total) masks[(int64_t)myi * kMaskCap + c64] = mym[r];
+                }
             }
             // COUNT stores a counter only where this bin's nodes kept the candidate (tbin
             // was cleared, and an earlier chunk's value is already there); FILL stores
