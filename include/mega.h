@@ -169,7 +169,10 @@ int mega_preprocess_noalloc(int32_t nump, const double *coo, int32_t numel, cons
 
 /* nsteps explicit steps of size dt, device resident: predictor, DO_INTERP,
  * coordinate update, (search + shapes when semi-Lagrangian), CONSTRUCT_FINT,
- * EBC and corrector.  MAIN.F90:304-658. */
+ * EBC and corrector.  MAIN.F90:304-658.  After the call every field
+ * mega_download returns is that of the last step; within the call, values no
+ * step reads (internal and external force, acceleration, Lagrangian
+ * coordinates, the RK wave speed) are stored on the last steps only. */
 int mega_step(mega_ctx *ctx, double dt, int32_t nsteps);
 
 /* nsteps explicit steps with the automatic time step (IAUTO_TS = 1,
