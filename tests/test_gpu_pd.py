@@ -224,7 +224,8 @@ def test_pd_damage_mixed_precision_steps(mat_type):
         assert errs[k] < bar, (k, errs[k], bar)
 
 
-def test_pd_drucker_prager_automatic_time_step_matches_reference_program():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pd_drucker_prager_automatic_time_step_matches_reference_program(precision):
     """IAUTO_TS = 1 with material type 3, whose wave speed the reference divides
     by 0.15 (FINT_PD.F90:539-542): every step's DLT and TIME and frames 1, 10,
     40 against the reference program (tests/golden/run_pd_dp3_auto.npz)."""
@@ -233,8 +234,8 @@ def test_pd_drucker_prager_automatic_time_step_matches_reference_program():
     fac = float(a["run_dlt_fac"][0])
     ref_dlt = np.diff(np.concatenate([[0.0], a["run_time"]]))
     prob = small(mm_units=True, material=damage_material(3))
-    m = mega.Mega(prob)
-    done, dlt = 0, []
+    m = mega.Mega(prob, precision=mega.MEGA_FP64 if precision == "fp64" else mega.MEGA_MIXED)
+    done, dlt, worst = 0, [], {}
     for k in (1, 10, 40):
         st, d = m.step_auto(fac, k - done)
         assert st == 0
@@ -244,8 +245,12 @@ def test_pd_drucker_prager_automatic_time_step_matches_reference_program():
                 "fint": rel_err(m.download("fint"), a[f"step{k}_fint"]),
                 "stress": rel_err(m.download("stress"), a[f"step{k}_stress"]),
                 "dam": float(np.abs(m.download("damage_state")[:, 2] - a[f"step{k}_dam"]).max())}
-        print(f"step {k}: " + " ".join(f"{q} {e:.2e}" for q, e in errs.items()), flush=True)
-        assert errs["x"] < 1e-12 and errs["fint"] < 1e-9 and errs["stress"] < 1e-9 and errs["dam"] < 1e-9
+        for q, e in errs.items():
+            worst[q] = max(worst.get(q, 0.0), e)
+        print(f"{precision} step {k}: " + " ".join(f"{q} {e:.2e}" for q, e in errs.items()), flush=True)
+        if precision == "mixed" and k <= 10:
+            # measured at step 10: x 2.5e-10, fint 3.6e-7, stress 4.9e-7, damage 1.2e-8
+            assert errs["x"] < 3e-9 and errs["fint"] < 4e-6 and errs["stress"] < 5e-6 and errs["dam"] < 1.2e-7
     m.close()
     t, tt = [], 0.0
     for d in dlt:
@@ -253,5 +258,18 @@ def test_pd_drucker_prager_automatic_time_step_matches_reference_program():
         t.append(tt)
     e_dlt = float(np.abs(np.array(dlt) - ref_dlt).max() / ref_dlt.max())
     e_t = float(np.abs(np.array(t) - a["run_time"]).max() / a["run_time"].max())
-    print(f"DLT {e_dlt:.2e}, TIME {e_t:.2e}", flush=True)
-    assert e_dlt < 1e-14 and e_t < 1e-14
+    print(f"{precision}: DLT {e_dlt:.2e}, TIME {e_t:.2e}", flush=True)
+    rel = np.abs(np.array(dlt) - ref_dlt) / ref_dlt
+    first = int(np.argmax(rel > 1e-3)) if (rel > 1e-3).any() else -1
+    print(f"{precision}: per-step DLT / reference " + " ".join(f"{v:.6f}" for v in (np.array(dlt) / ref_dlt)) +
+          f"; first step off by more than 1e-3: {first + 1 if first >= 0 else None}", flush=True)
+    if precision == "fp64":
+        assert e_dlt < 1e-14 and e_t < 1e-14
+        assert worst["x"] < 1e-12 and worst["fint"] < 1e-9 and worst["stress"] < 1e-9 and worst["dam"] < 1e-9
+    else:
+        # the first 14 steps' DLT equal the program's to 6 digits; from step 15 the
+        # particle that sets DLT differs (ratios down to 0.71 ~ 1/sqrt(2), as between
+        # an elastic and a halved secant modulus in ESTIMATE_MODULI), and the runs part
+        assert np.abs(rel[:14]).max() < 1e-5
+        ratio = np.array(dlt) / ref_dlt
+        assert np.isfinite(ratio).all() and ratio.min() > 0.6 and ratio.max() < 1.6
